@@ -1,0 +1,1182 @@
+/*
+ * fr_oracle.c -- CPU restatement of the HiFiLES FR residual + RK update.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  The product never links this file.
+ *
+ * Every function cites the reference lines it restates.  Floating-point
+ * operations are written in the same order as the reference CPU build with
+ * -D_NO_BLAS (src/global.cpp:38-103 dgemm), so that, compiled with
+ * -ffp-contract=off, the oracle reproduces the reference binary to the last bit
+ * on the same mesh and state.
+ *
+ * Parity pins (what checks each function; see tests/test_oracle_*.py):
+ *   gauss table, eval_lagrange/eval_d_lagrange, opp_0/opp_4  -- polynomial
+ *       exactness (interpolation/differentiation of degree-P polynomials),
+ *       Gauss nodes = roots of P_{n} to 1e-15, and reference residuals.
+ *   eval_d_vcjh_1d / compute_eta / opp_3 -- DG limit (eta=0) equals the
+ *       derivative of the right Radau polynomial; exactness of the corrected
+ *       divergence for polynomial fluxes; reference residuals (SD scheme).
+ *   set_transforms metrics -- closed forms on affine elements; reference residuals.
+ *   fluxes (Euler, NS viscous, Rusanov, LDG) -- free-stream preservation,
+ *       conservation, reference residuals from oracle/_ref on seeded states.
+ *   RK45 -- exactness for polynomial-in-time ODEs (order 4), reference
+ *       RK45 trajectories and the TGV 25-step residual norms of the reference's
+ *       own regression test (testcases/regression_tests.py:300-307).
+ */
+#include "fr_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* 1-D Gauss-Legendre points as tabulated (15 decimals) by the reference,
+ * data/loc_1d_gauss_pts.dat:1-60 (orders 0..6).  The tabulated values are used
+ * verbatim (not recomputed) because operator parity at 1e-15 depends on them. */
+static const double gauss_tab[7][7] = {
+  {0.0},
+  {-0.577350269189626, 0.577350269189626},
+  {-0.774596669241483, -0.000000000000000, 0.774596669241483},
+  {-0.861136311594053, -0.339981043584856, 0.339981043584856, 0.861136311594053},
+  {-0.906179845938664, -0.538469310105683, -0.000000000000000, 0.538469310105683,
+   0.906179845938664},
+  {-0.932469514203152, -0.661209386466264, -0.238619186083197, 0.238619186083197,
+   0.661209386466265, 0.932469514203152},
+  {-0.949107912342758, -0.741531185599395, -0.405845151377397, 0.000000000000000,
+   0.405845151377397, 0.741531185599394, 0.949107912342758}};
+
+
+/* ------------------------------------------------------------------------- */
+/* 1-D basis functions, src/funcs.cpp:57-250 */
+
+/* src/funcs.cpp:57-73 */
+static double eval_lagrange(double r, int mode, const double* pts, int n) {
+  double t = 1.0;
+  for (int i = 0; i < n; i++)
+    if (i != mode) t = t * ((r - pts[i]) / (pts[mode] - pts[i]));
+  return t;
+}
+
+/* src/funcs.cpp:78-105 */
+static double eval_d_lagrange(double r, int mode, const double* pts, int n) {
+  double t0 = 0.0;
+  for (int i = 0; i < n; i++) {
+    if (i == mode) continue;
+    double t1 = 1.0, t2 = 1.0;
+    for (int j = 0; j < n; j++) {
+      if (j != mode && j != i) t1 = t1 * (r - pts[j]);
+      if (j != mode) t2 = t2 * (pts[mode] - pts[j]);
+    }
+    t0 = t0 + (t1 / t2);
+  }
+  return t0;
+}
+
+/* src/funcs.cpp:161-179 */
+static double eval_legendre(double r, int mode) {
+  if (mode == 0) return 1.0;
+  if (mode == 1) return r;
+  return ((2 * mode - 1) * r * eval_legendre(r, mode - 1) -
+          (mode - 1) * eval_legendre(r, mode - 2)) / mode;
+}
+
+/* src/funcs.cpp:183-212 */
+static double eval_d_legendre(double r, int mode) {
+  double d = 0.0;
+  if (mode == 0) return 0;
+  if (r > -1.0 && r < 1.0) {
+    d = (mode * ((r * eval_legendre(r, mode)) - eval_legendre(r, mode - 1))) / ((r * r) - 1.0);
+  } else {
+    if (r == -1.0) d = pow(-1.0, mode - 1.0) * 0.5 * mode * (mode + 1.0);
+    if (r == 1.0) d = 0.5 * mode * (mode + 1.0);
+  }
+  return d;
+}
+
+/* src/funcs.cpp:216-250; mode 0 = left correction function, 1 = right */
+static double eval_d_vcjh_1d(double r, int mode, int order, double eta) {
+  double d = 0.0;
+  if (mode == 0) {
+    if (order == 0)
+      d = 0.5 * pow(-1.0, order) *
+          (eval_d_legendre(r, order) - ((eval_d_legendre(r, order + 1)) / (1.0 + eta)));
+    else
+      d = 0.5 * pow(-1.0, order) *
+          (eval_d_legendre(r, order) -
+           (((eta * eval_d_legendre(r, order - 1)) + eval_d_legendre(r, order + 1)) / (1.0 + eta)));
+  } else {
+    if (order == 0)
+      d = 0.5 * (eval_d_legendre(r, order) + ((eval_d_legendre(r, order + 1)) / (1.0 + eta)));
+    else
+      d = 0.5 * (eval_d_legendre(r, order) +
+                 (((eta * eval_d_legendre(r, order - 1)) + eval_d_legendre(r, order + 1)) /
+                  (1.0 + eta)));
+  }
+  return d;
+}
+
+/* src/funcs.cpp:1503-1546 (schemes 1..3; scheme 4 (c+) is not restated) */
+static double compute_eta(int scheme, int order) {
+  if (scheme == 1) return 0.0;
+  if (scheme == 2) return (1.0 * (order)) / (1.0 * (order + 1));
+  if (scheme == 3) return (1.0 * (order + 1)) / (1.0 * order);
+  fprintf(stderr, "fr_oracle: vcjh scheme %d not restated\n", scheme);
+  abort();
+}
+
+/* src/global.cpp:38-103, the no-BLAS dgemm: C = alpha*A*B + beta*C, column-major,
+ * accumulation order l = 0..Acols-1 per column.  Columns are independent, so the
+ * OpenMP split over j is bit-identical to the serial loop. */
+static void dgemm_ref(int Arows, int Bcols, int Acols, double alpha, double beta,
+                      const double* a, const double* b, double* c) {
+  if (Arows == 0 || Bcols == 0 || ((alpha == 0. || Acols == 0) && beta == 1.)) return;
+#pragma omp parallel for schedule(static)
+  for (int j = 0; j < Bcols; j++) {
+    double* cj = c + (long)j * Arows;
+    const double* bj = b + (long)j * Acols;
+    if (beta == 0.) {
+      for (int i = 0; i < Arows; i++) cj[i] = 0.;
+    } else if (beta != 1.) {
+      for (int i = 0; i < Arows; i++) cj[i] = beta * cj[i];
+    }
+    for (int l = 0; l < Acols; l++) {
+      double temp = alpha * bj[l];
+      const double* al = a + (long)l * Arows;
+      for (int i = 0; i < Arows; i++) cj[i] += temp * al[i];
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Solver state */
+
+struct or_solver {
+  or_deck deck;
+  or_phys ph;
+  int ele_type, order, nd, nf, nu, nfp, nfaces, nfpf, nsp, n_eles, n_verts;
+  double loc1d[8];
+  double *loc_upts;   /* (d,Nu) */
+  double *tloc_fpts;  /* (d,Nfp) */
+  double *tnorm_fpts; /* (d,Nfp) */
+  double *opp_0, *opp_1[3], *opp_2[3], *opp_3, *opp_4[3], *opp_5[3], *opp_6;
+  double *shape;      /* (d,nsp,N) */
+  double *detjac_upts, *JGinv_upts, *tdA_fpts, *norm_fpts, *pos_upts, *pos_fpts;
+  double *disu_upts[2], *div_tconf_upts, *disu_fpts, *tdisf_upts, *norm_tdisf_fpts,
+      *norm_tconf_fpts, *delta_disu_fpts, *grad_disu_upts, *grad_disu_fpts, *src_upts;
+  int n_inters;
+  int *face_l, *face_r; /* (n_inters*nfpf) flux-point indices fpt + Nfp*ele */
+};
+
+static void* xcalloc(size_t n, size_t sz) {
+  void* p = calloc(n ? n : 1, sz);
+  if (!p) { fprintf(stderr, "fr_oracle: out of memory\n"); abort(); }
+  return p;
+}
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
+/* src/input.cpp:89-331 defaults */
+void or_deck_defaults(or_deck* d) {
+  memset(d, 0, sizeof(*d));
+  d->equation = 0; d->viscous = 0; d->riemann_solve_type = 0; d->vis_riemann_solve_type = 0;
+  d->ic_form = 1; d->order = 3; d->adv_type = 3; d->dt = 1e-3;
+  d->tau = 0.; d->pen_fact = 0.5;
+  d->gamma = 1.4; d->prandtl = .72; d->S_gas = 120.; d->T_gas = 291.15; d->R_gas = 286.9;
+  d->mu_gas = 1.827E-5; d->fix_vis = 1.;
+  d->Mach_free_stream = 0.1; d->nx_free_stream = 1.; d->ny_free_stream = 0.; d->nz_free_stream = 0.;
+  d->Re_free_stream = 1600.; d->L_free_stream = 1.; d->T_free_stream = 300.; d->T_wall = 300.;
+  d->Mach_wall = 0.; d->nx_wall = 0.; d->ny_wall = 0.; d->nz_wall = 0.;
+  d->upts_type = 0; d->vcjh_scheme = 0; d->eta = 0.;
+  d->LES = 0; d->SGS_model = 0; d->filter_ratio = 1.; d->const_src = 0.;
+  d->dx_cyclic = INFINITY; d->dy_cyclic = INFINITY; d->dz_cyclic = INFINITY;
+}
+
+/* src/input.cpp:345-515 (the quantities the residual uses) */
+void or_setup_params(const or_deck* d, or_phys* o) {
+  memset(o, 0, sizeof(*o));
+  o->gamma = d->gamma; o->prandtl = d->prandtl; o->fix_vis = d->fix_vis;
+  o->tau = d->tau; o->pen_fact = d->pen_fact; o->const_src = d->const_src; o->dt = d->dt;
+  o->T_wall = d->T_wall;
+  if (d->viscous) {
+    if (d->ic_form == 0) {   /* src/input.cpp:392-399 */
+      o->fix_vis = 1.; o->R_ref = 1.; o->c_sth = 1.; o->rt_inf = 1.; o->mu_inf = 0.1;
+    } else {
+      double T_ref = d->T_free_stream, L_ref = d->L_free_stream;
+      double uvw_ref = d->Mach_free_stream * sqrt(d->gamma * d->R_gas * d->T_free_stream);
+      double mu_free_stream;
+      if (d->fix_vis) mu_free_stream = d->mu_gas;
+      else mu_free_stream = d->mu_gas * pow(d->T_free_stream / d->T_gas, 1.5) *
+                            ((d->T_gas + d->S_gas) / (d->T_free_stream + d->S_gas));
+      double rho_free_stream = (mu_free_stream * d->Re_free_stream) / (uvw_ref * d->L_free_stream);
+      double rho_ref = rho_free_stream;
+      double mu_ref = rho_ref * uvw_ref * L_ref;
+      o->R_ref = (d->R_gas * T_ref) / (uvw_ref * uvw_ref);
+      o->c_sth = d->S_gas / d->T_gas;
+      o->mu_inf = d->mu_gas / mu_ref;
+      o->rt_inf = d->T_gas * d->R_gas / (uvw_ref * uvw_ref);
+      double uvw_wall = d->Mach_wall * sqrt(d->gamma * d->R_gas * d->T_wall);
+      o->v_wall[0] = (uvw_wall * d->nx_wall) / uvw_ref;
+      o->v_wall[1] = (uvw_wall * d->ny_wall) / uvw_ref;
+      o->v_wall[2] = (uvw_wall * d->nz_wall) / uvw_ref;
+      o->T_wall = d->T_wall / T_ref;
+      o->prandtl_t = 0.9;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Element-type specific point sets and bases */
+
+static int idx3(int n, int k, int j, int i) { return k + n * j + n * n * i; }
+
+/* src/eles_hexas.cpp:917-928 / src/eles_quads.cpp:855-866 */
+static double nodal_basis(const or_solver* s, int idx, const double* loc) {
+  int n = s->order + 1;
+  if (s->ele_type == 4) {
+    int i = idx / (n * n), j = (idx - n * n * i) / n, k = idx - n * j - n * n * i;
+    return eval_lagrange(loc[0], k, s->loc1d, n) * eval_lagrange(loc[1], j, s->loc1d, n) *
+           eval_lagrange(loc[2], i, s->loc1d, n);
+  } else {
+    int i = idx / n, j = idx - n * i;
+    return eval_lagrange(loc[0], j, s->loc1d, n) * eval_lagrange(loc[1], i, s->loc1d, n);
+  }
+}
+
+/* src/eles_hexas.cpp:945-973 / src/eles_quads.cpp:888-910 */
+static double d_nodal_basis(const or_solver* s, int idx, int c, const double* loc) {
+  int n = s->order + 1;
+  const double* p = s->loc1d;
+  if (s->ele_type == 4) {
+    int i = idx / (n * n), j = (idx - n * n * i) / n, k = idx - n * j - n * n * i;
+    if (c == 0) return eval_d_lagrange(loc[0], k, p, n) * eval_lagrange(loc[1], j, p, n) * eval_lagrange(loc[2], i, p, n);
+    if (c == 1) return eval_lagrange(loc[0], k, p, n) * eval_d_lagrange(loc[1], j, p, n) * eval_lagrange(loc[2], i, p, n);
+    return eval_lagrange(loc[0], k, p, n) * eval_lagrange(loc[1], j, p, n) * eval_d_lagrange(loc[2], i, p, n);
+  } else {
+    int i = idx / n, j = idx - n * i;
+    if (c == 0) return eval_d_lagrange(loc[0], j, p, n) * eval_lagrange(loc[1], i, p, n);
+    return eval_lagrange(loc[0], j, p, n) * eval_d_lagrange(loc[1], i, p, n);
+  }
+}
+
+/* src/eles_hexas.cpp:1167-1205 / src/eles_quads.cpp:1107-1128 (VCJH branch) */
+static double div_vcjh_basis(const or_solver* s, int idx, const double* loc) {
+  int P = s->order, n = P + 1;
+  const double* p = s->loc1d;
+  double eta = s->deck.vcjh_scheme == 0 ? s->deck.eta : compute_eta(s->deck.vcjh_scheme, P);
+  if (s->ele_type == 4) {
+    int i = idx / s->nfpf, j = (idx - s->nfpf * i) / n, k = idx - s->nfpf * i - n * j;
+    switch (i) {
+      case 0: return -eval_lagrange(loc[0], P - k, p, n) * eval_lagrange(loc[1], j, p, n) * eval_d_vcjh_1d(loc[2], 0, P, eta);
+      case 1: return -eval_lagrange(loc[0], k, p, n) * eval_lagrange(loc[2], j, p, n) * eval_d_vcjh_1d(loc[1], 0, P, eta);
+      case 2: return eval_lagrange(loc[1], k, p, n) * eval_lagrange(loc[2], j, p, n) * eval_d_vcjh_1d(loc[0], 1, P, eta);
+      case 3: return eval_lagrange(loc[0], P - k, p, n) * eval_lagrange(loc[2], j, p, n) * eval_d_vcjh_1d(loc[1], 1, P, eta);
+      case 4: return -eval_lagrange(loc[1], P - k, p, n) * eval_lagrange(loc[2], j, p, n) * eval_d_vcjh_1d(loc[0], 0, P, eta);
+      default: return eval_lagrange(loc[0], k, p, n) * eval_lagrange(loc[1], j, p, n) * eval_d_vcjh_1d(loc[2], 1, P, eta);
+    }
+  } else {
+    int i = idx / s->nfpf, j = idx - s->nfpf * i;
+    switch (i) {
+      case 0: return -eval_lagrange(loc[0], j, p, n) * eval_d_vcjh_1d(loc[1], 0, P, eta);
+      case 1: return eval_lagrange(loc[1], j, p, n) * eval_d_vcjh_1d(loc[0], 1, P, eta);
+      case 2: return eval_lagrange(loc[0], P - j, p, n) * eval_d_vcjh_1d(loc[1], 1, P, eta);
+      default: return -eval_lagrange(loc[1], P - j, p, n) * eval_d_vcjh_1d(loc[0], 0, P, eta);
+    }
+  }
+}
+
+/* src/eles_hexas.cpp:240-256, 266-324, 572-625; src/eles_quads.cpp:237-250, 259-300, 443-480 */
+static void set_points(or_solver* s) {
+  int P = s->order, n = P + 1, d = s->nd;
+  const double* g = s->loc1d;
+  s->loc_upts = xcalloc((size_t)d * s->nu, sizeof(double));
+  s->tloc_fpts = xcalloc((size_t)d * s->nfp, sizeof(double));
+  s->tnorm_fpts = xcalloc((size_t)d * s->nfp, sizeof(double));
+  if (s->ele_type == 4) {
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++)
+        for (int k = 0; k < n; k++) {
+          int u = idx3(n, k, j, i);
+          s->loc_upts[0 + 3 * u] = g[k]; s->loc_upts[1 + 3 * u] = g[j]; s->loc_upts[2 + 3 * u] = g[i];
+        }
+    for (int i = 0; i < 6; i++)
+      for (int j = 0; j < n; j++)
+        for (int k = 0; k < n; k++) {
+          int f = k + n * j + n * n * i;
+          double* t = s->tloc_fpts + 3 * f;
+          double* tn = s->tnorm_fpts + 3 * f;
+          tn[0] = tn[1] = tn[2] = 0.0;
+          switch (i) {
+            case 0: t[0] = g[P - k]; t[1] = g[j]; t[2] = -1.0; tn[2] = -1.0; break;
+            case 1: t[0] = g[k]; t[1] = -1.0; t[2] = g[j]; tn[1] = -1.0; break;
+            case 2: t[0] = 1.0; t[1] = g[k]; t[2] = g[j]; tn[0] = 1.0; break;
+            case 3: t[0] = g[P - k]; t[1] = 1.0; t[2] = g[j]; tn[1] = 1.0; break;
+            case 4: t[0] = -1.0; t[1] = g[P - k]; t[2] = g[j]; tn[0] = -1.0; break;
+            default: t[0] = g[k]; t[1] = g[j]; t[2] = 1.0; tn[2] = 1.0; break;
+          }
+        }
+  } else {
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        int u = j + n * i;
+        s->loc_upts[0 + 2 * u] = g[j]; s->loc_upts[1 + 2 * u] = g[i];
+      }
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < n; j++) {
+        int f = j + n * i;
+        double* t = s->tloc_fpts + 2 * f;
+        double* tn = s->tnorm_fpts + 2 * f;
+        tn[0] = tn[1] = 0.0;
+        switch (i) {
+          case 0: t[0] = g[j]; t[1] = -1.0; tn[1] = -1.0; break;
+          case 1: t[0] = 1.0; t[1] = g[j]; tn[0] = 1.0; break;
+          case 2: t[0] = g[P - j]; t[1] = 1.0; tn[1] = 1.0; break;
+          default: t[0] = -1.0; t[1] = g[P - j]; tn[0] = -1.0; break;
+        }
+      }
+  }
+}
+
+/* src/eles.cpp:3620-4052 (set_opp_0 .. set_opp_6, dense forms) */
+static void set_operators(or_solver* s) {
+  int nu = s->nu, nfp = s->nfp, d = s->nd;
+  s->opp_0 = xcalloc((size_t)nfp * nu, sizeof(double));
+  s->opp_3 = xcalloc((size_t)nu * nfp, sizeof(double));
+  for (int i = 0; i < nu; i++)
+    for (int j = 0; j < nfp; j++)
+      s->opp_0[j + nfp * i] = nodal_basis(s, i, s->tloc_fpts + d * j);
+  for (int dim = 0; dim < d; dim++) {
+    s->opp_1[dim] = xcalloc((size_t)nfp * nu, sizeof(double));
+    s->opp_2[dim] = xcalloc((size_t)nu * nu, sizeof(double));
+    s->opp_4[dim] = xcalloc((size_t)nu * nu, sizeof(double));
+    s->opp_5[dim] = xcalloc((size_t)nu * nfp, sizeof(double));
+    for (int j = 0; j < nu; j++)
+      for (int k = 0; k < nfp; k++)
+        s->opp_1[dim][k + nfp * j] = nodal_basis(s, j, s->tloc_fpts + d * k) * s->tnorm_fpts[dim + d * k];
+    for (int j = 0; j < nu; j++)
+      for (int k = 0; k < nu; k++) {
+        double v = d_nodal_basis(s, j, dim, s->loc_upts + d * k);
+        s->opp_2[dim][k + nu * j] = v;
+        s->opp_4[dim][k + nu * j] = v;
+      }
+  }
+  /* fill_opp_3: src/eles_hexas.cpp:1148-1162 */
+  for (int i = 0; i < nfp; i++)
+    for (int j = 0; j < nu; j++)
+      s->opp_3[j + nu * i] = div_vcjh_basis(s, i, s->loc_upts + d * j);
+  for (int dim = 0; dim < d; dim++)
+    for (int j = 0; j < nfp; j++)
+      for (int k = 0; k < nu; k++)
+        s->opp_5[dim][k + nu * j] = s->opp_3[k + nu * j] * s->tnorm_fpts[dim + d * j];
+  s->opp_6 = s->opp_0;
+}
+
+/* Linear shape basis and its derivative, src/eles_hexas.cpp:1000-1020, 1075-1093;
+ * src/eles_quads.cpp:960-975, 1000-1015 (perfect square/cube node counts) */
+static void shape_basis(const or_solver* s, const double* loc, double* N, double* dN /* (nsp,d) */) {
+  double sp[2];
+  for (int i = 0; i < 2; i++) sp[i] = -1.0 + ((2.0 * i) / (1.0 * (2 - 1)));
+  if (s->ele_type == 4) {
+    for (int m = 0; m < 8; m++) {
+      int i = m / 4, j = (m - 4 * i) / 2, k = m - 2 * j - 4 * i;
+      if (N) N[m] = eval_lagrange(loc[0], k, sp, 2) * eval_lagrange(loc[1], j, sp, 2) * eval_lagrange(loc[2], i, sp, 2);
+      if (dN) {
+        dN[m + 8 * 0] = eval_d_lagrange(loc[0], k, sp, 2) * eval_lagrange(loc[1], j, sp, 2) * eval_lagrange(loc[2], i, sp, 2);
+        dN[m + 8 * 1] = eval_lagrange(loc[0], k, sp, 2) * eval_d_lagrange(loc[1], j, sp, 2) * eval_lagrange(loc[2], i, sp, 2);
+        dN[m + 8 * 2] = eval_lagrange(loc[0], k, sp, 2) * eval_lagrange(loc[1], j, sp, 2) * eval_d_lagrange(loc[2], i, sp, 2);
+      }
+    }
+  } else {
+    for (int m = 0; m < 4; m++) {
+      int i = m / 2, j = m - 2 * i;
+      if (N) N[m] = eval_lagrange(loc[0], j, sp, 2) * eval_lagrange(loc[1], i, sp, 2);
+      if (dN) {
+        dN[m + 4 * 0] = eval_d_lagrange(loc[0], j, sp, 2) * eval_lagrange(loc[1], i, sp, 2);
+        dN[m + 4 * 1] = eval_lagrange(loc[0], j, sp, 2) * eval_d_lagrange(loc[1], i, sp, 2);
+      }
+    }
+  }
+}
+
+/* src/eles.cpp:5744-5757 calc_pos */
+static void calc_pos(const or_solver* s, const double* loc, int e, double* pos) {
+  double N[8];
+  shape_basis(s, loc, N, NULL);
+  for (int i = 0; i < s->nd; i++) {
+    pos[i] = 0.0;
+    for (int j = 0; j < s->nsp; j++) pos[i] += N[j] * s->shape[i + s->nd * (j + s->nsp * e)];
+  }
+}
+
+/* src/eles.cpp:5841-5856 calc_d_pos: dpos(j,k) = dx_j/dr_k */
+static void calc_d_pos(const or_solver* s, const double* loc, int e, double dpos[3][3]) {
+  double dN[24];
+  shape_basis(s, loc, NULL, dN);
+  for (int j = 0; j < s->nd; j++)
+    for (int k = 0; k < s->nd; k++) {
+      dpos[j][k] = 0.0;
+      for (int i = 0; i < s->nsp; i++) dpos[j][k] += dN[i + s->nsp * k] * s->shape[j + s->nd * (i + s->nsp * e)];
+    }
+}
+
+#define JG(l, m, j, i) s->JGinv_upts[(l) + nd * ((m) + nd * ((j) + (long)nu * (i)))]
+
+/* src/eles.cpp:4557-4847 set_transforms (static mesh) */
+static void set_transforms(or_solver* s) {
+  int nd = s->nd, nu = s->nu, nfp = s->nfp, N = s->n_eles;
+  s->detjac_upts = xcalloc((size_t)nu * N, sizeof(double));
+  s->JGinv_upts = xcalloc((size_t)nd * nd * nu * N, sizeof(double));
+  s->pos_upts = xcalloc((size_t)nu * N * nd, sizeof(double));
+  s->tdA_fpts = xcalloc((size_t)nfp * N, sizeof(double));
+  s->norm_fpts = xcalloc((size_t)nfp * N * nd, sizeof(double));
+  s->pos_fpts = xcalloc((size_t)nfp * N * nd, sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double d[3][3], pos[3];
+    for (int j = 0; j < nu; j++) {
+      const double* loc = s->loc_upts + nd * j;
+      calc_pos(s, loc, i, pos);
+      for (int k = 0; k < nd; k++) s->pos_upts[j + nu * (i + (long)N * k)] = pos[k];
+      calc_d_pos(s, loc, i, d);
+      if (nd == 2) {
+        double xr = d[0][0], xs = d[0][1], yr = d[1][0], ys = d[1][1];
+        s->detjac_upts[j + nu * i] = xr * ys - xs * yr;
+        JG(0, 0, j, i) = ys; JG(0, 1, j, i) = -xs; JG(1, 0, j, i) = -yr; JG(1, 1, j, i) = xr;
+      } else {
+        double xr = d[0][0], xs = d[0][1], xt = d[0][2], yr = d[1][0], ys = d[1][1], yt = d[1][2],
+               zr = d[2][0], zs = d[2][1], zt = d[2][2];
+        s->detjac_upts[j + nu * i] = xr * (ys * zt - yt * zs) - xs * (yr * zt - yt * zr) + xt * (yr * zs - ys * zr);
+        JG(0, 0, j, i) = ys * zt - yt * zs; JG(0, 1, j, i) = xt * zs - xs * zt; JG(0, 2, j, i) = xs * yt - xt * ys;
+        JG(1, 0, j, i) = yt * zr - yr * zt; JG(1, 1, j, i) = xr * zt - xt * zr; JG(1, 2, j, i) = xt * yr - xr * yt;
+        JG(2, 0, j, i) = yr * zs - ys * zr; JG(2, 1, j, i) = xs * zr - xr * zs; JG(2, 2, j, i) = xr * ys - xs * yr;
+      }
+    }
+    for (int j = 0; j < nfp; j++) {
+      const double* loc = s->tloc_fpts + nd * j;
+      const double* tn = s->tnorm_fpts + nd * j;
+      calc_pos(s, loc, i, pos);
+      for (int k = 0; k < nd; k++) s->pos_fpts[j + nfp * (i + (long)N * k)] = pos[k];
+      calc_d_pos(s, loc, i, d);
+      double t[3], tdA;
+      if (nd == 2) {
+        t[0] = (tn[0] * d[1][1]) - (tn[1] * d[1][0]);
+        t[1] = -(tn[0] * d[0][1]) + (tn[1] * d[0][0]);
+        tdA = sqrt(t[0] * t[0] + t[1] * t[1]);
+      } else {
+        t[0] = ((tn[0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1])) + (tn[1] * (d[1][2] * d[2][0] - d[1][0] * d[2][2])) +
+                (tn[2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0])));
+        t[1] = ((tn[0] * (d[0][2] * d[2][1] - d[0][1] * d[2][2])) + (tn[1] * (d[0][0] * d[2][2] - d[0][2] * d[2][0])) +
+                (tn[2] * (d[0][1] * d[2][0] - d[0][0] * d[2][1])));
+        t[2] = ((tn[0] * (d[0][1] * d[1][2] - d[0][2] * d[1][1])) + (tn[1] * (d[0][2] * d[1][0] - d[0][0] * d[1][2])) +
+                (tn[2] * (d[0][0] * d[1][1] - d[0][1] * d[1][0])));
+        tdA = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      }
+      s->tdA_fpts[j + nfp * i] = tdA;
+      for (int k = 0; k < nd; k++) s->norm_fpts[j + nfp * (i + (long)N * k)] = t[k] / tdA;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Connectivity: face matching by shared vertices (src/geometry.cpp:2412-2700),
+ * periodic faces by translated centroids (src/geometry.cpp:489-563), flux points
+ * paired by position (the reference's rot_tag LUT, src/inters.cpp:187-296, encodes
+ * the same pairing). */
+
+static const int hex_face_nodes[6][4] = {{0, 1, 2, 3}, {0, 1, 4, 5}, {1, 3, 5, 7},
+                                         {2, 3, 6, 7}, {0, 2, 4, 6}, {4, 5, 6, 7}};
+static const int quad_face_nodes[4][2] = {{0, 1}, {1, 3}, {2, 3}, {0, 2}};
+
+typedef struct { int key[4]; int e, f; } face_rec;
+
+static int cmp_face(const void* a, const void* b) {
+  const face_rec *x = a, *y = b;
+  for (int i = 0; i < 4; i++) {
+    if (x->key[i] != y->key[i]) return x->key[i] < y->key[i] ? -1 : 1;
+  }
+  if (x->e != y->e) return x->e < y->e ? -1 : 1;
+  return x->f - y->f;
+}
+
+static void sort4(int* k, int n) {
+  for (int i = 1; i < n; i++)
+    for (int j = i; j > 0 && k[j - 1] > k[j]; j--) { int t = k[j]; k[j] = k[j - 1]; k[j - 1] = t; }
+}
+
+static void face_centroid(const or_solver* s, const int* c2v, const double* xv, int e, int f, double* c) {
+  int nv = s->ele_type == 4 ? 4 : 2;
+  int nsp = s->nsp;
+  for (int m = 0; m < s->nd; m++) c[m] = 0.0;
+  for (int k = 0; k < nv; k++) {
+    int node = s->ele_type == 4 ? hex_face_nodes[f][k] : quad_face_nodes[f][k];
+    int v = c2v[node + nsp * e];
+    for (int m = 0; m < s->nd; m++) c[m] += xv[m + s->nd * v] / nv;
+  }
+}
+
+/* Pair the flux points of (eL,fL) with those of (eR,fR); shift = xR - xL. */
+static void pair_points(or_solver* s, int eL, int fL, int eR, int fR, const double* shift, int* outL, int* outR) {
+  int nfpf = s->nfpf, nfp = s->nfp, N = s->n_eles, nd = s->nd;
+  for (int j = 0; j < nfpf; j++) {
+    int fl = fL * nfpf + j;
+    double pl[3];
+    for (int m = 0; m < nd; m++) pl[m] = s->pos_fpts[fl + nfp * (eL + (long)N * m)] + shift[m];
+    int best = -1;
+    double bd = 1e300;
+    for (int q = 0; q < nfpf; q++) {
+      int fr = fR * nfpf + q;
+      double dd = 0;
+      for (int m = 0; m < nd; m++) {
+        double t = s->pos_fpts[fr + nfp * (eR + (long)N * m)] - pl[m];
+        dd += t * t;
+      }
+      if (dd < bd) { bd = dd; best = fr; }
+    }
+    outL[j] = fl + nfp * eL;
+    outR[j] = best + nfp * eR;
+  }
+}
+
+static int set_connectivity(or_solver* s, const int* c2v, const double* xv) {
+  int N = s->n_eles, nfaces = s->nfaces, nv = s->ele_type == 4 ? 4 : 2;
+  long nrec = (long)N * nfaces;
+  face_rec* r = xcalloc(nrec, sizeof(face_rec));
+  for (int e = 0; e < N; e++)
+    for (int f = 0; f < nfaces; f++) {
+      face_rec* q = r + (long)e * nfaces + f;
+      q->e = e; q->f = f;
+      for (int k = 0; k < 4; k++) q->key[k] = -1;
+      for (int k = 0; k < nv; k++) {
+        int node = s->ele_type == 4 ? hex_face_nodes[f][k] : quad_face_nodes[f][k];
+        q->key[k] = c2v[node + s->nsp * e];
+      }
+      sort4(q->key, nv);
+    }
+  qsort(r, nrec, sizeof(face_rec), cmp_face);
+  /* partner[e*nfaces+f] = other (e,f) or -1 */
+  long* partner = xcalloc(nrec, sizeof(long));
+  for (long i = 0; i < nrec; i++) partner[i] = -1;
+  for (long i = 0; i + 1 < nrec; i++) {
+    if (memcmp(r[i].key, r[i + 1].key, sizeof(r[i].key)) == 0) {
+      long a = (long)r[i].e * nfaces + r[i].f, b = (long)r[i + 1].e * nfaces + r[i + 1].f;
+      partner[a] = b; partner[b] = a;
+      i++;
+    }
+  }
+  free(r);
+  /* periodic matching of the remaining faces */
+  double delta[3] = {s->deck.dx_cyclic, s->deck.dy_cyclic, s->deck.dz_cyclic};
+  long nun = 0;
+  long* un = xcalloc(nrec, sizeof(long));
+  for (long i = 0; i < nrec; i++) if (partner[i] < 0) un[nun++] = i;
+  double* cen = xcalloc((size_t)nun * 3, sizeof(double));
+  for (long i = 0; i < nun; i++) face_centroid(s, c2v, xv, (int)(un[i] / nfaces), (int)(un[i] % nfaces), cen + 3 * i);
+  const double tol = 1.e-6;
+  for (long i = 0; i < nun; i++) {
+    if (partner[un[i]] >= 0) continue;
+    for (long j = i + 1; j < nun; j++) {
+      if (partner[un[j]] >= 0) continue;
+      int ok = 0;
+      for (int m = 0; m < s->nd && !ok; m++) {
+        if (!isfinite(delta[m])) continue;
+        int match = fabs(fabs(cen[3 * i + m] - cen[3 * j + m]) - delta[m]) < tol;
+        for (int q = 0; q < s->nd && match; q++)
+          if (q != m && fabs(cen[3 * i + q] - cen[3 * j + q]) >= tol) match = 0;
+        ok = match;
+      }
+      if (ok) { partner[un[i]] = un[j]; partner[un[j]] = un[i]; break; }
+    }
+  }
+  for (long i = 0; i < nrec; i++)
+    if (partner[i] < 0) {
+      fprintf(stderr, "fr_oracle: unmatched face (ele %ld, face %ld); walls are not restated\n",
+              i / nfaces, i % nfaces);
+      free(partner); free(un); free(cen);
+      return -1;
+    }
+  /* interfaces in order of first appearance: L = first (e,f) in element order */
+  s->n_inters = (int)(nrec / 2);
+  s->face_l = xcalloc((size_t)s->n_inters * s->nfpf, sizeof(int));
+  s->face_r = xcalloc((size_t)s->n_inters * s->nfpf, sizeof(int));
+  int it = 0;
+  for (long a = 0; a < nrec; a++) {
+    long b = partner[a];
+    if (b < a) continue;
+    int eL = (int)(a / nfaces), fL = (int)(a % nfaces), eR = (int)(b / nfaces), fR = (int)(b % nfaces);
+    double cl[3] = {0, 0, 0}, cr[3] = {0, 0, 0}, shift[3] = {0, 0, 0};
+    face_centroid(s, c2v, xv, eL, fL, cl);
+    face_centroid(s, c2v, xv, eR, fR, cr);
+    for (int m = 0; m < s->nd; m++) {
+      double dlt = cr[m] - cl[m];
+      if (isfinite(delta[m]) && fabs(fabs(dlt) - delta[m]) < tol) shift[m] = dlt > 0 ? delta[m] : -delta[m];
+    }
+    pair_points(s, eL, fL, eR, fR, shift, s->face_l + (long)it * s->nfpf, s->face_r + (long)it * s->nfpf);
+    it++;
+  }
+  free(partner); free(un); free(cen);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+
+or_solver* or_create(const or_deck* deck, int ele_type, int n_verts, const double* xv, int n_eles,
+                     const int* c2v) {
+  if (ele_type != 1 && ele_type != 4) { fprintf(stderr, "fr_oracle: ele_type %d\n", ele_type); return NULL; }
+  if (deck->order < 0 || deck->order > 6 || deck->upts_type != 0 || deck->equation != 0 || deck->LES) {
+    fprintf(stderr, "fr_oracle: unsupported deck\n");
+    return NULL;
+  }
+  or_solver* s = xcalloc(1, sizeof(or_solver));
+  s->deck = *deck;
+  or_setup_params(deck, &s->ph);
+  s->ele_type = ele_type;
+  s->order = deck->order;
+  int n = s->order + 1;
+  s->nd = ele_type == 4 ? 3 : 2;
+  s->nf = s->nd + 2;
+  s->nu = ele_type == 4 ? n * n * n : n * n;
+  s->nfaces = ele_type == 4 ? 6 : 4;
+  s->nfpf = ele_type == 4 ? n * n : n;
+  s->nfp = s->nfaces * s->nfpf;
+  s->nsp = ele_type == 4 ? 8 : 4;
+  s->n_eles = n_eles;
+  s->n_verts = n_verts;
+  for (int i = 0; i < n; i++) s->loc1d[i] = gauss_tab[s->order][i];
+  set_points(s);
+  set_operators(s);
+  s->shape = xcalloc((size_t)s->nd * s->nsp * n_eles, sizeof(double));
+  for (int e = 0; e < n_eles; e++)
+    for (int j = 0; j < s->nsp; j++)
+      for (int m = 0; m < s->nd; m++)
+        s->shape[m + s->nd * (j + s->nsp * e)] = xv[m + s->nd * c2v[j + s->nsp * e]];
+  set_transforms(s);
+  if (set_connectivity(s, c2v, xv) != 0) { or_destroy(s); return NULL; }
+  long nuN = (long)s->nu * n_eles, nfpN = (long)s->nfp * n_eles;
+  int nf = s->nf, nd = s->nd;
+  s->disu_upts[0] = xcalloc(nuN * nf, sizeof(double));
+  s->disu_upts[1] = xcalloc(nuN * nf, sizeof(double));
+  s->div_tconf_upts = xcalloc(nuN * nf, sizeof(double));
+  s->src_upts = xcalloc(nuN * nf, sizeof(double));
+  s->disu_fpts = xcalloc(nfpN * nf, sizeof(double));
+  s->tdisf_upts = xcalloc(nuN * nf * nd, sizeof(double));
+  s->norm_tdisf_fpts = xcalloc(nfpN * nf, sizeof(double));
+  s->norm_tconf_fpts = xcalloc(nfpN * nf, sizeof(double));
+  if (deck->viscous) {
+    s->delta_disu_fpts = xcalloc(nfpN * nf, sizeof(double));
+    s->grad_disu_upts = xcalloc(nuN * nf * nd, sizeof(double));
+    s->grad_disu_fpts = xcalloc(nfpN * nf * nd, sizeof(double));
+  }
+  return s;
+}
+
+void or_destroy(or_solver* s) {
+  if (!s) return;
+  free(s->loc_upts); free(s->tloc_fpts); free(s->tnorm_fpts); free(s->opp_0); free(s->opp_3);
+  for (int d = 0; d < 3; d++) { free(s->opp_1[d]); free(s->opp_2[d]); free(s->opp_4[d]); free(s->opp_5[d]); }
+  free(s->shape); free(s->detjac_upts); free(s->JGinv_upts); free(s->tdA_fpts); free(s->norm_fpts);
+  free(s->pos_upts); free(s->pos_fpts);
+  free(s->disu_upts[0]); free(s->disu_upts[1]); free(s->div_tconf_upts); free(s->src_upts);
+  free(s->disu_fpts); free(s->tdisf_upts); free(s->norm_tdisf_fpts); free(s->norm_tconf_fpts);
+  free(s->delta_disu_fpts); free(s->grad_disu_upts); free(s->grad_disu_fpts);
+  free(s->face_l); free(s->face_r);
+  free(s);
+}
+
+int or_n_upts(const or_solver* s) { return s->nu; }
+int or_n_fpts(const or_solver* s) { return s->nfp; }
+int or_n_fields(const or_solver* s) { return s->nf; }
+int or_n_dims(const or_solver* s) { return s->nd; }
+int or_n_eles(const or_solver* s) { return s->n_eles; }
+int or_n_inters(const or_solver* s) { return s->n_inters; }
+void or_get_phys(const or_solver* s, or_phys* out) { *out = s->ph; }
+
+long or_get(const or_solver* s, int what, double* out) {
+  long nuN = (long)s->nu * s->n_eles, nfpN = (long)s->nfp * s->n_eles;
+  int nf = s->nf, nd = s->nd;
+  const double* src = NULL;
+  long n = 0;
+  switch (what) {
+    case 0: src = s->disu_upts[0]; n = nuN * nf; break;
+    case 1: src = s->disu_upts[1]; n = nuN * nf; break;
+    case 2: src = s->div_tconf_upts; n = nuN * nf; break;
+    case 3: src = s->disu_fpts; n = nfpN * nf; break;
+    case 4: src = s->grad_disu_upts; n = src ? nuN * nf * nd : 0; break;
+    case 5: src = s->grad_disu_fpts; n = src ? nfpN * nf * nd : 0; break;
+    case 6: src = s->detjac_upts; n = nuN; break;
+    case 7: src = s->JGinv_upts; n = nuN * nd * nd; break;
+    case 8: src = s->tdA_fpts; n = nfpN; break;
+    case 9: src = s->norm_fpts; n = nfpN * nd; break;
+    case 10: src = s->pos_upts; n = nuN * nd; break;
+    case 11: src = s->norm_tconf_fpts; n = nfpN * nf; break;
+    case 12: src = s->opp_0; n = (long)s->nfp * s->nu; break;
+    case 13: src = s->opp_3; n = (long)s->nu * s->nfp; break;
+    case 14: n = (long)s->nu * s->nu * nd; break;
+    case 15: src = s->loc_upts; n = (long)nd * s->nu; break;
+    case 16: src = s->tloc_fpts; n = (long)nd * s->nfp; break;
+    case 17: src = s->pos_fpts; n = nfpN * nd; break;
+    default: return -1;
+  }
+  if (out && n > 0) {
+    if (what == 14) {
+      for (int d = 0; d < nd; d++) memcpy(out + (long)d * s->nu * s->nu, s->opp_4[d], sizeof(double) * s->nu * s->nu);
+    } else {
+      memcpy(out, src, sizeof(double) * n);
+    }
+  }
+  return n;
+}
+
+void or_set_state(or_solver* s, const double* u) {
+  memcpy(s->disu_upts[0], u, sizeof(double) * (long)s->nu * s->n_eles * s->nf);
+}
+void or_zero_rk_register(or_solver* s) {
+  memset(s->disu_upts[1], 0, sizeof(double) * (long)s->nu * s->n_eles * s->nf);
+}
+
+long or_get_face_pairs(const or_solver* s, int* left, int* right) {
+  long n = (long)s->n_inters * s->nfpf;
+  if (left) memcpy(left, s->face_l, sizeof(int) * n);
+  if (right) memcpy(right, s->face_r, sizeof(int) * n);
+  return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Pointwise physics, src/flux.cpp */
+
+/* src/flux.cpp:36-75 (2-D) and 77-128 (3-D), Euler part; f is (nf,nd) col-major */
+static void calc_invf(int nd, double gamma, const double* u, double* f) {
+  int nf = nd + 2;
+#define F(k, l) f[(k) + nf * (l)]
+  if (nd == 2) {
+    double vx = u[1] / u[0], vy = u[2] / u[0];
+    double p = (gamma - 1.0) * (u[3] - (0.5 * u[0] * ((vx * vx) + (vy * vy))));
+    F(0, 0) = u[1]; F(1, 0) = p + (u[1] * vx); F(2, 0) = u[2] * vx; F(3, 0) = vx * (u[3] + p);
+    F(0, 1) = u[2]; F(1, 1) = u[1] * vy; F(2, 1) = p + (u[2] * vy); F(3, 1) = vy * (u[3] + p);
+  } else {
+    double vx = u[1] / u[0], vy = u[2] / u[0], vz = u[3] / u[0];
+    double p = (gamma - 1.0) * (u[4] - (0.5 * u[0] * ((vx * vx) + (vy * vy) + (vz * vz))));
+    F(0, 0) = u[1]; F(1, 0) = p + (u[1] * vx); F(2, 0) = u[2] * vx; F(3, 0) = u[3] * vx; F(4, 0) = vx * (u[4] + p);
+    F(0, 1) = u[2]; F(1, 1) = u[1] * vy; F(2, 1) = p + (u[2] * vy); F(3, 1) = u[3] * vy; F(4, 1) = vy * (u[4] + p);
+    F(0, 2) = u[3]; F(1, 2) = u[1] * vz; F(2, 2) = u[2] * vz; F(3, 2) = p + (u[3] * vz); F(4, 2) = vz * (u[4] + p);
+  }
+#undef F
+}
+
+/* src/flux.cpp:132-259 (2-D) and 263-431 (3-D), NS viscous flux without SA.
+ * g is (nf,nd) col-major: g(k,m) = d u_k / d x_m */
+static void calc_visf(int nd, const or_phys* ph, const double* u, const double* g, double* f) {
+  int nf = nd + 2;
+#define F(k, l) f[(k) + nf * (l)]
+#define G(k, l) g[(k) + nf * (l)]
+  double mu_t = 0.0;
+  if (nd == 2) {
+    double rho = u[0], mom_x = u[1], mom_y = u[2], ene = u[3];
+    double rho_dx = G(0, 0), mom_x_dx = G(1, 0), mom_y_dx = G(2, 0), ene_dx = G(3, 0);
+    double rho_dy = G(0, 1), mom_x_dy = G(1, 1), mom_y_dy = G(2, 1), ene_dy = G(3, 1);
+    double uu = mom_x / rho, v = mom_y / rho;
+    double inte = ene / rho - 0.5 * (uu * uu + v * v);
+    double rt_ratio = (ph->gamma - 1.0) * inte / (ph->rt_inf);
+    double mu = (ph->mu_inf) * pow(rt_ratio, 1.5) * (1. + (ph->c_sth)) / (rt_ratio + (ph->c_sth));
+    mu = mu + ph->fix_vis * (ph->mu_inf - mu);
+    double du_dx = (mom_x_dx - rho_dx * uu) / rho, du_dy = (mom_x_dy - rho_dy * uu) / rho;
+    double dv_dx = (mom_y_dx - rho_dx * v) / rho, dv_dy = (mom_y_dy - rho_dy * v) / rho;
+    double dke_dx = 0.5 * (uu * uu + v * v) * rho_dx + rho * (uu * du_dx + v * dv_dx);
+    double dke_dy = 0.5 * (uu * uu + v * v) * rho_dy + rho * (uu * du_dy + v * dv_dy);
+    double de_dx = (ene_dx - dke_dx - rho_dx * inte) / rho;
+    double de_dy = (ene_dy - dke_dy - rho_dy * inte) / rho;
+    double diag = (du_dx + dv_dy) / 3.0;
+    double tauxx = 2.0 * (mu + mu_t) * (du_dx - diag);
+    double tauxy = (mu + mu_t) * (du_dy + dv_dx);
+    double tauyy = 2.0 * (mu + mu_t) * (dv_dy - diag);
+    double kq = (mu / ph->prandtl + (mu_t == 0.0 ? 0.0 : mu_t / ph->prandtl_t)) * (ph->gamma);
+    F(0, 0) = 0.0; F(1, 0) = -tauxx; F(2, 0) = -tauxy; F(3, 0) = -(uu * tauxx + v * tauxy + kq * de_dx);
+    F(0, 1) = 0.0; F(1, 1) = -tauxy; F(2, 1) = -tauyy; F(3, 1) = -(uu * tauxy + v * tauyy + kq * de_dy);
+  } else {
+    double rho = u[0], mom_x = u[1], mom_y = u[2], mom_z = u[3], ene = u[4];
+    double rho_dx = G(0, 0), mom_x_dx = G(1, 0), mom_y_dx = G(2, 0), mom_z_dx = G(3, 0), ene_dx = G(4, 0);
+    double rho_dy = G(0, 1), mom_x_dy = G(1, 1), mom_y_dy = G(2, 1), mom_z_dy = G(3, 1), ene_dy = G(4, 1);
+    double rho_dz = G(0, 2), mom_x_dz = G(1, 2), mom_y_dz = G(2, 2), mom_z_dz = G(3, 2), ene_dz = G(4, 2);
+    double uu = mom_x / rho, v = mom_y / rho, w = mom_z / rho;
+    double inte = ene / rho - 0.5 * (uu * uu + v * v + w * w);
+    double rt_ratio = (ph->gamma - 1.0) * inte / (ph->rt_inf);
+    double mu = (ph->mu_inf) * pow(rt_ratio, 1.5) * (1 + (ph->c_sth)) / (rt_ratio + (ph->c_sth));
+    mu = mu + ph->fix_vis * (ph->mu_inf - mu);
+    double du_dx = (mom_x_dx - rho_dx * uu) / rho, du_dy = (mom_x_dy - rho_dy * uu) / rho, du_dz = (mom_x_dz - rho_dz * uu) / rho;
+    double dv_dx = (mom_y_dx - rho_dx * v) / rho, dv_dy = (mom_y_dy - rho_dy * v) / rho, dv_dz = (mom_y_dz - rho_dz * v) / rho;
+    double dw_dx = (mom_z_dx - rho_dx * w) / rho, dw_dy = (mom_z_dy - rho_dy * w) / rho, dw_dz = (mom_z_dz - rho_dz * w) / rho;
+    double dke_dx = 0.5 * (uu * uu + v * v + w * w) * rho_dx + rho * (uu * du_dx + v * dv_dx + w * dw_dx);
+    double dke_dy = 0.5 * (uu * uu + v * v + w * w) * rho_dy + rho * (uu * du_dy + v * dv_dy + w * dw_dy);
+    double dke_dz = 0.5 * (uu * uu + v * v + w * w) * rho_dz + rho * (uu * du_dz + v * dv_dz + w * dw_dz);
+    double de_dx = (ene_dx - dke_dx - rho_dx * inte) / rho;
+    double de_dy = (ene_dy - dke_dy - rho_dy * inte) / rho;
+    double de_dz = (ene_dz - dke_dz - rho_dz * inte) / rho;
+    double diag = (du_dx + dv_dy + dw_dz) / 3.0;
+    double tauxx = 2.0 * (mu + mu_t) * (du_dx - diag);
+    double tauyy = 2.0 * (mu + mu_t) * (dv_dy - diag);
+    double tauzz = 2.0 * (mu + mu_t) * (dw_dz - diag);
+    double tauxy = (mu + mu_t) * (du_dy + dv_dx);
+    double tauxz = (mu + mu_t) * (du_dz + dw_dx);
+    double tauyz = (mu + mu_t) * (dv_dz + dw_dy);
+    double kq = (mu / ph->prandtl + (mu_t == 0.0 ? 0.0 : mu_t / ph->prandtl_t)) * (ph->gamma);
+    F(0, 0) = 0.0; F(1, 0) = -tauxx; F(2, 0) = -tauxy; F(3, 0) = -tauxz;
+    F(4, 0) = -(uu * tauxx + v * tauxy + w * tauxz + kq * de_dx);
+    F(0, 1) = 0.0; F(1, 1) = -tauxy; F(2, 1) = -tauyy; F(3, 1) = -tauyz;
+    F(4, 1) = -(uu * tauxy + v * tauyy + w * tauyz + kq * de_dy);
+    F(0, 2) = 0.0; F(1, 2) = -tauxz; F(2, 2) = -tauyz; F(3, 2) = -tauzz;
+    F(4, 2) = -(uu * tauxz + v * tauyz + w * tauzz + kq * de_dz);
+  }
+#undef F
+#undef G
+}
+
+/* src/inters.cpp:311-365, with grid velocity zero (static mesh) */
+static void rusanov_flux(int nd, double gamma, const double* ul, const double* ur, const double* fl,
+                         const double* fr, const double* n, double* fn) {
+  int nf = nd + 2;
+  double fnl[5], fnr[5];
+  for (int k = 0; k < nf; k++) {
+    fnl[k] = 0.; fnr[k] = 0.;
+    for (int l = 0; l < nd; l++) { fnl[k] += fl[k + nf * l] * n[l]; fnr[k] += fr[k + nf * l] * n[l]; }
+  }
+  double vx_l = ul[1] / ul[0], vx_r = ur[1] / ur[0], vy_l = ul[2] / ul[0], vy_r = ur[2] / ur[0];
+  double vn_l, vn_r, vn_g, p_l, p_r;
+  const double vg0 = 0.0;
+  if (nd == 2) {
+    vn_l = vx_l * n[0] + vy_l * n[1];
+    vn_r = vx_r * n[0] + vy_r * n[1];
+    vn_g = vg0 * n[0] + vg0 * n[1];
+    p_l = (gamma - 1.0) * (ul[3] - (0.5 * ul[0] * ((vx_l * vx_l) + (vy_l * vy_l))));
+    p_r = (gamma - 1.0) * (ur[3] - (0.5 * ur[0] * ((vx_r * vx_r) + (vy_r * vy_r))));
+  } else {
+    double vz_l = ul[3] / ul[0], vz_r = ur[3] / ur[0];
+    vn_l = vx_l * n[0] + vy_l * n[1] + vz_l * n[2];
+    vn_r = vx_r * n[0] + vy_r * n[1] + vz_r * n[2];
+    vn_g = vg0 * n[0] + vg0 * n[1] + vg0 * n[2];
+    p_l = (gamma - 1.0) * (ul[4] - (0.5 * ul[0] * ((vx_l * vx_l) + (vy_l * vy_l) + (vz_l * vz_l))));
+    p_r = (gamma - 1.0) * (ur[4] - (0.5 * ur[0] * ((vx_r * vx_r) + (vy_r * vy_r) + (vz_r * vz_r))));
+  }
+  double vn_av_mag = sqrt(0.25 * (vn_l + vn_r) * (vn_l + vn_r));
+  double c_av = sqrt((gamma * (p_l + p_r)) / (ul[0] + ur[0]));
+  double eig = fabs(vn_av_mag - vn_g + c_av);
+  for (int k = 0; k < nf; k++) fn[k] = 0.5 * ((fnl[k] + fnr[k]) - eig * (ur[k] - ul[k]));
+}
+
+/* LDG switch direction, src/inters.cpp:556-566 and 650-660 */
+static double ldg_pen(int nd, double pen, const double* n) {
+  if (nd == 2) { if ((n[0] + n[1]) < 0.) pen = -pen; }
+  else { if ((n[0] + n[1] + sqrt(2.) * n[2]) < 0.) pen = -pen; }
+  return pen;
+}
+
+/* src/inters.cpp:550-641 (flux_spec 0) */
+static void ldg_flux(int nd, const double* ul, const double* ur, const double* fl, const double* fr,
+                     const double* n, double* fn, double tau, double pen) {
+  int nf = nd + 2;
+  double fc[15];
+  pen = ldg_pen(nd, pen, n);
+  double nx = n[0], ny = n[1], nz = nd == 3 ? n[2] : 0.0;
+#define FL(k, l) fl[(k) + nf * (l)]
+#define FR(k, l) fr[(k) + nf * (l)]
+  for (int k = 0; k < nf; k++) {
+    if (nd == 2) {
+      fc[k + nf * 0] = 0.5 * (FL(k, 0) + FR(k, 0)) + pen * nx * (nx * (FL(k, 0) - FR(k, 0)) + ny * (FL(k, 1) - FR(k, 1))) + tau * nx * (ul[k] - ur[k]);
+      fc[k + nf * 1] = 0.5 * (FL(k, 1) + FR(k, 1)) + pen * ny * (nx * (FL(k, 0) - FR(k, 0)) + ny * (FL(k, 1) - FR(k, 1))) + tau * ny * (ul[k] - ur[k]);
+    } else {
+      fc[k + nf * 0] = 0.5 * (FL(k, 0) + FR(k, 0)) + pen * nx * (nx * (FL(k, 0) - FR(k, 0)) + ny * (FL(k, 1) - FR(k, 1)) + nz * (FL(k, 2) - FR(k, 2))) + tau * nx * (ul[k] - ur[k]);
+      fc[k + nf * 1] = 0.5 * (FL(k, 1) + FR(k, 1)) + pen * ny * (nx * (FL(k, 0) - FR(k, 0)) + ny * (FL(k, 1) - FR(k, 1)) + nz * (FL(k, 2) - FR(k, 2))) + tau * ny * (ul[k] - ur[k]);
+      fc[k + nf * 2] = 0.5 * (FL(k, 2) + FR(k, 2)) + pen * nz * (nx * (FL(k, 0) - FR(k, 0)) + ny * (FL(k, 1) - FR(k, 1)) + nz * (FL(k, 2) - FR(k, 2))) + tau * nz * (ul[k] - ur[k]);
+    }
+  }
+#undef FL
+#undef FR
+  for (int k = 0; k < nf; k++) {
+    fn[k] = fc[k + nf * 0] * n[0];
+    for (int l = 1; l < nd; l++) fn[k] += fc[k + nf * l] * n[l];
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* CalcResidual steps */
+
+#define UPT(a, j, i, k) (a)[(j) + (long)nu * ((i) + (long)N * (k))]
+#define UPT4(a, j, i, k, l) (a)[(j) + (long)nu * ((i) + (long)N * ((k) + (long)nf * (l)))]
+
+/* src/eles.cpp:1256-1327 */
+static void extrapolate_solution(or_solver* s) {
+  dgemm_ref(s->nfp, s->nf * s->n_eles, s->nu, 1.0, 0.0, s->opp_0, s->disu_upts[0], s->disu_fpts);
+}
+
+/* src/eles.cpp:1667-1766 */
+static void calculate_gradient(or_solver* s) {
+  long nuN = (long)s->nu * s->n_eles;
+  for (int d = 0; d < s->nd; d++)
+    dgemm_ref(s->nu, s->nf * s->n_eles, s->nu, 1.0, 0.0, s->opp_4[d], s->disu_upts[0], s->grad_disu_upts + nuN * s->nf * d);
+}
+
+/* src/eles.cpp:1331-1418 */
+static void evaluate_invFlux(or_solver* s) {
+  int nu = s->nu, N = s->n_eles, nf = s->nf, nd = s->nd;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double u[5], f[15];
+    for (int j = 0; j < nu; j++) {
+      for (int k = 0; k < nf; k++) u[k] = UPT(s->disu_upts[0], j, i, k);
+      calc_invf(nd, s->ph.gamma, u, f);
+      for (int k = 0; k < nf; k++)
+        for (int l = 0; l < nd; l++) {
+          double t = 0.;
+          for (int m = 0; m < nd; m++) t += JG(l, m, j, i) * f[k + nf * m];
+          UPT4(s->tdisf_upts, j, i, k, l) = t;
+        }
+    }
+  }
+}
+
+/* src/int_inters.cpp:225-354 */
+static void common_invFlux(or_solver* s) {
+  int nf = s->nf, nd = s->nd, nfp = s->nfp, N = s->n_eles;
+  long nfpN = (long)nfp * N, npts = (long)s->n_inters * s->nfpf;
+#pragma omp parallel for schedule(static)
+  for (long q = 0; q < npts; q++) {
+    int a = s->face_l[q], b = s->face_r[q];
+    double ul[5], ur[5], fl[15], fr[15], n[3] = {0, 0, 0}, fn[5];
+    for (int k = 0; k < nf; k++) { ul[k] = s->disu_fpts[a + nfpN * k]; ur[k] = s->disu_fpts[b + nfpN * k]; }
+    for (int m = 0; m < nd; m++) n[m] = s->norm_fpts[a + nfpN * m];
+    calc_invf(nd, s->ph.gamma, ul, fl);
+    calc_invf(nd, s->ph.gamma, ur, fr);
+    rusanov_flux(nd, s->ph.gamma, ul, ur, fl, fr, n, fn);
+    double tdl = s->tdA_fpts[a], tdr = s->tdA_fpts[b];
+    for (int k = 0; k < nf; k++) {
+      s->norm_tconf_fpts[a + nfpN * k] = fn[k] * tdl;
+      s->norm_tconf_fpts[b + nfpN * k] = -fn[k] * tdr;
+    }
+    if (s->deck.viscous) {
+      double pen = ldg_pen(nd, s->ph.pen_fact, n);
+      for (int k = 0; k < nf; k++) {
+        double uc = 0.5 * (ul[k] + ur[k]) - pen * (ul[k] - ur[k]);
+        s->delta_disu_fpts[a + nfpN * k] = (uc - ul[k]);
+        s->delta_disu_fpts[b + nfpN * k] = (uc - ur[k]);
+      }
+    }
+  }
+}
+
+/* src/eles.cpp:1770-1968 */
+static void correct_gradient(or_solver* s) {
+  int nu = s->nu, N = s->n_eles, nf = s->nf, nd = s->nd;
+  long nuN = (long)nu * N;
+  for (int d = 0; d < nd; d++)
+    dgemm_ref(nu, nf * N, s->nfp, 1.0, 1.0, s->opp_5[d], s->delta_disu_fpts, s->grad_disu_upts + nuN * nf * d);
+  double* g = s->grad_disu_upts;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < nu; j++) {
+      double detjac = s->detjac_upts[j + nu * i];
+      double inv_detjac = 1.0 / detjac;
+      double rx = JG(0, 0, j, i) * inv_detjac, ry = JG(0, 1, j, i) * inv_detjac;
+      double sx = JG(1, 0, j, i) * inv_detjac, sy = JG(1, 1, j, i) * inv_detjac;
+      if (nd == 2) {
+        for (int k = 0; k < nf; k++) {
+          double ur = UPT4(g, j, i, k, 0), us = UPT4(g, j, i, k, 1);
+          UPT4(g, j, i, k, 0) = ur * rx + us * sx;
+          UPT4(g, j, i, k, 1) = ur * ry + us * sy;
+        }
+      } else {
+        double rz = JG(0, 2, j, i) * inv_detjac, sz = JG(1, 2, j, i) * inv_detjac;
+        double tx = JG(2, 0, j, i) * inv_detjac, ty = JG(2, 1, j, i) * inv_detjac, tz = JG(2, 2, j, i) * inv_detjac;
+        for (int k = 0; k < nf; k++) {
+          double ur = UPT4(g, j, i, k, 0), us = UPT4(g, j, i, k, 1), ut = UPT4(g, j, i, k, 2);
+          UPT4(g, j, i, k, 0) = ur * rx + us * sx + ut * tx;
+          UPT4(g, j, i, k, 1) = ur * ry + us * sy + ut * ty;
+          UPT4(g, j, i, k, 2) = ur * rz + us * sz + ut * tz;
+        }
+      }
+    }
+}
+
+/* src/eles.cpp:1973-2052 */
+static void extrapolate_corrected_gradient(or_solver* s) {
+  long nuN = (long)s->nu * s->n_eles, nfpN = (long)s->nfp * s->n_eles;
+  for (int d = 0; d < s->nd; d++)
+    dgemm_ref(s->nfp, s->nf * s->n_eles, s->nu, 1.0, 0.0, s->opp_6, s->grad_disu_upts + nuN * s->nf * d,
+              s->grad_disu_fpts + nfpN * s->nf * d);
+}
+
+/* src/eles.cpp:2321-2453 (static mesh, no LES / wall model) */
+static void evaluate_viscFlux(or_solver* s) {
+  int nu = s->nu, N = s->n_eles, nf = s->nf, nd = s->nd;
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double u[5], g[15], f[15];
+    for (int j = 0; j < nu; j++) {
+      for (int k = 0; k < nf; k++) {
+        u[k] = UPT(s->disu_upts[0], j, i, k);
+        for (int m = 0; m < nd; m++) g[k + nf * m] = UPT4(s->grad_disu_upts, j, i, k, m);
+      }
+      calc_visf(nd, &s->ph, u, g, f);
+      for (int k = 0; k < nf; k++)
+        for (int l = 0; l < nd; l++)
+          for (int m = 0; m < nd; m++) UPT4(s->tdisf_upts, j, i, k, l) += JG(l, m, j, i) * f[k + nf * m];
+    }
+  }
+}
+
+/* src/eles.cpp:1423-1512 */
+static void extrapolate_totalFlux(or_solver* s) {
+  long nuN = (long)s->nu * s->n_eles;
+  int cols = s->nf * s->n_eles;
+  dgemm_ref(s->nfp, cols, s->nu, 1.0, 0.0, s->opp_1[0], s->tdisf_upts, s->norm_tdisf_fpts);
+  for (int d = 1; d < s->nd; d++)
+    dgemm_ref(s->nfp, cols, s->nu, 1.0, 1.0, s->opp_1[d], s->tdisf_upts + nuN * s->nf * d, s->norm_tdisf_fpts);
+}
+
+/* src/eles.cpp:1517-1589 */
+static void calculate_divergence(or_solver* s) {
+  long nuN = (long)s->nu * s->n_eles;
+  int cols = s->nf * s->n_eles;
+  dgemm_ref(s->nu, cols, s->nu, 1.0, 0.0, s->opp_2[0], s->tdisf_upts, s->div_tconf_upts);
+  for (int d = 1; d < s->nd; d++)
+    dgemm_ref(s->nu, cols, s->nu, 1.0, 1.0, s->opp_2[d], s->tdisf_upts + nuN * s->nf * d, s->div_tconf_upts);
+}
+
+/* src/int_inters.cpp:359-470 (no LES) */
+static void common_viscFlux(or_solver* s) {
+  int nf = s->nf, nd = s->nd, nfp = s->nfp, N = s->n_eles;
+  long nfpN = (long)nfp * N, npts = (long)s->n_inters * s->nfpf;
+#pragma omp parallel for schedule(static)
+  for (long q = 0; q < npts; q++) {
+    int a = s->face_l[q], b = s->face_r[q];
+    double ul[5], ur[5], gl[15], gr[15], fl[15], fr[15], n[3] = {0, 0, 0}, fn[5];
+    for (int k = 0; k < nf; k++) { ul[k] = s->disu_fpts[a + nfpN * k]; ur[k] = s->disu_fpts[b + nfpN * k]; }
+    for (int m = 0; m < nd; m++)
+      for (int l = 0; l < nf; l++) {
+        gl[l + nf * m] = s->grad_disu_fpts[a + nfpN * (l + (long)nf * m)];
+        gr[l + nf * m] = s->grad_disu_fpts[b + nfpN * (l + (long)nf * m)];
+      }
+    calc_visf(nd, &s->ph, ul, gl, fl);
+    calc_visf(nd, &s->ph, ur, gr, fr);
+    for (int m = 0; m < nd; m++) n[m] = s->norm_fpts[a + nfpN * m];
+    ldg_flux(nd, ul, ur, fl, fr, n, fn, s->ph.tau, s->ph.pen_fact);
+    double tdl = s->tdA_fpts[a], tdr = s->tdA_fpts[b];
+    for (int k = 0; k < nf; k++) {
+      s->norm_tconf_fpts[a + nfpN * k] += fn[k] * tdl;
+      s->norm_tconf_fpts[b + nfpN * k] += -fn[k] * tdr;
+    }
+  }
+}
+
+/* src/eles.cpp:1594-1661 */
+static void calculate_corrected_divergence(or_solver* s) {
+  long n = (long)s->n_eles * s->nf * s->nfp;
+  for (long i = 0; i < n; i++) s->norm_tconf_fpts[i] += -1.0 * s->norm_tdisf_fpts[i];
+  dgemm_ref(s->nu, s->nf * s->n_eles, s->nfp, 1.0, 1.0, s->opp_3, s->norm_tconf_fpts, s->div_tconf_upts);
+  long m = (long)s->n_eles * s->nf * s->nu;
+  for (long i = 0; i < m; i++)
+    if (isnan(s->div_tconf_upts[i])) { fprintf(stderr, "fr_oracle: NaN in residual\n"); break; }
+}
+
+/* src/solver.cpp:65-225 */
+void or_residual(or_solver* s) {
+  extrapolate_solution(s);
+  if (s->deck.viscous) calculate_gradient(s);
+  evaluate_invFlux(s);
+  common_invFlux(s);
+  if (s->deck.viscous) {
+    correct_gradient(s);
+    extrapolate_corrected_gradient(s);
+    evaluate_viscFlux(s);
+  }
+  extrapolate_totalFlux(s);
+  calculate_divergence(s);
+  if (s->deck.viscous) common_viscFlux(s);
+  calculate_corrected_divergence(s);
+}
+
+/* src/eles.cpp:1003-1198, dt_type 0 */
+void or_advance(or_solver* s, int stage, int adv_type) {
+  int nu = s->nu, N = s->n_eles, nf = s->nf;
+  double dt = s->ph.dt, cs = s->ph.const_src;
+  if (adv_type == 0) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nf; i++)
+      for (int ic = 0; ic < N; ic++)
+        for (int inp = 0; inp < nu; inp++)
+          UPT(s->disu_upts[0], inp, ic, i) -= dt * (UPT(s->div_tconf_upts, inp, ic, i) / s->detjac_upts[inp + nu * ic] - cs - UPT(s->src_upts, inp, ic, i));
+    return;
+  }
+  static const double rk4a[5] = {0.0, -0.417890474499852, -1.192151694642677, -1.697784692471528, -1.514183444257156};
+  static const double rk4b[5] = {0.149659021999229, 0.379210312999627, 0.822955029386982, 0.699450455949122, 0.153057247968152};
+  double a = rk4a[stage], b = rk4b[stage];
+#pragma omp parallel for schedule(static)
+  for (int ic = 0; ic < N; ic++)
+    for (int i = 0; i < nf; i++)
+      for (int inp = 0; inp < nu; inp++) {
+        double rhs = -UPT(s->div_tconf_upts, inp, ic, i) / s->detjac_upts[inp + nu * ic] + cs + UPT(s->src_upts, inp, ic, i);
+        double res = UPT(s->disu_upts[1], inp, ic, i);
+        res = a * res + dt * rhs;
+        UPT(s->disu_upts[1], inp, ic, i) = res;
+        UPT(s->disu_upts[0], inp, ic, i) += b * res;
+      }
+}
+
+/* set_ics, ic_form 7 (Taylor-Green), src/eles.cpp:434-470, 555-568 */
+void or_set_ics_tgv(or_solver* s, double rho) {
+  int nu = s->nu, N = s->n_eles, nf = s->nf;
+  double gamma = s->ph.gamma;
+  for (int i = 0; i < N; i++)
+    for (int j = 0; j < nu; j++) {
+      double pos[3];
+      for (int k = 0; k < 3; k++) pos[k] = s->pos_upts[j + nu * (i + (long)N * k)];
+      double ics[5];
+      double p = 100 + rho / 16.0 * (cos(2.0 * pos[0]) + cos(2.0 * pos[1])) * (cos(2.0 * pos[2]) + 2.0);
+      ics[0] = rho;
+      ics[1] = sin(pos[0]) * cos(pos[1]) * cos(pos[2]);
+      ics[2] = -1.0 * cos(pos[0]) * sin(pos[1]) * cos(pos[2]);
+      ics[3] = 0.0;
+      ics[4] = p / (gamma - 1.0) + 0.5 * rho * (ics[1] * ics[1] + ics[2] * ics[2] + ics[3] * ics[3]);
+      for (int k = 0; k < nf; k++) UPT(s->disu_upts[0], j, i, k) = ics[k];
+    }
+}
+
+/* src/HiFiLES.cpp:154-187 */
+void or_run_steps(or_solver* s, int n_steps) {
+  int nst = s->deck.adv_type == 3 ? 5 : 1;
+  for (int it = 0; it < n_steps; it++)
+    for (int st = 0; st < nst; st++) {
+      or_residual(s);
+      or_advance(s, st, s->deck.adv_type);
+    }
+}
+
+/* src/output.cpp:1452-1522 + src/eles.cpp:6086-6116 */
+void or_norm_residual(const or_solver* s, int norm_type, double* out) {
+  int nu = s->nu, N = s->n_eles, nf = s->nf;
+  for (int k = 0; k < 5; k++) out[k] = 0.0;
+  for (int f = 0; f < nf && f < 5; f++) {
+    double sum = 0.;
+    for (int i = 0; i < N; i++) {
+      double cell = 0.;
+      for (int j = 0; j < nu; j++) {
+        double r = UPT(s->div_tconf_upts, j, i, f) / s->detjac_upts[j + nu * i] - s->ph.const_src - UPT(s->src_upts, j, i, f);
+        if (norm_type == 0) cell = fmax(cell, fabs(r));
+        else if (norm_type == 1) cell += fabs(r);
+        else cell += r * r;
+      }
+      if (norm_type == 0) sum = fmax(cell, sum);
+      else sum += cell;
+    }
+    long n_upts = (long)N * nu;
+    if (norm_type == 0) out[f] = sum;
+    else if (norm_type == 1) out[f] = sum / n_upts;
+    else out[f] = sqrt(sum) / n_upts;
+  }
+}

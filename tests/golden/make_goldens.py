@@ -1,0 +1,219 @@
+#!/usr/bin/env python3
+"""Writes the golden residual/RK fixtures of tests/golden/*.npz from the reference
+solver built by oracle/Makefile.ref (oracle/_ref/HiFiLES_cpu, serial, -D_NO_BLAS).
+
+Runs only in the build container (it needs /root/reference for the cubature
+tables and the reference binary); the fixtures it writes are plain data.
+
+Protocol (state injection through the reference's restart reader):
+  1. write a Gambit mesh, an input deck and Rest_000000000_p0000.dat holding a
+     seeded state u0 printed with %.17g (exact doubles; the restart header holds
+     the same 1-D points as the solver, so its interpolation opp_r is the identity);
+  2. residual cases: one forward-Euler step (adv_type 0) with dt = DT_BIG; the
+     restart written after the step holds u1 = u0 - dt*div/J to 15 digits, so
+     R = (u0 - u1)/dt recovers div/J to ~1e-15 relative wherever |R| >> |u0|/dt;
+  3. RK45 cases: n steps of adv_type 3 at the deck's dt; the final state is kept
+     to the 15 digits the writer prints.
+
+    python3 tests/golden/make_goldens.py        (from the repository root)
+"""
+import math
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import frmesh  # noqa: E402
+from oracle import pyoracle as po  # noqa: E402
+
+REF_HOME = "/root/reference"
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "HiFiLES_cpu")
+DT_BIG = 1.0e10
+TWO_PI = 6.2831853071795862
+
+
+def deck_text(c, restart_iter=0):
+    lines = [
+        "equation 0", "viscous %d" % c["viscous"], "riemann_solve_type 0", "vis_riemann_solve_type 0",
+        "ic_form %d" % c.get("ic_form", 1), "test_case 0", "order %d" % c["order"], "dt_type 0",
+        "dt %.17g" % c["dt"], "n_steps %d" % c["n_steps"], "adv_type %d" % c["adv_type"],
+        "tau %.17g" % c.get("tau", 0.0), "pen_fact %.17g" % c.get("pen_fact", 0.5),
+        "restart_flag 1", "restart_iter %d" % restart_iter, "n_restart_files 1",
+        "mesh_file mesh.neu",
+        "dx_cyclic %.17g" % c["cyc"][0], "dy_cyclic %.17g" % c["cyc"][1],
+        "dz_cyclic %.17g" % c["cyc"][2],
+        "p_res 2", "write_type 0", "inters_cub_order 3", "volume_cub_order 3",
+        "monitor_res_freq 1", "plot_freq 1000000", "data_file_name golden",
+        "restart_dump_freq %d" % c["n_steps"], "res_norm_type 1", "error_norm_type 1", "res_norm_field 0",
+        "upts_type_quad 0", "vcjh_scheme_quad %d" % c["vcjh"], "eta_quad 0.0", "sparse_quad 0",
+        "upts_type_hexa 0", "vcjh_scheme_hexa %d" % c["vcjh"], "eta_hexa 0.0", "sparse_hexa 0", "sparse_pri 0",
+        "gamma 1.4", "prandtl 0.72", "S_gas 120.", "T_gas 291.15", "R_gas 286.9", "mu_gas 1.827E-05",
+        "fix_vis 1", "Mach_free_stream %.17g" % c.get("mach", 0.1), "nx_free_stream 1.",
+        "ny_free_stream 0.", "nz_free_stream 0.", "Re_free_stream %.17g" % c.get("Re", 1600.0),
+        "L_free_stream 1.", "T_free_stream 300", "T_wall 300",
+        "rho_bound 1.", "u_bound 1.0", "v_bound 0.0", "w_bound 0.0", "p_bound 71.428571428571431",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+def oracle_deck(c):
+    return po.default_deck(
+        equation=0, viscous=c["viscous"], order=c["order"], adv_type=c["adv_type"], dt=c["dt"],
+        tau=c.get("tau", 0.0), pen_fact=c.get("pen_fact", 0.5), ic_form=c.get("ic_form", 1),
+        gamma=1.4, prandtl=0.72, Mach_free_stream=c.get("mach", 0.1), nx_free_stream=1.0,
+        Re_free_stream=c.get("Re", 1600.0), L_free_stream=1.0, T_free_stream=300.0, T_wall=300.0,
+        vcjh_scheme=c["vcjh"], eta=0.0, dx_cyclic=c["cyc"][0], dy_cyclic=c["cyc"][1],
+        dz_cyclic=c["cyc"][2])
+
+
+def write_restart(path, ele_type, loc1d, order, u0):
+    """u0: (Nu, N, nf) in the reference's point order."""
+    nu, n, nf = u0.shape
+    name = "HEXAS" if ele_type == 4 else "QUADS"
+    what = "hexahedral" if ele_type == 4 else "quadrilateral"
+    out = ["0", name, "Order", "%d" % order, "Number of solution points per %s element" % what,
+           "%d" % nu, "Location of solution points in 1D", " ".join("%.17g" % v for v in loc1d) + " ",
+           "n_eles", "%d" % n, "ele2global_ele array", " ".join("%d" % i for i in range(n)) + " ", "data"]
+    for e in range(n):
+        out.append("%d" % e)
+        for j in range(nu):
+            out.append(" ".join("%.17g" % u0[j, e, k] for k in range(nf)) + " ")
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n\n")
+
+
+def read_restart(path, nu, n, nf):
+    with open(path) as fh:
+        L = fh.read().split("\n")
+    i = L.index("data") + 1
+    u = np.zeros((nu, n, nf))
+    for e in range(n):
+        ele = int(L[i]); i += 1
+        for j in range(nu):
+            u[j, ele, :] = [float(v) for v in L[i].split()]
+            i += 1
+    return u
+
+
+def run_reference(c, xv, conn_g, u0, loc1d):
+    wd = tempfile.mkdtemp(prefix="golden_")
+    try:
+        c2v = frmesh.gambit_to_internal(c["ele_type"], conn_g)
+        bfaces = [(e, frmesh.internal_face_to_gambit(c["ele_type"], f))
+                  for e, f in frmesh.boundary_faces(c["ele_type"], c2v)]
+        frmesh.write_gambit(os.path.join(wd, "mesh.neu"), xv, conn_g, c["ele_type"],
+                            bc_groups=[("Cyclic", bfaces)])
+        with open(os.path.join(wd, "input"), "w") as fh:
+            fh.write(deck_text(c))
+        write_restart(os.path.join(wd, "Rest_000000000_p0000.dat"), c["ele_type"], loc1d, c["order"], u0)
+        env = dict(os.environ, HIFILES_HOME=REF_HOME)
+        r = subprocess.run([REF_BIN, "input"], cwd=wd, env=env, capture_output=True, text=True, timeout=3600)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        out = os.path.join(wd, "Rest_%09d_p0000.dat" % c["n_steps"])
+        return read_restart(out, *u0.shape), r.stdout
+    finally:
+        shutil.rmtree(wd, ignore_errors=True)
+
+
+CASES = [
+    # name, element, sizes, order, scheme, physics, state
+    dict(name="hex_p3_ns_tgv_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=2, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-3, seed=1234, perturb=0.0, kind="residual"),
+    dict(name="hex_p3_ns_perturbed_3x3x3", ele_type=4, n=(3, 3, 3), order=3, vcjh=1, viscous=1, tau=0.5,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=7, perturb=0.1, kind="residual"),
+    dict(name="hex_p2_euler_3x2x2", ele_type=4, n=(3, 2, 2), order=2, vcjh=3, viscous=0,
+         state="tgv", noise=1e-2, seed=11, perturb=0.08, kind="residual"),
+    dict(name="hex_p4_ns_2x2x2", ele_type=4, n=(2, 2, 2), order=4, vcjh=2, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-3, seed=5, perturb=0.0, kind="residual"),
+    dict(name="quad_p3_euler_vortex_4x4", ele_type=1, n=(4, 4), order=3, vcjh=1, viscous=0,
+         state="vortex", noise=0.0, seed=0, perturb=0.0, kind="residual"),
+    dict(name="quad_p3_ns_perturbed_4x3", ele_type=1, n=(4, 3), order=3, vcjh=2, viscous=1, tau=1.0,
+         pen_fact=0.5, state="vortex", noise=1e-3, seed=3, perturb=0.1, kind="residual"),
+    dict(name="hex_p3_ns_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=2, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-3, seed=1234, perturb=0.0, kind="rk45", steps=3, dt=1e-3),
+]
+
+
+def build_case(c):
+    if c["ele_type"] == 4:
+        nx, ny, nz = c["n"]
+        xv, conn = frmesh.box_hex(nx, ny, nz, TWO_PI, TWO_PI, TWO_PI)
+        c["cyc"] = (TWO_PI, TWO_PI, TWO_PI)
+        h = TWO_PI / max(c["n"])
+    else:
+        nx, ny = c["n"]
+        xv, conn = frmesh.box_quad(nx, ny, 20.0, 20.0, origin=(-10.0, -10.0))
+        c["cyc"] = (20.0, 20.0, float("inf"))
+        h = 20.0 / max(c["n"])
+    if c["perturb"] > 0:
+        xv = frmesh.perturb_interior(xv, c["perturb"] * h, seed=c["seed"] + 100)
+    return xv, conn
+
+
+def main():
+    if not os.path.exists(REF_BIN):
+        sys.exit("build the reference first: make -f oracle/Makefile.ref")
+    only = sys.argv[1:]
+    for c in CASES:
+        if only and c["name"] not in only:
+            continue
+        xv, conn_g = build_case(c)
+        c2v = frmesh.gambit_to_internal(c["ele_type"], conn_g)
+        if c["kind"] == "residual":
+            c.update(adv_type=0, dt=DT_BIG, n_steps=1)
+        else:
+            c.update(adv_type=3, n_steps=c["steps"])
+        o = po.Oracle(oracle_deck(c), c["ele_type"], xv, c2v)
+        pos = o.get("pos_upts")
+        if c["state"] == "tgv":
+            u0 = frmesh.tgv_state(pos, p0=1.0 / (1.4 * 0.1 ** 2))
+        else:
+            u0 = frmesh.shu_vortex_state(pos)
+        if c["noise"] > 0:
+            u0 = u0 * (1.0 + frmesh.seeded_perturbation(u0.shape, c["noise"], c["seed"]))
+        u0 = np.array([float("%.17g" % v) for v in u0.ravel()]).reshape(u0.shape)
+        loc1d = o.get("loc_upts")[0, : c["order"] + 1]
+        u1, log = run_reference(c, xv, conn_g, u0, loc1d)
+        rec = dict(xv=xv, conn_gambit=conn_g, ele_type=c["ele_type"], order=c["order"],
+                   vcjh=c["vcjh"], viscous=c["viscous"], tau=c.get("tau", 0.0),
+                   pen_fact=c.get("pen_fact", 0.5), cyc=np.array(c["cyc"]), dt=c["dt"],
+                   u0=u0, kind=c["kind"])
+        if c["kind"] == "residual":
+            R = (u0 - u1) / c["dt"]
+            rec["R"] = R
+            # oracle on the same mesh/state
+            o.set_state(u0)
+            o.residual()
+            Ro = o.get("div_tconf_upts") / o.get("detjac_upts")[:, :, None]
+            rel = np.abs(R - Ro).max(axis=(0, 1)) / np.abs(Ro).max(axis=(0, 1))
+            # the oracle's own forward-Euler state printed as the reference prints it
+            o.advance(0, 0)
+            u1o = o.get("disu_upts")
+            same = np.mean([float("%.15g" % a) == b for a, b in zip(u1o.ravel(), u1.ravel())])
+            print("%-30s ref-vs-oracle max rel per field: %s; identical 15-digit states: %.4f" % (
+                c["name"], " ".join("%.2e" % v for v in rel), same))
+        else:
+            rec["steps"] = c["steps"]
+            rec["u_final"] = u1
+            o.set_state(u0)
+            o.zero_rk_register()
+            o.run_steps(c["steps"])
+            uo = o.get("disu_upts")
+            rel = np.abs(u1 - uo).max(axis=(0, 1)) / np.abs(uo).max(axis=(0, 1))
+            same = np.mean([float("%.15g" % a) == b for a, b in zip(uo.ravel(), u1.ravel())])
+            print("%-30s ref-vs-oracle RK45 state max rel: %s; identical 15-digit states: %.4f" % (
+                c["name"], " ".join("%.2e" % v for v in rel), same))
+        np.savez_compressed(os.path.join(HERE, c["name"] + ".npz"), **rec)
+
+
+if __name__ == "__main__":
+    main()
