@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Throughput of the FR residual + RK45 update on one MI355X.
+
+Workload (the headline configuration): 3-D compressible Navier-Stokes,
+Taylor-Green initial field (Mach 0.1, Re 1600, Pr 0.71) plus a seeded 1e-3
+perturbation, periodic cube [0,2pi]^3 with 32^3 hexes at P=3 (64 solution points
+per element, 10,485,760 DOF), Rusanov + LDG, RK45, dt = 1e-3, FP64.  A step is one
+RK45 step (5 stages, i.e. 5 residual evaluations + updates); the metric counts
+DOF x stages per second.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+Prints one JSON line.  `roofline` is for the dominant kernel (flux, divergence
+and RK update, one launch per stage): algorithmic bytes per launch / its average
+duration from HIP events recorded around every launch in the timed region.
+`cpu_baseline` is the oracle (the C restatement of the reference's dense-operator
+CPU residual) on the same mesh and state, one RK45 step, timed on this host.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hifiles-solver_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import hifr  # noqa: E402
+
+METRIC = "GDOF-RK-stages/s, 3D NS hex P=3, FP64 vs mixed; % of HBM roofline"
+HBM_PEAK_GBS = 8000.0
+TWO_PI = 6.2831853071795862
+
+
+def deck(order, n_steps):
+    return hifr.deck_text(viscous=1, order=order, vcjh_scheme_hexa=1, dt=1e-3, adv_type=3, tau=1.0, pen_fact=0.5,
+                          gamma=1.4, prandtl=0.71, Mach_free_stream=0.1, nx_free_stream=1.0,
+                          Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, fix_vis=1,
+                          dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI, n_steps=n_steps)
+
+
+def tgv_state(pos, seed=1234):
+    import frmesh
+    u = frmesh.tgv_state(pos, p0=1.0 / (1.4 * 0.1 ** 2))
+    return u * (1.0 + frmesh.seeded_perturbation(u.shape, 1e-3, seed))
+
+
+def algorithmic_bytes(order, nd=3):
+    """Minimum HBM bytes per element of the two kernels of a Navier-Stokes stage
+    (FP64 values, int32 neighbour table):
+      gradient-trace kernel: read u, neighbour u_f, JGinv, detJ, neighbour index;
+                             write grad u_f;
+      flux/RK kernel:        read u, RK register, neighbour u_f and grad u_f,
+                             JGinv, detJ, face normal + tdA, neighbour index;
+                             write u, RK register, next u_f."""
+    n = order + 1
+    nu, nfp, nf = n ** nd, 2 * nd * n ** (nd - 1), nd + 2
+    grad = 8 * (nu * nf + nfp * nf + nu * nd * nd + nu + nfp * nf * nd) + 4 * nfp
+    rk = 8 * (4 * nu * nf + nfp * nf + nfp * nf * nd + nu * nd * nd + nu + nfp * (nd + 1) + nfp * nf) + 4 * nfp
+    return grad, rk
+
+
+def cpu_baseline(s, order):
+    from oracle import pyoracle as po
+    xv = s.get("xv").reshape(-1, 3)
+    c2v = s.get("c2v").reshape(-1, 8).astype(np.int32)
+    threads = min(16, os.cpu_count() or 1)
+    od = po.default_deck(equation=0, viscous=1, order=order, adv_type=3, dt=1e-3, tau=1.0, pen_fact=0.5,
+                         gamma=1.4, prandtl=0.71, Mach_free_stream=0.1, nx_free_stream=1.0,
+                         Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, vcjh_scheme=1,
+                         dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI)
+    o = po.Oracle(od, 4, xv, c2v, threads=threads)
+    o.set_state(tgv_state(o.get("pos_upts")))
+    o.zero_rk_register()
+    o.run_steps(1)  # warm-up
+    t0 = time.perf_counter()
+    o.run_steps(1)
+    dt = time.perf_counter() - t0
+    dof = o.nu * o.n_eles * o.nf
+    return {"value": round(dof * 5 / dt / 1e9, 4), "unit": "GDOF-stage/s", "cores": threads, "kind": "port",
+            "sample": "1 RK45 step (5 residual stages) of the same 32^3 P=3 Navier-Stokes workload, dense "
+                      "reference-order operators, %d OpenMP threads, %.2f s" % (threads, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the residual would shard by element partition with face-trace halos).", file=sys.stderr)
+        sys.exit(2)
+    n, order = args.n, args.order
+    t_setup = time.perf_counter()
+    s = hifr.Solver.box(deck(order, args.steps), hifr.HEX, (n, n, n), (TWO_PI,) * 3, (0.0,) * 3)
+    s.set_state(tgv_state(s.get("pos_upts")))
+    t_setup = time.perf_counter() - t_setup
+    s.run(args.warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    total_ms, kms, kl = s.run_timed(args.steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    norms = s.norms(1)
+    if not np.all(np.isfinite(norms)):
+        raise SystemExit("non-finite residual")
+    dof = s.nu * s.n_eles * s.nf
+    stages = 5 * args.steps
+    value = dof * stages / wall / 1e9
+    grad_b, rk_b = algorithmic_bytes(order)
+    avg_rk_ms = kms[2] / kl[2]
+    avg_grad_ms = kms[1] / kl[1] if kl[1] else 0.0
+    achieved = rk_b * s.n_eles / (avg_rk_ms * 1e-3) / 1e9
+    stage_bytes = (grad_b + rk_b) * s.n_eles
+    rec = {
+        "metric": METRIC, "value": round(value, 4), "unit": "GDOF-stage/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3-D Navier-Stokes Taylor-Green (Mach 0.1, Re 1600), %d^3 periodic hexes, P=%d, "
+                               "Rusanov+LDG, RK45 dt=1e-3, FP64" % (n, order),
+                   "n_eles": s.n_eles, "order": order, "dof": dof, "stages_per_step": 5},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "fr_tensor_kernel ADVANCE (flux, divergence, RK update, next traces)",
+                     "bytes_per_launch": rk_b * s.n_eles, "avg_launch_ms": round(avg_rk_ms, 5)},
+        "stage": {"avg_grad_kernel_ms": round(avg_grad_ms, 5), "avg_rk_kernel_ms": round(avg_rk_ms, 5),
+                  "algorithmic_bytes": stage_bytes,
+                  "achieved_GBs": round(stage_bytes / ((avg_rk_ms + avg_grad_ms) * 1e-3) / 1e9, 1),
+                  "event_total_ms": round(total_ms, 3)},
+        "residual_L1": [float("%.6e" % v) for v in norms],
+        "setup_s": round(t_setup, 2),
+    }
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline(s, order)
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

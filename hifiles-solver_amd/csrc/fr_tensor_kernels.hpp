@@ -1,0 +1,468 @@
+// Element-block kernels of the FR residual for tensor-product elements (quads,
+// hexes) on gfx950.
+//
+// One thread per solution point, EPB elements per workgroup.  The dense element
+// operators of the reference (opp_0..opp_6, src/eles.cpp:3620-4052) are applied in
+// sum-factorised form from their 1-D factors: every flux point lies on one line of
+// solution points, so interpolation to a face, the reference-space derivative and
+// the VCJH correction lift are length-(P+1) sums along that line, read from LDS.
+//
+// A residual needs the neighbour traces of u (and, for Navier-Stokes, of the
+// corrected gradient), so one RK stage is split at the two points where data must
+// cross elements:
+//   TRACE_U    u -> u at the element's flux points                  (first stage only)
+//   TRACE_GRAD u, neighbour u_f -> corrected physical grad u at flux points
+//   RESID      u, neighbour u_f (+ grad u_f) -> div of the continuous flux
+//   ADVANCE    RESID + RK update (src/eles.cpp:1091-1188) + traces of the new u,
+//              so the next stage starts from TRACE_GRAD.
+// Each kernel recomputes what it needs element-locally instead of storing
+// intermediates: the minimum traffic schedule (u read twice per stage, traces
+// written once and read by the neighbour once).
+//
+// Layouts (first index fastest): u, res, div (Nu,N,nf); u_f (Nfp,N,nf);
+// grad u_f (Nfp,N,nf,d) (the reference's grad_disu_fpts); detJ (Nu,N); JGinv
+// component-major (d*d, Nu*N).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fr_physics.hpp"
+
+namespace hifr {
+
+constexpr int MAXN1 = 8;
+
+// 1-D factors of the element operators (Lagrange basis on the solution points).
+struct Ops1D {
+  double lL[MAXN1], lR[MAXN1];  // l_m(-1), l_m(+1): interpolation to the faces
+  double D[MAXN1][MAXN1];       // D[a][m] = l'_m(xi_a): differentiation
+  double gL[MAXN1], gR[MAXN1];  // g_L'(xi_a), g_R'(xi_a): VCJH correction derivatives
+};
+
+enum { MODE_TRACE_U = 0, MODE_TRACE_GRAD = 1, MODE_RESID = 2, MODE_ADVANCE = 3 };
+
+struct StageArgs {
+  int N;
+  double* u;             // state (read; written by ADVANCE)
+  double* res;           // RK register (ADVANCE)
+  double* div;           // div_tconf_upts output (RESID; ADVANCE if non-null)
+  const double* uf_in;   // traces of the current state
+  double* uf_out;        // traces written (TRACE_U, ADVANCE)
+  const double* gf_in;   // gradient traces (RESID/ADVANCE, viscous)
+  double* gf_out;        // gradient traces written (TRACE_GRAD)
+  const double* detj;
+  const double* jg;
+  const int* nbr;        // neighbour trace index per element flux point; bit 31: LDG switch -
+  const double* fgeo;    // own-side outward normal (d) and tdA, component-major
+  double rk_a, rk_b, dt;
+  int adv_type;          // 3 RK45 stage, 0 forward Euler
+  Ops1D ops;
+  Phys ph;
+};
+
+template <int P, int ND, int EPB>
+struct TP {
+  static constexpr int n = P + 1;
+  static constexpr int NU = ND == 3 ? n * n * n : n * n;
+  static constexpr int NFPF = ND == 3 ? n * n : n;
+  static constexpr int NFP = 2 * ND * NFPF;
+  static constexpr int NF = ND + 2;
+  static constexpr int BLOCK = ((EPB * NU + 63) / 64) * 64;
+  static constexpr int stride(int dir) { return dir == 0 ? 1 : (dir == 1 ? n : n * n); }
+};
+
+// Flux point -> (direction of its line, side 0 = -1 end / 1 = +1 end, first
+// solution point of the line).  Face layouts of src/eles_hexas.cpp:266-324 and
+// src/eles_quads.cpp:259-300.
+template <int P, int ND>
+__device__ __forceinline__ void face_line(int fp, int& dir, int& side, int& base) {
+  constexpr int n = P + 1;
+  if constexpr (ND == 3) {
+    const int f = fp / (n * n), q = fp - f * n * n, k = q % n, j = q / n;
+    switch (f) {
+      case 0: dir = 2; side = 0; base = (P - k) + n * j; break;
+      case 1: dir = 1; side = 0; base = k + n * n * j; break;
+      case 2: dir = 0; side = 1; base = n * k + n * n * j; break;
+      case 3: dir = 1; side = 1; base = (P - k) + n * n * j; break;
+      case 4: dir = 0; side = 0; base = n * (P - k) + n * n * j; break;
+      default: dir = 2; side = 1; base = k + n * j; break;
+    }
+  } else {
+    const int f = fp / n, j = fp - f * n;
+    switch (f) {
+      case 0: dir = 1; side = 0; base = j; break;
+      case 1: dir = 0; side = 1; base = n * j; break;
+      case 2: dir = 1; side = 1; base = P - j; break;
+      default: dir = 0; side = 0; base = n * (P - j); break;
+    }
+  }
+}
+
+// Solution point p, direction dir -> position along its line, first point of the
+// line, and the flux points at the -1 and +1 ends.
+template <int P, int ND>
+__device__ __forceinline__ void line_of(int p, int dir, int& pos, int& base, int& fpL, int& fpR) {
+  constexpr int n = P + 1;
+  const int a = p % n, b = (p / n) % n;
+  if constexpr (ND == 3) {
+    const int c = p / (n * n);
+    if (dir == 0) { pos = a; base = p - a; fpL = 4 * n * n + (P - b) + n * c; fpR = 2 * n * n + b + n * c; }
+    else if (dir == 1) { pos = b; base = p - n * b; fpL = n * n + a + n * c; fpR = 3 * n * n + (P - a) + n * c; }
+    else { pos = c; base = p - n * n * c; fpL = (P - a) + n * b; fpR = 5 * n * n + a + n * b; }
+  } else {
+    if (dir == 0) { pos = a; base = p - a; fpL = 3 * n + P - b; fpR = n + b; }
+    else { pos = b; base = p - n * b; fpL = a; fpR = 2 * n + P - a; }
+  }
+}
+
+template <int P, int ND, int EPB, bool VISC, int MODE>
+struct SmemLayout {
+  using T = TP<P, ND, EPB>;
+  static constexpr int U = EPB * T::NF * T::NU;
+  static constexpr int Q = (MODE == MODE_TRACE_U) ? 0 : EPB * T::NFP * T::NF;
+  static constexpr int G = (VISC && MODE != MODE_TRACE_U) ? EPB * T::NF * ND * T::NU : 0;
+  static constexpr int F = (MODE == MODE_RESID || MODE == MODE_ADVANCE) ? EPB * T::NF * ND * T::NU : 0;
+  static constexpr int TOTAL = U + Q + G + F;
+};
+
+template <int P, int ND, int EPB, bool VISC, int MODE>
+__global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK))
+fr_tensor_kernel(const StageArgs a) {
+  using T = TP<P, ND, EPB>;
+  using S = SmemLayout<P, ND, EPB, VISC, MODE>;
+  constexpr int n = T::n, NU = T::NU, NFP = T::NFP, NF = T::NF, BLOCK = T::BLOCK;
+  __shared__ double smem[S::TOTAL];
+  double* s_u = smem;                 // [EPB][NF][NU]
+  double* s_q = s_u + S::U;           // [EPB][NFP][NF]   delta (LDG) then jump
+  double* s_g = s_q + S::Q;           // [EPB][NF*ND][NU] physical gradient
+  double* s_f = s_g + S::G;           // [EPB][NF*ND][NU] transformed flux
+
+  const int N = a.N;
+  const long NN = (long)N;
+  const int tid = threadIdx.x;
+  const int el = tid / NU;
+  const int p = tid - el * NU;
+  const int e = blockIdx.x * EPB + el;
+  const bool active = (tid < EPB * NU) && (e < N);
+  const Ops1D& ops = a.ops;
+
+  // ---- load u ----
+  double u_own[NF];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      u_own[k] = a.u[p + NU * (e + NN * k)];
+      s_u[(el * NF + k) * NU + p] = u_own[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- traces of u only (start of the first stage) ----
+  if constexpr (MODE == MODE_TRACE_U) {
+    for (int task = tid; task < EPB * NFP; task += BLOCK) {
+      const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+      if (te >= N) continue;
+      int dir, side, base;
+      face_line<P, ND>(fp, dir, side, base);
+      const int st = T::stride(dir);
+      const double* lw = side ? ops.lR : ops.lL;
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        double v = 0.0;
+#pragma unroll
+        for (int m = 0; m < n; m++) v += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
+        a.uf_out[fp + NFP * (te + NN * k)] = v;
+      }
+    }
+    return;
+  } else {
+    // ---- LDG common solution -> delta = u_c - u_own at each flux point
+    //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346) ----
+    if constexpr (VISC) {
+      for (int task = tid; task < EPB * NFP; task += BLOCK) {
+        const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+        if (te >= N) continue;
+        int dir, side, base;
+        face_line<P, ND>(fp, dir, side, base);
+        const int st = T::stride(dir);
+        const double* lw = side ? ops.lR : ops.lL;
+        int nb = a.nbr[fp + NFP * te];
+        const double pen = nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact;
+        nb &= 0x7fffffff;
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          double uo = 0.0;
+#pragma unroll
+          for (int m = 0; m < n; m++) uo += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
+          const double un = a.uf_in[nb + NFP * NN * k];
+          const double uc = 0.5 * (uo + un) - pen * (uo - un);
+          s_q[(tl * NFP + fp) * NF + k] = uc - uo;
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- per solution point: corrected physical gradient, flux, transformed flux ----
+    if (active) {
+      const long pe = p + NU * (long)e;
+      double Jg[ND][ND];
+#pragma unroll
+      for (int l = 0; l < ND; l++)
+#pragma unroll
+        for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * NU + pe];
+      ViscT<ND> vt;
+      if constexpr (VISC) {
+        const double inv_detj = 1.0 / a.detj[pe];
+        int pos[ND], lb[ND], fL[ND], fR[ND];
+#pragma unroll
+        for (int d = 0; d < ND; d++) line_of<P, ND>(p, d, pos[d], lb[d], fL[d], fR[d]);
+        double gp[NF][ND];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          double gr[ND];
+#pragma unroll
+          for (int d = 0; d < ND; d++) {
+            const int st = T::stride(d);
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < n; m++) v += ops.D[pos[d]][m] * s_u[(el * NF + k) * NU + lb[d] + m * st];
+            v += ops.gL[pos[d]] * s_q[(el * NFP + fL[d]) * NF + k] + ops.gR[pos[d]] * s_q[(el * NFP + fR[d]) * NF + k];
+            gr[d] = v;
+          }
+          // reference -> physical (src/eles.cpp:1815-1870): d/dx_m = sum_d (JGinv(d,m)/J) d/dr_d
+#pragma unroll
+          for (int m = 0; m < ND; m++) {
+            double v = 0.0;
+#pragma unroll
+            for (int d = 0; d < ND; d++) v += gr[d] * (Jg[d][m] * inv_detj);
+            gp[k][m] = v;
+            s_g[(el * NF * ND + k * ND + m) * NU + p] = v;
+          }
+        }
+        if constexpr (MODE != MODE_TRACE_GRAD) visc_tensor<ND>(u_own, gp, a.ph, vt);
+      }
+      if constexpr (MODE != MODE_TRACE_GRAD) {
+        // transformed flux tdisf(k,l) = sum_m JGinv(l,m) F(k,m)  (src/eles.cpp:1397-1405, 2437-2444)
+#pragma unroll
+        for (int l = 0; l < ND; l++) {
+          double fi[NF];
+          inv_flux_dot<ND>(u_own, Jg[l], a.ph.gamma, fi);
+          if constexpr (VISC) {
+            double fv[NF];
+            visc_flux_dot<ND>(vt, Jg[l], fv);
+#pragma unroll
+            for (int k = 0; k < NF; k++) fi[k] += fv[k];
+          }
+#pragma unroll
+          for (int k = 0; k < NF; k++) s_f[(el * NF * ND + k * ND + l) * NU + p] = fi[k];
+        }
+      }
+    }
+    __syncthreads();
+
+    if constexpr (MODE == MODE_TRACE_GRAD) {
+      // ---- corrected gradient at the flux points (extrapolate_corrected_gradient) ----
+      for (int task = tid; task < EPB * NFP; task += BLOCK) {
+        const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+        if (te >= N) continue;
+        int dir, side, base;
+        face_line<P, ND>(fp, dir, side, base);
+        const int st = T::stride(dir);
+        const double* lw = side ? ops.lR : ops.lL;
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+#pragma unroll
+          for (int x = 0; x < ND; x++) {
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < n; m++) v += lw[m] * s_g[(tl * NF * ND + k * ND + x) * NU + base + m * st];
+            a.gf_out[fp + NFP * (te + NN * (k + NF * x))] = v;
+          }
+      }
+      return;
+    } else {
+      // ---- volume divergence (calculate_divergence) ----
+      double div[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) div[k] = 0.0;
+      if (active) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+          int pos, lb, fpL, fpR;
+          line_of<P, ND>(p, d, pos, lb, fpL, fpR);
+          const int st = T::stride(d);
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < n; m++) v += ops.D[pos][m] * s_f[(el * NF * ND + k * ND + d) * NU + lb + m * st];
+            div[k] += v;
+          }
+        }
+      }
+
+      // ---- own-side viscous normal flux at the flux points, F_v(u_f, grad u_f).n,
+      //      kept in s_q until the interface loop below consumes it ----
+      if constexpr (VISC) {
+        for (int task = tid; task < EPB * NFP; task += BLOCK) {
+          const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+          if (te >= N) continue;
+          int dir, side, base;
+          face_line<P, ND>(fp, dir, side, base);
+          const int st = T::stride(dir);
+          const double* lw = side ? ops.lR : ops.lL;
+          const long fe = fp + NFP * (long)te;
+          double nrm[ND];
+#pragma unroll
+          for (int l = 0; l < ND; l++) nrm[l] = a.fgeo[l * NN * NFP + fe];
+          double uo[NF], g[NF][ND];
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < n; m++) v += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
+            uo[k] = v;
+#pragma unroll
+            for (int c = 0; c < ND; c++) {
+              double w = 0.0;
+#pragma unroll
+              for (int q = 0; q < n; q++) w += lw[q] * s_g[(tl * NF * ND + k * ND + c) * NU + base + q * st];
+              g[k][c] = w;
+            }
+          }
+          ViscT<ND> t;
+          visc_tensor<ND>(uo, g, a.ph, t);
+          double fa[NF];
+          visc_flux_dot<ND>(t, nrm, fa);
+#pragma unroll
+          for (int k = 0; k < NF; k++) s_q[(tl * NFP + fp) * NF + k] = fa[k];
+        }
+      }
+
+      // ---- interface fluxes and jumps at the flux points
+      //      (common_invFlux/common_viscFlux, extrapolate_totalFlux, daxpy) ----
+      for (int task = tid; task < EPB * NFP; task += BLOCK) {
+        const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+        if (te >= N) continue;
+        int dir, side, base;
+        face_line<P, ND>(fp, dir, side, base);
+        const int st = T::stride(dir);
+        const double* lw = side ? ops.lR : ops.lL;
+        const long fe = fp + NFP * (long)te;
+        int nb = a.nbr[fe];
+        const double pen = nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact;
+        nb &= 0x7fffffff;
+        double nrm[ND];
+#pragma unroll
+        for (int l = 0; l < ND; l++) nrm[l] = a.fgeo[l * NN * NFP + fe];
+        const double tdA = a.fgeo[ND * NN * NFP + fe];
+        double uo[NF], un[NF], fn[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          double v = 0.0;
+#pragma unroll
+          for (int m = 0; m < n; m++) v += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
+          uo[k] = v;
+          un[k] = a.uf_in[nb + NFP * NN * k];
+        }
+        rusanov<ND>(uo, un, nrm, a.ph.gamma, fn);
+        double jump[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) jump[k] = fn[k] * tdA;
+        if constexpr (VISC) {
+          double nn = 0.0;
+#pragma unroll
+          for (int l = 0; l < ND; l++) nn += nrm[l] * nrm[l];
+          double fa[NF], fb[NF];
+#pragma unroll
+          for (int k = 0; k < NF; k++) fa[k] = s_q[(tl * NFP + fp) * NF + k];
+          {
+            double g[NF][ND];
+#pragma unroll
+            for (int k = 0; k < NF; k++)
+#pragma unroll
+              for (int m = 0; m < ND; m++) g[k][m] = a.gf_in[nb + NFP * NN * (k + NF * m)];
+            ViscT<ND> t;
+            visc_tensor<ND>(un, g, a.ph, t);
+            visc_flux_dot<ND>(t, nrm, fb);
+          }
+#pragma unroll
+          for (int k = 0; k < NF; k++)
+            jump[k] += ldg_normal<ND>(fa[k], fb[k], uo[k], un[k], nn, pen, a.ph.tau) * tdA;
+        }
+        // minus the normal transformed discontinuous flux (opp_1 = opp_0 o tnorm)
+        const double sgn = side ? 1.0 : -1.0;
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          double v = 0.0;
+#pragma unroll
+          for (int m = 0; m < n; m++) v += lw[m] * s_f[(tl * NF * ND + k * ND + dir) * NU + base + m * st];
+          s_q[(tl * NFP + fp) * NF + k] = jump[k] - sgn * v;
+        }
+      }
+      __syncthreads();
+
+      // ---- correction lift (calculate_corrected_divergence) and output ----
+      if (active) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+          int pos, lb, fpL, fpR;
+          line_of<P, ND>(p, d, pos, lb, fpL, fpR);
+#pragma unroll
+          for (int k = 0; k < NF; k++)
+            div[k] += ops.gR[pos] * s_q[(el * NFP + fpR) * NF + k] - ops.gL[pos] * s_q[(el * NFP + fpL) * NF + k];
+        }
+        const long pe = p + NU * (long)e;
+        if (MODE == MODE_RESID || a.div) {
+#pragma unroll
+          for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = div[k];
+        }
+        if constexpr (MODE == MODE_ADVANCE) {
+          const double detj = a.detj[pe];
+          if (a.adv_type == 3) {
+            // low-storage RK45 stage (src/eles.cpp:1155-1178)
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+              const long idx = pe + NU * NN * k;
+              const double rhs = -div[k] / detj + a.ph.const_src;
+              const double r = a.rk_a * a.res[idx] + a.dt * rhs;
+              a.res[idx] = r;
+              const double unew = s_u[(el * NF + k) * NU + p] + a.rk_b * r;
+              a.u[idx] = unew;
+              s_u[(el * NF + k) * NU + p] = unew;
+            }
+          } else {
+            // forward Euler (src/eles.cpp:1067-1079)
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+              const long idx = pe + NU * NN * k;
+              const double unew = s_u[(el * NF + k) * NU + p] - a.dt * (div[k] / detj - a.ph.const_src);
+              a.u[idx] = unew;
+              s_u[(el * NF + k) * NU + p] = unew;
+            }
+          }
+        }
+      }
+      if constexpr (MODE == MODE_ADVANCE) {
+        __syncthreads();
+        // ---- traces of the new state for the next stage ----
+        for (int task = tid; task < EPB * NFP; task += BLOCK) {
+          const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+          if (te >= N) continue;
+          int dir, side, base;
+          face_line<P, ND>(fp, dir, side, base);
+          const int st = T::stride(dir);
+          const double* lw = side ? ops.lR : ops.lL;
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < n; m++) v += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
+            a.uf_out[fp + NFP * (te + NN * k)] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+}  // namespace hifr

@@ -1,0 +1,207 @@
+"""Python binding (ctypes) of libhifr.so: the gfx950 FR residual / RK library.
+
+`Solver` wraps the host solver API of include/hifr_solver.h (deck + mesh ->
+elements, interfaces, device context); `ctx()` exposes the device ABI of
+include/hifr.h for timing.  Arrays are returned as Fortran-ordered numpy arrays
+with the reference's index order, e.g. u[j, i, k] = disu_upts(j, i, k).
+
+The library must be built (make -C hifiles-solver_amd); there is no fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhifr.so")
+_LIB = None
+
+HEX, QUAD = 4, 1
+
+_GET = {"disu_upts": 0, "div_tconf_upts": 1, "disu_fpts": 2, "grad_disu_fpts": 3, "detjac_upts": 4,
+        "pos_upts": 5, "opp_0": 6, "opp_3": 7, "opp_4": 8, "JGinv_upts": 9, "tdA_fpts": 10,
+        "norm_fpts": 11, "xv": 12, "c2v": 13, "rk_register": 14}
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libhifr.so is not built: make -C hifiles-solver_amd")
+        L = C.CDLL(LIB_PATH)
+        vp, dp, ip, cp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_char_p
+        L.hifr_solver_from_deck.argtypes = [cp, C.POINTER(vp), cp, C.c_int]
+        L.hifr_solver_from_arrays.argtypes = [cp, C.c_int, C.c_int, dp, C.c_int, ip, C.POINTER(vp), cp, C.c_int]
+        L.hifr_solver_box.argtypes = [cp, C.c_int, ip, dp, dp, C.c_int, C.POINTER(vp), cp, C.c_int]
+        L.hifr_solver_destroy.argtypes = [vp]
+        L.hifr_solver_sizes.argtypes = [vp, ip]
+        L.hifr_solver_get.argtypes = [vp, C.c_int, dp]
+        L.hifr_solver_get.restype = C.c_long
+        L.hifr_solver_set_state.argtypes = [vp, dp]
+        L.hifr_solver_interfaces.argtypes = [vp, ip, ip]
+        L.hifr_solver_interfaces.restype = C.c_long
+        for f in ("hifr_solver_residual",):
+            getattr(L, f).argtypes = [vp]
+        L.hifr_solver_advance.argtypes = [vp, C.c_int]
+        L.hifr_solver_run.argtypes = [vp, C.c_int]
+        L.hifr_solver_norms.argtypes = [vp, C.c_int, dp]
+        L.hifr_solver_write_restart.argtypes = [vp, cp]
+        L.hifr_solver_ctx.argtypes = [vp]
+        L.hifr_solver_ctx.restype = vp
+        L.hifr_solver_dt.argtypes = [vp]
+        L.hifr_solver_dt.restype = C.c_double
+        L.hifr_solver_adv_type.argtypes = [vp]
+        L.hifr_run_timed.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp, ip]
+        L.hifr_synchronize.argtypes = [vp]
+        L.hifr_run.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.hifr_status_string.argtypes = [C.c_int]
+        L.hifr_status_string.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def deck_text(**kw):
+    """Input deck text (reference key names) with the keys the solver requires."""
+    base = {"equation": 0, "viscous": 0, "order": 3, "mesh_file": "none.neu", "n_steps": 1,
+            "riemann_solve_type": 0, "vis_riemann_solve_type": 0, "adv_type": 3, "dt_type": 0,
+            "dt": 1e-3, "ic_form": 1}
+    base.update(kw)
+    lines = []
+    for k, v in base.items():
+        if isinstance(v, float):
+            lines.append("%s %.17g" % (k, v))
+        else:
+            lines.append("%s %s" % (k, v))
+    return "\n".join(lines) + "\n"
+
+
+class Solver:
+    """Host solver + device context for one mesh of one element type."""
+
+    def __init__(self, handle):
+        self._h = handle
+        sz = (C.c_int * 6)()
+        lib().hifr_solver_sizes(self._h, sz)
+        self.nu, self.nfp, self.nf, self.nd, self.n_eles, self.n_inters = list(sz)
+
+    @staticmethod
+    def _check(rc, err):
+        if rc != 0:
+            raise RuntimeError(err.value.decode(errors="replace"))
+
+    @classmethod
+    def from_deck(cls, path):
+        h, err = C.c_void_p(), C.create_string_buffer(1024)
+        cls._check(lib().hifr_solver_from_deck(path.encode(), C.byref(h), err, 1024), err)
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, deck, ele_type, xv, c2v):
+        xv = np.ascontiguousarray(xv, dtype=np.float64)
+        c2v = np.ascontiguousarray(c2v, dtype=np.int32)
+        h, err = C.c_void_p(), C.create_string_buffer(1024)
+        rc = lib().hifr_solver_from_arrays(deck.encode(), int(ele_type), xv.shape[0], _dp(xv), c2v.shape[0],
+                                           c2v.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h), err, 1024)
+        cls._check(rc, err)
+        return cls(h)
+
+    @classmethod
+    def box(cls, deck, ele_type, n, length, origin, digits=0):
+        nd = 3 if ele_type == HEX else 2
+        ni = (C.c_int * 3)(*(list(n) + [1] * (3 - len(n))))
+        ll = (C.c_double * 3)(*(list(length) + [1.0] * (3 - len(length))))
+        oo = (C.c_double * 3)(*(list(origin) + [0.0] * (3 - len(origin))))
+        h, err = C.c_void_p(), C.create_string_buffer(1024)
+        rc = lib().hifr_solver_box(deck.encode(), int(ele_type), ni, ll, oo, int(digits), C.byref(h), err, 1024)
+        cls._check(rc, err)
+        del nd
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            lib().hifr_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape_of(self, name):
+        nu, nfp, N, nf, nd = self.nu, self.nfp, self.n_eles, self.nf, self.nd
+        return {"disu_upts": (nu, N, nf), "div_tconf_upts": (nu, N, nf), "rk_register": (nu, N, nf),
+                "disu_fpts": (nfp, N, nf), "grad_disu_fpts": (nfp, N, nf, nd), "detjac_upts": (nu, N),
+                "pos_upts": (nu, N, nd), "opp_0": (nfp, nu), "opp_3": (nu, nfp), "opp_4": (nu, nu, nd),
+                "JGinv_upts": (nd, nd, nu, N), "tdA_fpts": (nfp, N), "norm_fpts": (nfp, N, nd)}.get(name)
+
+    def get(self, name):
+        n = lib().hifr_solver_get(self._h, _GET[name], None)
+        if n < 0:
+            raise RuntimeError("hifr_solver_get(%s) failed" % name)
+        out = np.zeros(n, dtype=np.float64)
+        if lib().hifr_solver_get(self._h, _GET[name], _dp(out)) != n:
+            raise RuntimeError("hifr_solver_get(%s) failed" % name)
+        shp = self.shape_of(name)
+        return out.reshape(shp, order="F") if shp else out
+
+    def set_state(self, u):
+        flat = np.ascontiguousarray(np.asarray(u, dtype=np.float64).reshape(-1, order="F"))
+        if flat.size != self.nu * self.n_eles * self.nf:
+            raise ValueError("state size")
+        if lib().hifr_solver_set_state(self._h, _dp(flat)) != 0:
+            raise RuntimeError("set_state failed")
+
+    def interfaces(self):
+        n = lib().hifr_solver_interfaces(self._h, None, None)
+        left = np.zeros(n, dtype=np.int32)
+        right = np.zeros(n, dtype=np.int32)
+        lib().hifr_solver_interfaces(self._h, left.ctypes.data_as(C.POINTER(C.c_int)),
+                                     right.ctypes.data_as(C.POINTER(C.c_int)))
+        return left, right
+
+    def residual(self):
+        if lib().hifr_solver_residual(self._h) != 0:
+            raise RuntimeError("residual failed")
+        return self.get("div_tconf_upts")
+
+    def advance(self, stage):
+        if lib().hifr_solver_advance(self._h, int(stage)) != 0:
+            raise RuntimeError("advance failed")
+
+    def run(self, n_steps):
+        if lib().hifr_solver_run(self._h, int(n_steps)) != 0:
+            raise RuntimeError("run failed")
+
+    def norms(self, norm_type=1):
+        out = np.zeros(5)
+        rc = lib().hifr_solver_norms(self._h, int(norm_type), _dp(out))
+        if rc != 0:
+            raise RuntimeError("norms failed (%d)" % rc)
+        return out[: self.nf]
+
+    def ctx(self):
+        return lib().hifr_solver_ctx(self._h)
+
+    def dt(self):
+        return lib().hifr_solver_dt(self._h)
+
+    def adv_type(self):
+        return lib().hifr_solver_adv_type(self._h)
+
+    def run_timed(self, n_steps):
+        """(total_ms, kernel_ms[3], launches[3]) for n_steps with per-kernel HIP events."""
+        tot = C.c_double()
+        km = (C.c_double * 3)()
+        kl = (C.c_int * 3)()
+        rc = lib().hifr_run_timed(self.ctx(), int(n_steps), self.adv_type(), self.dt(), C.byref(tot), km, kl)
+        if rc != 0:
+            raise RuntimeError("hifr_run_timed: %s" % lib().hifr_status_string(rc).decode())
+        return tot.value, list(km), list(kl)
+
+    def synchronize(self):
+        lib().hifr_synchronize(self.ctx())

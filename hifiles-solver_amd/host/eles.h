@@ -1,0 +1,90 @@
+// Element classes of the host solver: the reference's eles / eles_quads / eles_hexas
+// surface (include/eles.h, src/eles.cpp, src/eles_quads.cpp, src/eles_hexas.cpp)
+// reduced to what the residual needs.  They build the solution/flux point sets,
+// the dense element operators opp_0, opp_3, opp_4 in the reference layouts and the
+// static-mesh metrics, hand them to the device context (include/hifr.h) and then
+// drive the residual and RK update through it.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/hifr.h"
+#include "input.h"
+
+namespace hifr_host {
+
+class eles {
+ public:
+  virtual ~eles();
+
+  // src/eles.cpp:84-405 (storage sizes, operators) for n_eles elements
+  std::string setup(const input& in, int n_eles);
+  void set_shape_node(int spt, int ele, const double* pos);
+  // src/eles.cpp:4557-4847
+  void set_transforms();
+  // src/eles.cpp:5744-5757
+  void calc_pos(const double* loc, int ele, double* pos) const;
+
+  // device side (replaces mv_all_cpu_gpu, src/eles.cpp:775)
+  std::string mv_all_cpu_gpu(const input& in);
+  hifr_ctx* ctx = nullptr;
+
+  int ele_type = 0, n_dims = 0, n_fields = 0, order = 0;
+  int n_upts_per_ele = 0, n_fpts_per_ele = 0, n_inters_per_ele = 0, n_fpts_per_inter = 0;
+  int n_eles = 0, n_spts_per_ele = 0;
+  std::vector<double> loc_1d_upts;
+  std::vector<double> loc_upts, tloc_fpts, tnorm_fpts;  // (d, Nu), (d, Nfp), (d, Nfp)
+  std::vector<double> opp_0, opp_3, opp_4;              // (Nfp,Nu), (Nu,Nfp), (Nu,Nu,d)
+  std::vector<double> shape;                            // (d, nspts, N)
+  std::vector<double> detjac_upts, JGinv_upts, tdA_fpts, norm_fpts, pos_upts, pos_fpts;
+  double eta = 0.0;
+
+  // local faces as shape-node lists (for connectivity)
+  virtual const int* face_nodes(int face, int& n) const = 0;
+
+ protected:
+  virtual void set_points() = 0;
+  virtual double eval_nodal_basis(int idx, const double* loc) const = 0;
+  virtual double eval_d_nodal_basis(int idx, int cpnt, const double* loc) const = 0;
+  virtual double eval_div_vcjh_basis(int idx, const double* loc) const = 0;
+  virtual void eval_shape(const double* loc, double* N, double* dN) const = 0;
+  void set_opp_0();
+  void set_opp_3();
+  void set_opp_4();
+};
+
+class eles_quads : public eles {
+ public:
+  const int* face_nodes(int face, int& n) const override;
+
+ protected:
+  void set_points() override;
+  double eval_nodal_basis(int idx, const double* loc) const override;
+  double eval_d_nodal_basis(int idx, int cpnt, const double* loc) const override;
+  double eval_div_vcjh_basis(int idx, const double* loc) const override;
+  void eval_shape(const double* loc, double* N, double* dN) const override;
+};
+
+class eles_hexas : public eles {
+ public:
+  const int* face_nodes(int face, int& n) const override;
+
+ protected:
+  void set_points() override;
+  double eval_nodal_basis(int idx, const double* loc) const override;
+  double eval_d_nodal_basis(int idx, int cpnt, const double* loc) const override;
+  double eval_div_vcjh_basis(int idx, const double* loc) const override;
+  void eval_shape(const double* loc, double* N, double* dN) const override;
+};
+
+// 1-D bases, src/funcs.cpp:57-250, 1503-1546
+double eval_lagrange(double r, int mode, const std::vector<double>& pts);
+double eval_d_lagrange(double r, int mode, const std::vector<double>& pts);
+double eval_legendre(double r, int mode);
+double eval_d_legendre(double r, int mode);
+double eval_d_vcjh_1d(double r, int mode, int order, double eta);
+double compute_eta(int scheme, int order);
+// 1-D Gauss points as tabulated by data/loc_1d_gauss_pts.dat (orders 0..6)
+std::vector<double> gauss_points(int order);
+
+}  // namespace hifr_host

@@ -1,0 +1,280 @@
+// Mesh reader/generator and face connectivity.
+#include "geometry.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
+#include <unordered_map>
+
+namespace hifr_host {
+
+namespace {
+
+// Gambit brick / quad node order -> internal shape-node position
+// (src/geometry.cpp:1800-1855)
+const int kHexPos[8] = {0, 2, 4, 6, 1, 3, 5, 7};
+const int kQuadPos[4] = {0, 1, 3, 2};
+
+}  // namespace
+
+std::string read_gambit(const std::string& path, mesh& m) {
+  std::ifstream f(path);
+  if (!f) return "Unable to open mesh file " + path;
+  std::string line;
+  for (int i = 0; i < 6; i++) std::getline(f, line);
+  int nn = 0, ne = 0, ngrp = 0, nbc = 0, nd = 0;
+  f >> nn >> ne >> ngrp >> nbc >> nd;
+  if (nd != 2 && nd != 3) return "Invalid mesh dimensionality";
+  std::getline(f, line);
+  std::getline(f, line);
+  std::getline(f, line);
+  m.n_dims = nd;
+  m.n_verts = nn;
+  m.xv.assign((size_t)nn * nd, 0.0);
+  for (int i = 0; i < nn; i++) {
+    std::getline(f, line);
+    std::istringstream ss(line);
+    int id;
+    ss >> id;
+    for (int k = 0; k < nd; k++) ss >> m.xv[(size_t)(id - 1) * nd + k];
+  }
+  std::getline(f, line);  // ENDOFSECTION
+  std::getline(f, line);  // ELEMENTS/CELLS
+  m.n_eles = ne;
+  m.ele_type = 0;
+  for (int e = 0; e < ne; e++) {
+    int id, gtype, nv;
+    f >> id >> gtype >> nv;
+    int et = gtype == 4 ? 4 : (gtype == 2 ? 1 : -1);
+    if (et < 0 || (et == 4 && nv != 8) || (et == 1 && nv != 4))
+      return "only linear quads and 8-node hexes are supported";
+    if (m.ele_type == 0) {
+      m.ele_type = et;
+      m.n_spts = nv;
+      m.c2v.assign((size_t)ne * nv, 0);
+    } else if (m.ele_type != et) {
+      return "mixed element types are not supported";
+    }
+    const int* pos = et == 4 ? kHexPos : kQuadPos;
+    for (int k = 0; k < nv; k++) {
+      int v;
+      f >> v;
+      m.c2v[(size_t)(id - 1) * nv + pos[k]] = v - 1;
+    }
+  }
+  if (!f) return "error reading Gambit elements";
+  return "";
+}
+
+void make_box(int ele_type, const int* nxyz, const double* l, const double* o, int digits, mesh& m) {
+  const int nd = ele_type == 4 ? 3 : 2;
+  m.n_dims = nd;
+  m.ele_type = ele_type;
+  m.n_spts = ele_type == 4 ? 8 : 4;
+  int nv[3] = {1, 1, 1};
+  for (int d = 0; d < nd; d++) nv[d] = nxyz[d] + 1;
+  std::vector<double> c[3];
+  for (int d = 0; d < nd; d++) {
+    c[d].resize(nv[d]);
+    for (int i = 0; i < nv[d]; i++) {
+      double x = o[d] + l[d] * i / nxyz[d];
+      if (digits > 0) {
+        char buf[64];
+        std::snprintf(buf, sizeof buf, "%.*e", digits - 1, x);
+        x = std::strtod(buf, nullptr);
+      }
+      c[d][i] = x;
+    }
+  }
+  m.n_verts = nv[0] * nv[1] * nv[2];
+  m.xv.assign((size_t)m.n_verts * nd, 0.0);
+  for (int k = 0; k < nv[2]; k++)
+    for (int j = 0; j < nv[1]; j++)
+      for (int i = 0; i < nv[0]; i++) {
+        const size_t v = i + (size_t)nv[0] * (j + (size_t)nv[1] * k);
+        m.xv[v * nd + 0] = c[0][i];
+        m.xv[v * nd + 1] = c[1][j];
+        if (nd == 3) m.xv[v * nd + 2] = c[2][k];
+      }
+  auto vid = [&](int i, int j, int k) { return i + nv[0] * (j + nv[1] * k); };
+  const int nz = nd == 3 ? nxyz[2] : 1;
+  m.n_eles = nxyz[0] * nxyz[1] * nz;
+  m.c2v.assign((size_t)m.n_eles * m.n_spts, 0);
+  size_t e = 0;
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < nxyz[1]; j++)
+      for (int i = 0; i < nxyz[0]; i++, e++) {
+        int* c2v = &m.c2v[e * m.n_spts];
+        if (nd == 3) {
+          // Gambit brick order (x fastest), then the reader's permutation
+          const int g[8] = {vid(i, j, k),     vid(i + 1, j, k),     vid(i, j + 1, k),     vid(i + 1, j + 1, k),
+                            vid(i, j, k + 1), vid(i + 1, j, k + 1), vid(i, j + 1, k + 1), vid(i + 1, j + 1, k + 1)};
+          for (int q = 0; q < 8; q++) c2v[kHexPos[q]] = g[q];
+        } else {
+          const int g[4] = {vid(i, j, 0), vid(i + 1, j, 0), vid(i + 1, j + 1, 0), vid(i, j + 1, 0)};
+          for (int q = 0; q < 4; q++) c2v[kQuadPos[q]] = g[q];
+        }
+      }
+}
+
+namespace {
+
+struct FaceKey {
+  std::array<int, 4> v;
+  bool operator==(const FaceKey& o) const { return v == o.v; }
+};
+struct FaceKeyHash {
+  size_t operator()(const FaceKey& k) const {
+    size_t h = 1469598103934665603ull;
+    for (int x : k.v) h = (h ^ (size_t)(unsigned)x) * 1099511628211ull;
+    return h;
+  }
+};
+
+}  // namespace
+
+std::string build_interfaces(const eles& el, const mesh& m, const double* delta, int_inters& out) {
+  const int N = el.n_eles, nfaces = el.n_inters_per_ele, nd = el.n_dims, nfp = el.n_fpts_per_ele,
+            nfpf = el.n_fpts_per_inter, ns = m.n_spts;
+  const long nrec = (long)N * nfaces;
+  std::vector<long> partner(nrec, -1);
+  {
+    std::unordered_map<FaceKey, long, FaceKeyHash> first;
+    first.reserve(nrec);
+    for (int e = 0; e < N; e++)
+      for (int f = 0; f < nfaces; f++) {
+        int nv;
+        const int* fn = el.face_nodes(f, nv);
+        FaceKey k;
+        k.v = {-1, -1, -1, -1};
+        for (int q = 0; q < nv; q++) k.v[q] = m.c2v[(size_t)e * ns + fn[q]];
+        std::sort(k.v.begin(), k.v.begin() + nv);
+        const long id = (long)e * nfaces + f;
+        auto it = first.find(k);
+        if (it == first.end()) {
+          first.emplace(k, id);
+        } else {
+          if (partner[it->second] >= 0) return "a face is shared by more than two elements";
+          partner[it->second] = id;
+          partner[id] = it->second;
+        }
+      }
+  }
+  // centroids of the unmatched faces
+  std::vector<long> un;
+  for (long i = 0; i < nrec; i++)
+    if (partner[i] < 0) un.push_back(i);
+  auto centroid = [&](long id, double* c) {
+    const int e = (int)(id / nfaces), f = (int)(id % nfaces);
+    int nv;
+    const int* fn = el.face_nodes(f, nv);
+    for (int d = 0; d < nd; d++) c[d] = 0.0;
+    for (int q = 0; q < nv; q++)
+      for (int d = 0; d < nd; d++) c[d] += m.xv[(size_t)m.c2v[(size_t)e * ns + fn[q]] * nd + d] / nv;
+  };
+  const double tol = 1.e-6;
+  const double bin = 1.e-3;
+  std::vector<double> cen(un.size() * 3, 0.0);
+  std::unordered_map<long long, std::vector<int>> grid;
+  auto key = [&](const double* c) {
+    long long k = 0;
+    for (int d = 0; d < 3; d++) k = k * 2000003LL + (long long)std::floor((d < nd ? c[d] : 0.0) / bin);
+    return k;
+  };
+  for (size_t i = 0; i < un.size(); i++) {
+    centroid(un[i], &cen[3 * i]);
+    grid[key(&cen[3 * i])].push_back((int)i);
+  }
+  for (size_t i = 0; i < un.size(); i++) {
+    if (partner[un[i]] >= 0) continue;
+    bool found = false;
+    for (int m2 = 0; m2 < nd && !found; m2++) {
+      if (!std::isfinite(delta[m2])) continue;
+      for (int sgn = -1; sgn <= 1 && !found; sgn += 2) {
+        double t[3] = {cen[3 * i], cen[3 * i + 1], cen[3 * i + 2]};
+        t[m2] += sgn * delta[m2];
+        // search the neighbouring bins of the translated centroid
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; d++) {
+          lo[d] = d < nd ? -1 : 0;
+          hi[d] = d < nd ? 1 : 0;
+        }
+        for (int a = lo[0]; a <= hi[0] && !found; a++)
+          for (int b = lo[1]; b <= hi[1] && !found; b++)
+            for (int cc = lo[2]; cc <= hi[2] && !found; cc++) {
+              double q[3] = {t[0] + a * bin, t[1] + b * bin, t[2] + cc * bin};
+              auto it = grid.find(key(q));
+              if (it == grid.end()) continue;
+              for (int j : it->second) {
+                if ((size_t)j == i || partner[un[j]] >= 0) continue;
+                bool ok = true;
+                for (int d = 0; d < nd; d++) {
+                  const double dd = cen[3 * j + d] - cen[3 * i + d];
+                  if (d == m2) ok = ok && std::fabs(std::fabs(dd) - delta[d]) < tol;
+                  else ok = ok && std::fabs(dd) < tol;
+                }
+                if (ok) {
+                  partner[un[i]] = un[j];
+                  partner[un[j]] = un[i];
+                  found = true;
+                  break;
+                }
+              }
+            }
+      }
+    }
+  }
+  for (long i = 0; i < nrec; i++)
+    if (partner[i] < 0) {
+      char buf[160];
+      std::snprintf(buf, sizeof buf, "face %ld of element %ld has no neighbour (walls are not built)", i % nfaces,
+                    i / nfaces);
+      return buf;
+    }
+  // pair flux points by position
+  out.n_inters = (int)(nrec / 2);
+  out.left.assign((size_t)out.n_inters * nfpf, 0);
+  out.right.assign((size_t)out.n_inters * nfpf, 0);
+  const size_t NN = (size_t)N;
+  int it = 0;
+  for (long a = 0; a < nrec; a++) {
+    const long b = partner[a];
+    if (b < a) continue;
+    const int eL = (int)(a / nfaces), fL = (int)(a % nfaces), eR = (int)(b / nfaces), fR = (int)(b % nfaces);
+    double cl[3], cr[3], shift[3] = {0, 0, 0};
+    centroid(a, cl);
+    centroid(b, cr);
+    for (int d = 0; d < nd; d++) {
+      const double dd = cr[d] - cl[d];
+      if (std::isfinite(delta[d]) && std::fabs(std::fabs(dd) - delta[d]) < tol) shift[d] = dd > 0 ? delta[d] : -delta[d];
+    }
+    for (int j = 0; j < nfpf; j++) {
+      const int fl = fL * nfpf + j;
+      double pl[3];
+      for (int d = 0; d < nd; d++) pl[d] = el.pos_fpts[fl + nfp * (eL + NN * d)] + shift[d];
+      int best = -1;
+      double bd = 1e300;
+      for (int q = 0; q < nfpf; q++) {
+        const int fr = fR * nfpf + q;
+        double dd = 0.0;
+        for (int d = 0; d < nd; d++) {
+          const double t = el.pos_fpts[fr + nfp * (eR + NN * d)] - pl[d];
+          dd += t * t;
+        }
+        if (dd < bd) {
+          bd = dd;
+          best = fr;
+        }
+      }
+      out.left[(size_t)it * nfpf + j] = fl + nfp * eL;
+      out.right[(size_t)it * nfpf + j] = best + nfp * eR;
+    }
+    it++;
+  }
+  return "";
+}
+
+}  // namespace hifr_host

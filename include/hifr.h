@@ -1,0 +1,133 @@
+/*
+ * hifr.h -- C ABI of the MI355X (gfx950) flux-reconstruction residual and RK update.
+ *
+ * This is the drop-in boundary for the GPU side of HiFiLES's hot path.  In the
+ * reference the element and interface classes call C++-linkage kernel wrappers
+ * and legacy cuBLAS directly (include/cuda_kernels.h:29-118, src/eles.cpp:1313,
+ * 1470, 1566, 1644, 1648, 1738, 1922, 2025).  Here the same classes hand their
+ * operators, metrics, face tables and state to one context and call one entry
+ * point per residual / RK stage; all device memory is owned by the context.
+ *
+ * Plain C types only: host arrays are double / int32 in the reference's
+ * column-major layouts (first index fastest), as filled by eles_* and int_inters.
+ * Every call returns a hifr_status; nothing calls exit().  One context drives one
+ * element type on the current HIP device; calls on one context must not overlap.
+ */
+#ifndef HIFR_H
+#define HIFR_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  HIFR_OK = 0,
+  HIFR_ERR_ARG = 1,          /* bad argument / size / order of calls */
+  HIFR_ERR_UNSUPPORTED = 2,  /* element type, order or option not built */
+  HIFR_ERR_DEVICE = 3,       /* HIP runtime error (no device, launch failure, OOM) */
+  HIFR_ERR_NOT_TENSOR = 4,   /* tensor-product element operators lack the expected structure */
+  HIFR_ERR_STATE = 5,        /* entry point called before its inputs were set */
+  HIFR_ERR_NAN = 6           /* NaN in the residual (src/eles.cpp:1634-1638) */
+} hifr_status;
+
+/* Element types use the reference's numbering (src/eles_*.cpp setup_ele_type_specific). */
+enum { HIFR_QUAD = 1, HIFR_HEX = 4 };
+
+/* Arithmetic of the residual kernels. */
+enum { HIFR_FP64 = 0 };
+
+/* What hifr_download can copy out (reference layouts). */
+enum {
+  HIFR_DISU_UPTS = 0,       /* disu_upts(0)     (Nu,N,nf)      */
+  HIFR_RK_REGISTER = 1,     /* disu_upts(1)     (Nu,N,nf)      */
+  HIFR_DIV_TCONF_UPTS = 2,  /* div_tconf_upts(0)(Nu,N,nf)     after hifr_residual */
+  HIFR_DISU_FPTS = 3,       /* disu_fpts        (Nfp,N,nf)     current traces */
+  HIFR_GRAD_DISU_FPTS = 4   /* grad_disu_fpts   (Nfp,N,nf,d)   after hifr_residual (viscous) */
+};
+
+/* Physical and numerical parameters after non-dimensionalisation
+ * (run_input after input::setup_params, src/input.cpp:345-515). */
+typedef struct {
+  int ele_type;              /* HIFR_QUAD or HIFR_HEX */
+  int order;                 /* polynomial order P */
+  int n_eles;
+  int viscous;               /* 0 Euler, 1 Navier-Stokes (LDG) */
+  int riemann_solve_type;    /* 0 Rusanov (only inviscid flux built) */
+  int vis_riemann_solve_type;/* 0 LDG */
+  int precision;             /* HIFR_FP64 */
+  double gamma, prandtl, mu_inf, rt_inf, c_sth, fix_vis;
+  double tau, pen_fact;      /* LDG penalty and switch */
+  double const_src;
+} hifr_config;
+
+typedef struct hifr_ctx hifr_ctx;
+
+const char* hifr_status_string(hifr_status s);
+
+/* Creates a context on the current HIP device.
+ * Replaces eles::setup's device allocations (src/eles.cpp:84-405, mv_all_cpu_gpu :775). */
+hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out);
+hifr_status hifr_destroy(hifr_ctx* ctx);
+
+/* Stream for all later work (a hipStream_t; NULL = default stream). */
+hifr_status hifr_set_stream(hifr_ctx* ctx, void* hip_stream);
+
+/* Element operators in the reference layouts, as built by eles::set_opp_0/3/4
+ * (src/eles.cpp:3620-4052): opp_0 (Nfp,Nu), opp_3 (Nu,Nfp), opp_4 (Nu,Nu,d) with the
+ * d matrices stacked.  opp_1, opp_2, opp_5, opp_6 follow from these
+ * (src/eles.cpp:3700, 3771, 3956, 4018).  Replaces the opp_* cp_cpu_gpu uploads and
+ * every cublasDgemm/bespoke_SPMV of src/eles.cpp.  For quads and hexes the
+ * matrices must have the tensor-product structure of Lagrange bases on a 1-D
+ * point set; the context keeps only their 1-D factors. */
+hifr_status hifr_set_operators(hifr_ctx* ctx, const double* opp_0, const double* opp_3,
+                               const double* opp_4);
+
+/* Static-mesh metrics of eles::set_transforms (src/eles.cpp:4557-4847):
+ * detjac_upts (Nu,N), JGinv_upts (d,d,Nu,N), tdA_fpts (Nfp,N), norm_fpts (Nfp,N,d). */
+hifr_status hifr_set_metrics(hifr_ctx* ctx, const double* detjac_upts, const double* JGinv_upts,
+                             const double* tdA_fpts, const double* norm_fpts);
+
+/* Interior and periodic interfaces: for each of n_pts interface points, the
+ * element flux-point index (fpt + Nfp*ele) on the left and on the right.  This is
+ * the resolved form of the pointer tables int_inters::set_interior builds with the
+ * rot_tag lookup (src/int_inters.cpp:95-178, src/inters.cpp:187-296).  The left
+ * side's normal is used for the whole point, as in the reference. */
+hifr_status hifr_set_interfaces(hifr_ctx* ctx, int n_pts, const int32_t* left, const int32_t* right);
+
+/* disu_upts(0) (Nu,N,nf); zeroes the RK register disu_upts(1)
+ * (set_disu_upts_to_zero_other_levels, src/eles.cpp:407-425) and refreshes the traces. */
+hifr_status hifr_upload_state(hifr_ctx* ctx, const double* disu_upts);
+hifr_status hifr_download(hifr_ctx* ctx, int what, double* host);
+
+/* One CalcResidual (src/solver.cpp:65-225): div_tconf_upts(0) on the device. */
+hifr_status hifr_residual(hifr_ctx* ctx);
+
+/* CalcResidual followed by AdvanceSolution (src/eles.cpp:1003-1198, dt_type 0):
+ * adv_type 3 = low-storage RK45 stage `stage` (0..4), adv_type 0 = forward Euler. */
+hifr_status hifr_advance(hifr_ctx* ctx, int stage, int adv_type, double dt);
+
+/* n_steps full time steps (5 stages each for adv_type 3), src/HiFiLES.cpp:154-187. */
+hifr_status hifr_run(hifr_ctx* ctx, int n_steps, int adv_type, double dt);
+
+/* CalcNormResidual (src/output.cpp:1452-1522) on the last residual:
+ * norm_type 0 max, 1 L1, 2 L2 of div/J - const_src per field; out[nf]. */
+hifr_status hifr_norm_residual(hifr_ctx* ctx, int norm_type, double* out);
+
+hifr_status hifr_synchronize(hifr_ctx* ctx);
+
+/* Times n_steps of hifr_run with HIP events on the context's stream.
+ * total_ms: whole loop.  kernel_ms[i] / kernel_launches[i]: summed duration and
+ * launch count of kernel kind i (0 trace, 1 gradient trace, 2 flux/divergence/RK),
+ * each bracketed by its own events.  Arrays may be NULL. */
+hifr_status hifr_run_timed(hifr_ctx* ctx, int n_steps, int adv_type, double dt, double* total_ms,
+                           double* kernel_ms, int* kernel_launches);
+
+/* Sizes for callers that allocate host buffers. */
+hifr_status hifr_sizes(const hifr_ctx* ctx, int* n_upts, int* n_fpts, int* n_fields, int* n_dims);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,66 @@
+/*
+ * hifr_solver.h -- C API of the host solver that sits above the device ABI
+ * (hifr.h).  It restates the reference's driver: read the input deck and mesh
+ * (src/input.cpp, src/geometry.cpp GeoPreprocess), set up the elements and
+ * interfaces, set the initial condition or read a restart (InitSolution,
+ * src/solver.cpp:352-444), then CalcResidual / AdvanceSolution / CalcNormResidual.
+ * The reference has no library API (only the HiFiLES executable); this API is what
+ * the bundled `hifr_run` executable and the Python tests call.
+ *
+ * All functions return 0 on success; on failure a message is copied into err
+ * (errlen bytes) when err is non-NULL.
+ */
+#ifndef HIFR_SOLVER_H
+#define HIFR_SOLVER_H
+
+#include "hifr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hifr_solver hifr_solver;
+
+/* Deck file + the Gambit mesh it names. */
+int hifr_solver_from_deck(const char* deck_path, hifr_solver** out, char* err, int errlen);
+
+/* Deck given as text; mesh given as arrays (one linear element type; c2v in the
+ * internal shape-node order). */
+int hifr_solver_from_arrays(const char* deck_text, int ele_type, int n_verts, const double* xv, int n_eles,
+                            const int* c2v, hifr_solver** out, char* err, int errlen);
+
+/* Deck given as text; structured box mesh of n[0] x n[1] (x n[2]) elements on
+ * [o, o+l], vertex coordinates rounded to `digits` significant digits (0: exact). */
+int hifr_solver_box(const char* deck_text, int ele_type, const int* n, const double* l, const double* o,
+                    int digits, hifr_solver** out, char* err, int errlen);
+
+void hifr_solver_destroy(hifr_solver* s);
+
+/* n_upts, n_fpts per element, n_fields, n_dims, n_eles, n_inters */
+void hifr_solver_sizes(const hifr_solver* s, int* out6);
+
+/* Host arrays (reference layouts):
+ *  0 disu_upts (Nu,N,nf)  1 div_tconf_upts  2 disu_fpts (Nfp,N,nf)  3 grad_disu_fpts (Nfp,N,nf,d)
+ *  4 detjac_upts (Nu,N)   5 pos_upts (Nu,N,d)  6 opp_0  7 opp_3  8 opp_4
+ *  9 JGinv_upts (d,d,Nu,N) 10 tdA_fpts (Nfp,N) 11 norm_fpts (Nfp,N,d) 12 xv (n_verts,d) 13 c2v (as doubles)
+ *  14 rk register (Nu,N,nf)
+ * Returns the number of doubles (out may be NULL to query). */
+long hifr_solver_get(hifr_solver* s, int what, double* out);
+int hifr_solver_set_state(hifr_solver* s, const double* disu_upts);
+/* Flux-point pairs of the interior/periodic interfaces (fpt + Nfp*ele). */
+long hifr_solver_interfaces(const hifr_solver* s, int* left, int* right);
+
+int hifr_solver_residual(hifr_solver* s);
+int hifr_solver_advance(hifr_solver* s, int stage);
+int hifr_solver_run(hifr_solver* s, int n_steps);
+int hifr_solver_norms(hifr_solver* s, int norm_type, double* out);
+/* Writes Rest_<iter>_p0000.dat in the reference's restart format (src/output.cpp:1057-1107). */
+int hifr_solver_write_restart(hifr_solver* s, const char* path);
+hifr_ctx* hifr_solver_ctx(hifr_solver* s);
+double hifr_solver_dt(const hifr_solver* s);
+int hifr_solver_adv_type(const hifr_solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,77 @@
+"""Test cases shared by the CPU and GPU tests: the golden fixtures of
+tests/golden/*.npz and the decks that reproduce them for the oracle and for the
+GPU solver."""
+import glob
+import os
+
+import numpy as np
+
+import frmesh
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+TWO_PI = 6.2831853071795862
+
+# physical parameters of tests/golden/make_goldens.py decks
+PHYS = dict(gamma=1.4, prandtl=0.72, Mach_free_stream=0.1, nx_free_stream=1.0, Re_free_stream=1600.0,
+            L_free_stream=1.0, T_free_stream=300.0, T_wall=300.0)
+
+
+def golden_names(kind=None):
+    names = sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz")))
+    if kind is None:
+        return names
+    return [n for n in names if str(load_golden(n)["kind"]) == kind]
+
+
+def load_golden(name):
+    with np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def mesh_of(g):
+    ele_type = int(g["ele_type"])
+    return ele_type, g["xv"], frmesh.gambit_to_internal(ele_type, g["conn_gambit"])
+
+
+def product_deck(g, host_only=False, **over):
+    import hifr
+    ele_type = int(g["ele_type"])
+    cyc = [float(v) for v in g["cyc"]]
+    kw = dict(viscous=int(g["viscous"]), order=int(g["order"]), tau=float(g["tau"]),
+              pen_fact=float(g["pen_fact"]), dt=float(g["dt"]),
+              adv_type=3 if str(g["kind"]) == "rk45" else 0,
+              dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2], fix_vis=1)
+    kw.update(PHYS)
+    if ele_type == 4:
+        kw["vcjh_scheme_hexa"] = int(g["vcjh"])
+    else:
+        kw["vcjh_scheme_quad"] = int(g["vcjh"])
+    if host_only:
+        kw["hifr_host_only"] = 1
+    kw.update(over)
+    return hifr.deck_text(**kw)
+
+
+def oracle_deck(g, **over):
+    from oracle import pyoracle as po
+    cyc = [float(v) for v in g["cyc"]]
+    kw = dict(equation=0, viscous=int(g["viscous"]), order=int(g["order"]),
+              adv_type=3 if str(g["kind"]) == "rk45" else 0, dt=float(g["dt"]), tau=float(g["tau"]),
+              pen_fact=float(g["pen_fact"]), vcjh_scheme=int(g["vcjh"]), eta=0.0,
+              dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2])
+    kw.update(PHYS)
+    kw.update(over)
+    return po.default_deck(**kw)
+
+
+def field_rel_err(a, b):
+    """max |a-b| / max |b| per conserved variable (last axis of a (.., nf) array)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    nf = b.shape[-1] if b.ndim == 3 else b.shape[2]
+    out = []
+    for k in range(nf):
+        bk = b[..., k] if b.ndim == 3 else b[:, :, k]
+        ak = a[..., k] if a.ndim == 3 else a[:, :, k]
+        out.append(float(np.abs(ak - bk).max() / max(np.abs(bk).max(), 1e-300)))
+    return np.array(out)

@@ -1,0 +1,248 @@
+"""Mathematical pins of the oracle, independent of its own formulas.
+
+Each check is chosen so that a plausible mistake fails it:
+  * Gauss table          -- numpy's Gauss-Legendre nodes.
+  * opp_0 / opp_4        -- exact interpolation / differentiation of degree-P polynomials.
+  * opp_3, DG (eta=0)    -- the DG-FR identity g_L'(x_j) = -l_j(-1)/w_j with Gauss weights.
+  * opp_3, SD            -- the closed form g_L = (-1)^P/2 (1-x) P_P(x) (eta = P/(P+1)).
+  * opp_3, any eta       -- sum_j w_j g_L'(x_j) = g_L(1) - g_L(-1) = -1.
+  * metrics              -- total volume = box volume on perturbed meshes; closed surfaces
+                            (sum of n tdA = 0, sum of x.n tdA = d * volume).
+  * residual             -- free-stream preservation, discrete conservation, manufactured
+                            solutions for the Euler flux (advected vortex) and for the viscous
+                            stress / heat flux terms (shear layer, conduction), with the
+                            expected order of convergence.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+import frmesh
+from oracle import pyoracle as po
+
+TWO_PI = cases.TWO_PI
+
+
+def _periodic_box(ele_type, n, L=TWO_PI, perturb=0.0, seed=0):
+    if ele_type == 4:
+        xv, conn = frmesh.box_hex(n, n, n, L, L, L)
+    else:
+        xv, conn = frmesh.box_quad(n, n, L, L)
+    if perturb:
+        xv = frmesh.perturb_interior(xv, perturb * L / n, seed)
+    return xv, frmesh.gambit_to_internal(ele_type, conn)
+
+
+def _oracle(ele_type, n, order, viscous=0, vcjh=1, perturb=0.0, L=TWO_PI, **kw):
+    xv, c2v = _periodic_box(ele_type, n, L, perturb)
+    d = dict(equation=0, viscous=viscous, order=order, adv_type=0, dt=1e-3, tau=kw.pop("tau", 1.0),
+             pen_fact=0.5, vcjh_scheme=vcjh, eta=0.0, dx_cyclic=L, dy_cyclic=L, dz_cyclic=L)
+    d.update(cases.PHYS)
+    d.update(kw)
+    return po.Oracle(po.default_deck(**d), ele_type, xv, c2v)
+
+
+def _weights_1d(order):
+    return np.polynomial.legendre.leggauss(order + 1)[1]
+
+
+@pytest.mark.parametrize("order", range(0, 7))
+def test_gauss_table_matches_legendre_roots(order):
+    o = _oracle(1, 2, order)
+    x = o.get("loc_upts")[0, : order + 1]
+    ref = np.polynomial.legendre.leggauss(order + 1)[0]
+    assert np.abs(x - ref).max() <= 1.2e-15
+
+
+@pytest.mark.parametrize("ele_type,order", [(1, 1), (1, 3), (1, 5), (4, 2), (4, 3), (4, 4)])
+def test_interpolation_and_derivative_exact(ele_type, order):
+    o = _oracle(ele_type, 2, order)
+    nd = 3 if ele_type == 4 else 2
+    rng = np.random.default_rng(order)
+    coef = rng.standard_normal((nd, order + 1))
+    P = np.polynomial.polynomial
+
+    def f(loc):
+        return np.prod([P.polyval(loc[d], coef[d]) for d in range(nd)], axis=0)
+
+    def df(loc, d):
+        parts = [P.polyval(loc[k], coef[k]) if k != d else P.polyval(loc[k], P.polyder(coef[k]))
+                 for k in range(nd)]
+        return np.prod(parts, axis=0)
+
+    lu, lf = o.get("loc_upts"), o.get("tloc_fpts")
+    u = f(lu)
+    np.testing.assert_allclose(o.get("opp_0") @ u, f(lf), rtol=0, atol=1e-12 * np.abs(u).max())
+    D = o.get("opp_4")
+    for d in range(nd):
+        np.testing.assert_allclose(D[:, :, d] @ u, df(lu, d), rtol=0, atol=1e-11 * np.abs(u).max())
+
+
+def _left_end_column(o, order):
+    """opp_3 column of the quad flux point at r = -1 on the line s = x_0 (face 3, last point)."""
+    n = order + 1
+    fpL = 3 * n + order
+    return o.get("opp_3")[:n, fpL], o.get("opp_0")[fpL, :n]
+
+
+@pytest.mark.parametrize("order", range(1, 6))
+def test_dg_correction_is_the_dg_lift(order):
+    o = _oracle(1, 2, order, vcjh=1)
+    col, lm1 = _left_end_column(o, order)
+    # column = -g_L'(x_j) * l_0(x_0) = -g_L'(x_j);  DG: g_L'(x_j) = -l_j(-1) / w_j
+    np.testing.assert_allclose(col, lm1 / _weights_1d(order), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("order", range(1, 6))
+def test_sd_correction_closed_form(order):
+    o = _oracle(1, 2, order, vcjh=2)
+    col, _ = _left_end_column(o, order)
+    x = o.get("loc_upts")[0, : order + 1]
+    L = np.polynomial.legendre
+    cP = np.zeros(order + 1)
+    cP[order] = 1.0
+    gLd = (-1) ** order / 2.0 * (-L.legval(x, cP) + (1 - x) * L.legval(x, L.legder(cP)))
+    np.testing.assert_allclose(-col, gLd, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("vcjh", [1, 2, 3])
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_correction_integrates_to_minus_one(vcjh, order):
+    o = _oracle(1, 2, order, vcjh=vcjh)
+    col, _ = _left_end_column(o, order)
+    assert abs(np.dot(_weights_1d(order), -col) + 1.0) < 1e-13
+
+
+@pytest.mark.parametrize("ele_type", [1, 4])
+def test_metrics_volume_and_closed_surfaces(ele_type):
+    n, order = 3, 3
+    o = _oracle(ele_type, n, order, perturb=0.12)
+    nd = 3 if ele_type == 4 else 2
+    w1 = _weights_1d(order)
+    wu = np.einsum("i,j,k->kji", w1, w1, w1).ravel() if nd == 3 else np.outer(w1, w1).ravel()
+    vol = np.einsum("j,je->e", wu, o.get("detjac_upts"))
+    assert abs(vol.sum() - TWO_PI ** nd) < 1e-11 * TWO_PI ** nd
+    assert vol.min() > 0
+    wf = np.outer(w1, w1).ravel() if nd == 3 else w1
+    nfaces = 2 * nd
+    wfp = np.tile(wf, nfaces)
+    tdA, nrm, pos = o.get("tdA_fpts"), o.get("norm_fpts"), o.get("pos_fpts")
+    flux = np.einsum("f,fe,fed->ed", wfp, tdA, nrm)
+    assert np.abs(flux).max() < 1e-12
+    xn = np.einsum("f,fe,fed,fed->e", wfp, tdA, nrm, pos)
+    np.testing.assert_allclose(xn, nd * vol, rtol=1e-12)
+
+
+def _state_const(o, rho=1.2, vel=(0.3, -0.2, 0.1), p=2.0, gamma=1.4):
+    nd = o.nd
+    u = np.zeros((o.nu, o.n_eles, o.nf))
+    u[..., 0] = rho
+    for i in range(nd):
+        u[..., 1 + i] = rho * vel[i]
+    u[..., nd + 1] = p / (gamma - 1) + 0.5 * rho * sum(v * v for v in vel[:nd])
+    return u
+
+
+@pytest.mark.parametrize("ele_type,viscous", [(1, 0), (1, 1), (4, 0), (4, 1)])
+def test_free_stream_preservation(ele_type, viscous):
+    o = _oracle(ele_type, 3, 3, viscous=viscous, perturb=0.12, vcjh=2)
+    o.set_state(_state_const(o))
+    div = o.residual()
+    assert np.abs(div).max() < 1e-12 * 10.0
+
+
+@pytest.mark.parametrize("ele_type,viscous", [(1, 1), (4, 1)])
+def test_discrete_conservation(ele_type, viscous):
+    o = _oracle(ele_type, 3, 3, viscous=viscous, perturb=0.1, vcjh=3)
+    pos = o.get("pos_upts")
+    u = _state_const(o)
+    rng = np.random.default_rng(3)
+    u *= 1.0 + 0.05 * np.sin(pos[..., :1] + 2 * pos[..., 1:2]) + 0.01 * rng.uniform(-1, 1, u.shape)
+    o.set_state(u)
+    div = o.residual()
+    w1 = _weights_1d(3)
+    wu = np.einsum("i,j,k->kji", w1, w1, w1).ravel() if o.nd == 3 else np.outer(w1, w1).ravel()
+    total = np.einsum("j,jek->k", wu, div)
+    scale = np.einsum("j,jek->k", wu, np.abs(div))
+    assert np.all(np.abs(total) < 1e-13 * scale + 1e-14)
+
+
+def _vortex_error(n, order):
+    L = 20.0
+    xv, conn = frmesh.box_quad(n, n, L, L, origin=(-10.0, -10.0))
+    c2v = frmesh.gambit_to_internal(1, conn)
+    d = po.default_deck(equation=0, viscous=0, order=order, adv_type=0, dt=1e-3, vcjh_scheme=1,
+                        dx_cyclic=L, dy_cyclic=L)
+    o = po.Oracle(d, 1, xv, c2v)
+    pos = o.get("pos_upts")
+    o.set_state(frmesh.shu_vortex_state(pos))
+    R = o.residual() / o.get("detjac_upts")[:, :, None]
+    h = 1e-5
+    dudt = (frmesh.shu_vortex_state(pos, t=h) - frmesh.shu_vortex_state(pos, t=-h)) / (2 * h)
+    e = R + dudt
+    return math.sqrt((e ** 2).mean() / (dudt ** 2).mean())
+
+
+def test_euler_residual_converges_for_advected_vortex():
+    # relative L2 truncation error of the residual, order P once the core (radius 1)
+    # is resolved: 40^2 -> 80^2 elements on [-10, 10]^2
+    e1, e2 = _vortex_error(40, 3), _vortex_error(80, 3)
+    order = math.log(e1 / e2, 2)
+    assert e2 < 5e-3 and order > 2.8, (e1, e2, order)
+
+
+def _viscous_field(o, case):
+    """State and exact viscous contribution to div/J of two fields:
+    'shear': rho=1, u = U sin(y), p const  -> only tau_xy and viscous heating;
+    'heat':  u = 0, p const, rho = 1/(1 + eps sin x) -> only heat conduction."""
+    ph = o.phys()
+    mu, gam, Pr = ph.mu_inf, ph.gamma, ph.prandtl
+    pos = o.get("pos_upts")
+    x, y = pos[..., 0], pos[..., 1]
+    nd = o.nd
+    u = np.zeros((o.nu, o.n_eles, o.nf))
+    exact = np.zeros_like(u)
+    p = 50.0
+    if case == "shear":
+        U = 2.0
+        vel = U * np.sin(y)
+        u[..., 0] = 1.0
+        u[..., 1] = vel
+        u[..., nd + 1] = p / (gam - 1) + 0.5 * vel ** 2
+        exact[..., 1] = mu * U * np.sin(y)
+        exact[..., nd + 1] = -mu * U ** 2 * np.cos(2 * y)
+    else:
+        eps = 0.1
+        u[..., 0] = 1.0 / (1.0 + eps * np.sin(x))
+        u[..., nd + 1] = p / (gam - 1)
+        exact[..., nd + 1] = (mu * gam / Pr) * p * eps * np.sin(x) / (gam - 1)
+    return u, exact
+
+
+def _manufactured_viscous(ele_type, n, case):
+    """Viscous part of the residual (Navier-Stokes minus Euler on the same state)
+    against the exact viscous divergence; returns (relative error of the active
+    component, max |viscous part of the density residual|)."""
+    ons = _oracle(ele_type, n, 3, viscous=1, vcjh=1, tau=0.0)
+    oeu = _oracle(ele_type, n, 3, viscous=0, vcjh=1)
+    u, exact = _viscous_field(ons, case)
+    ons.set_state(u)
+    oeu.set_state(u)
+    J = ons.get("detjac_upts")[:, :, None]
+    Rv = ons.residual() / J - oeu.residual() / J
+    k = 1 if case == "shear" else ons.nd + 1
+    err = math.sqrt(((Rv[..., k] - exact[..., k]) ** 2).mean() / (exact[..., k] ** 2).mean())
+    return err, np.abs(Rv[..., 0]).max()
+
+
+@pytest.mark.parametrize("ele_type", [1, 4])
+@pytest.mark.parametrize("case", ["shear", "heat"])
+def test_viscous_terms_manufactured(ele_type, case):
+    # second-derivative terms: the pointwise residual error of the viscous part is
+    # O(h^(P-1)) at P = 3; a wrong sign, factor or dropped term leaves an O(1) error
+    e1, r1 = _manufactured_viscous(ele_type, 4, case)
+    e2, r2 = _manufactured_viscous(ele_type, 8, case)
+    assert e2 < 0.08 and e1 / e2 > 2.5, (e1, e2)
+    assert r2 == 0.0
