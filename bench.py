@@ -51,18 +51,36 @@ def tgv_state(pos, seed=1234):
 
 
 def algorithmic_bytes(order, nd=3):
-    """Minimum HBM bytes per element of the two kernels of a Navier-Stokes stage
-    (FP64 values, int32 neighbour table):
-      gradient-trace kernel: read u, neighbour u_f, JGinv, detJ, neighbour index;
-                             write grad u_f;
-      flux/RK kernel:        read u, RK register, neighbour u_f and grad u_f,
-                             JGinv, detJ, face normal + tdA, neighbour index;
-                             write u, RK register, next u_f."""
+    """HBM bytes per element that the two kernels of a Navier-Stokes stage must
+    move (FP64 values, int32 neighbour table), counting every array once:
+      gradient-trace kernel: read u, neighbour u_f, JGinv, detJ, face normals,
+                             neighbour index; write the viscous normal flux
+                             traces F_v.n (nf per flux point);
+      flux/RK kernel:        read u, RK register, neighbour u_f, own and neighbour
+                             F_v.n, JGinv, detJ, face normal + tdA, neighbour
+                             index; write u, RK register, next u_f."""
     n = order + 1
     nu, nfp, nf = n ** nd, 2 * nd * n ** (nd - 1), nd + 2
-    grad = 8 * (nu * nf + nfp * nf + nu * nd * nd + nu + nfp * nf * nd) + 4 * nfp
-    rk = 8 * (4 * nu * nf + nfp * nf + nfp * nf * nd + nu * nd * nd + nu + nfp * (nd + 1) + nfp * nf) + 4 * nfp
+    grad = 8 * (nu * nf + nfp * nf + nu * nd * nd + nu + nfp * nd + nfp * nf) + 4 * nfp
+    rk = 8 * (4 * nu * nf + nfp * nf + 2 * nfp * nf + nu * nd * nd + nu + nfp * (nd + 1) + nfp * nf) + 4 * nfp
     return grad, rk
+
+
+# HBM traffic per element of each stage kernel at 32^3, P=3, from rocprofv3 --pmc
+# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the factors read off a
+# 1 GiB copy kernel (profiles/fr_stage_tgv32_p3_fp64.md).
+MEASURED_TRAFFIC = {3: {"grad": (227408.0 * 2.0 + 122880.0) * 1024 / 32768,
+                        "rk": (388554.0 * 2.0 + 297642.7) * 1024 / 32768}}
+
+
+def survey_model_bytes(order, nd=3):
+    """Bytes per element of one Navier-Stokes stage in the three-phase model
+    (u -> u_f; u + u_f -> grad u_f; u + u_f + grad u_f -> flux, div, RK) that
+    counts every operand once per phase; this schedule moves fewer bytes."""
+    n = order + 1
+    nu, nfp, nf = n ** nd, 2 * nd * n ** (nd - 1), nd + 2
+    return 8 * (3 * nu * nf + nu * nf + 2 * nu * nf + 3 * nfp * nf + 2 * nfp * nf * nd + 2 * nu * (1 + nd * nd)
+                + 2 * nfp * (nd + 1))
 
 
 def cpu_baseline(s, order):
@@ -95,11 +113,25 @@ def main():
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy kernel of MIB MiB (byte-counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(the residual would shard by element partition with face-trace halos).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import ctypes as C
+        L = hifr.lib()
+        L.hifr_calibrate_copy.argtypes = [C.c_long, C.c_int, C.POINTER(C.c_double)]
+        nd = args.copy_calibration * (1 << 20) // 8
+        ms = C.c_double()
+        rc = L.hifr_calibrate_copy(nd, 10, C.byref(ms))
+        if rc != 0:
+            raise SystemExit("copy calibration failed (%d)" % rc)
+        print(json.dumps({"copy_bytes_read": nd * 8, "copy_bytes_written": nd * 8, "ms_per_copy": ms.value,
+                          "GBs": 2 * nd * 8 / (ms.value * 1e-3) / 1e9}), flush=True)
+        return
     n, order = args.n, args.order
     t_setup = time.perf_counter()
     s = hifr.Solver.box(deck(order, args.steps), hifr.HEX, (n, n, n), (TWO_PI,) * 3, (0.0,) * 3)
@@ -122,6 +154,8 @@ def main():
     avg_grad_ms = kms[1] / kl[1] if kl[1] else 0.0
     achieved = rk_b * s.n_eles / (avg_rk_ms * 1e-3) / 1e9
     stage_bytes = (grad_b + rk_b) * s.n_eles
+    stage_ms = avg_rk_ms + avg_grad_ms
+    traffic = MEASURED_TRAFFIC.get(order) if n == 32 else None
     rec = {
         "metric": METRIC, "value": round(value, 4), "unit": "GDOF-stage/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
@@ -130,12 +164,18 @@ def main():
                                "Rusanov+LDG, RK45 dt=1e-3, FP64" % (n, order),
                    "n_eles": s.n_eles, "order": order, "dof": dof, "stages_per_step": 5},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic["rk"] * s.n_eles) if traffic else None,
                      "kernel": "fr_tensor_kernel ADVANCE (flux, divergence, RK update, next traces)",
-                     "bytes_per_launch": rk_b * s.n_eles, "avg_launch_ms": round(avg_rk_ms, 5)},
+                     "bytes_per_launch": rk_b * s.n_eles, "bytes_per_element": rk_b,
+                     "avg_launch_ms": round(avg_rk_ms, 5)},
         "stage": {"avg_grad_kernel_ms": round(avg_grad_ms, 5), "avg_rk_kernel_ms": round(avg_rk_ms, 5),
+                  "grad_kernel_bytes_per_element": grad_b,
+                  "grad_kernel_traffic": round(traffic["grad"] * s.n_eles) if traffic else None,
                   "algorithmic_bytes": stage_bytes,
-                  "achieved_GBs": round(stage_bytes / ((avg_rk_ms + avg_grad_ms) * 1e-3) / 1e9, 1),
+                  "achieved_GBs": round(stage_bytes / (stage_ms * 1e-3) / 1e9, 1),
+                  "three_phase_model_bytes_per_dof": round(survey_model_bytes(order) / s.nu / s.nf, 2),
+                  "three_phase_model_GBs": round(survey_model_bytes(order) * s.n_eles / (stage_ms * 1e-3) / 1e9, 1),
                   "event_total_ms": round(total_ms, 3)},
         "residual_L1": [float("%.6e" % v) for v in norms],
         "setup_s": round(t_setup, 2),

@@ -35,13 +35,20 @@ __device__ __forceinline__ void inv_flux_dot(const double* u, const double* w, d
   out[ND + 1] = vw * (u[ND + 1] + p);
 }
 
-// Dynamic viscosity: Sutherland's law blended by fix_vis (src/flux.cpp:308-310);
-// fix_vis == 1 gives mu_inf.
+// Dynamic viscosity.  SUTH = false: constant viscosity mu_inf (fix_vis = 1, every
+// workload here); SUTH = true: Sutherland's law blended by fix_vis
+// (src/flux.cpp:308-310), with rt^1.5 formed as rt*sqrt(rt), within an ulp of pow.
+// The choice is a template parameter so that the constant-viscosity kernels do not
+// carry the register footprint of the Sutherland branch.
+template <bool SUTH>
 __device__ __forceinline__ double viscosity(const Phys& ph, double inte) {
-  if (ph.fix_vis == 1.0) return ph.mu_inf;
-  const double rt = (ph.gamma - 1.0) * inte / ph.rt_inf;
-  const double mu = ph.mu_inf * pow(rt, 1.5) * (1.0 + ph.c_sth) / (rt + ph.c_sth);
-  return mu + ph.fix_vis * (ph.mu_inf - mu);
+  if constexpr (!SUTH) {
+    return ph.mu_inf;
+  } else {
+    const double rt = (ph.gamma - 1.0) * inte / ph.rt_inf;
+    const double mu = ph.mu_inf * (rt * sqrt(rt)) * (1.0 + ph.c_sth) / (rt + ph.c_sth);
+    return mu + ph.fix_vis * (ph.mu_inf - mu);
+  }
 }
 
 // Laminar viscous stress tau and energy flux column of the Navier-Stokes viscous
@@ -53,7 +60,7 @@ struct ViscT {
 };
 
 // g(k, m) = d u_k / d x_m (physical gradient of the conservative state)
-template <int ND>
+template <int ND, bool SUTH>
 __device__ __forceinline__ void visc_tensor(const double* u, const double (*g)[ND], const Phys& ph, ViscT<ND>& t) {
   const double rho = u[0], irho = 1.0 / rho;
   double v[ND];
@@ -64,7 +71,7 @@ __device__ __forceinline__ void visc_tensor(const double* u, const double (*g)[N
     ke += v[i] * v[i];
   }
   const double inte = u[ND + 1] * irho - 0.5 * ke;
-  const double mu = viscosity(ph, inte);
+  const double mu = viscosity<SUTH>(ph, inte);
   double dv[ND][ND];
   double de[ND];
 #pragma unroll
@@ -77,6 +84,57 @@ __device__ __forceinline__ void visc_tensor(const double* u, const double (*g)[N
     }
     const double dke = 0.5 * ke * g[0][m] + rho * rdv;
     de[m] = (g[ND + 1][m] - dke - g[0][m] * inte) * irho;
+  }
+  double diag = 0.0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) diag += dv[i][i];
+  diag = diag / 3.0;
+#pragma unroll
+  for (int i = 0; i < ND; i++)
+#pragma unroll
+    for (int j = 0; j < ND; j++)
+      t.tau[i][j] = (i == j) ? 2.0 * mu * (dv[i][i] - diag) : mu * (dv[i][j] + dv[j][i]);
+  const double kq = mu / ph.prandtl * ph.gamma;
+#pragma unroll
+  for (int m = 0; m < ND; m++) {
+    double w = 0.0;
+#pragma unroll
+    for (int i = 0; i < ND; i++) w += v[i] * t.tau[i][m];
+    t.qe[m] = -(w + kq * de[m]);
+  }
+}
+
+
+// Same as visc_tensor with the gradient supplied one column (d/dx_m of all
+// fields) at a time by gcol(m, out[NF]); keeps only one column live.
+template <int ND, bool SUTH, class GCol>
+__device__ __forceinline__ void visc_tensor_cols(const double* u, GCol&& gcol, const Phys& ph, ViscT<ND>& t) {
+  constexpr int NF = ND + 2;
+  const double rho = u[0], irho = 1.0 / rho;
+  double v[ND];
+  double ke = 0.0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    v[i] = u[1 + i] * irho;
+    ke += v[i] * v[i];
+  }
+  const double inte = u[ND + 1] * irho - 0.5 * ke;
+  const double mu = viscosity<SUTH>(ph, inte);
+  double dv[ND][ND];
+  double de[ND];
+#pragma unroll
+  for (int m = 0; m < ND; m++) {
+    double g[NF];
+    gcol(m, g);
+    double rdv = 0.0;
+#pragma unroll
+    for (int i = 0; i < ND; i++) {
+      dv[i][m] = (g[1 + i] - g[0] * v[i]) * irho;
+      rdv += v[i] * dv[i][m];
+    }
+    const double dke = 0.5 * ke * g[0] + rho * rdv;
+    de[m] = (g[ND + 1] - dke - g[0] * inte) * irho;
+    __builtin_amdgcn_sched_barrier(0);
   }
   double diag = 0.0;
 #pragma unroll

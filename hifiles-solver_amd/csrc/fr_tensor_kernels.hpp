@@ -7,21 +7,23 @@
 // solution points, so interpolation to a face, the reference-space derivative and
 // the VCJH correction lift are length-(P+1) sums along that line, read from LDS.
 //
-// A residual needs the neighbour traces of u (and, for Navier-Stokes, of the
-// corrected gradient), so one RK stage is split at the two points where data must
-// cross elements:
+// A residual needs data from the neighbours twice (the state traces, and for
+// Navier-Stokes the viscous flux traces), so one RK stage is split at those points:
 //   TRACE_U    u -> u at the element's flux points                  (first stage only)
-//   TRACE_GRAD u, neighbour u_f -> corrected physical grad u at flux points
-//   RESID      u, neighbour u_f (+ grad u_f) -> div of the continuous flux
+//   TRACE_GRAD u, neighbour u_f -> corrected physical grad u -> viscous normal flux
+//              F_v(u_f, grad u_f).n at every flux point (5 values instead of the
+//              15-component gradient trace: the LDG flux only needs normal
+//              components, and the neighbour's value is the exact negation)
+//   RESID      u, neighbour u_f, viscous normal fluxes -> div of the continuous flux
 //   ADVANCE    RESID + RK update (src/eles.cpp:1091-1188) + traces of the new u,
 //              so the next stage starts from TRACE_GRAD.
 // Each kernel recomputes what it needs element-locally instead of storing
-// intermediates: the minimum traffic schedule (u read twice per stage, traces
-// written once and read by the neighbour once).
+// intermediates (u is read twice per stage, every trace is written once and read
+// once by the neighbour).
 //
-// Layouts (first index fastest): u, res, div (Nu,N,nf); u_f (Nfp,N,nf);
-// grad u_f (Nfp,N,nf,d) (the reference's grad_disu_fpts); detJ (Nu,N); JGinv
-// component-major (d*d, Nu*N).
+// Layouts (first index fastest): u, res, div (Nu,N,nf); u_f and the viscous normal
+// flux traces (Nfp,N,nf); optional gradient traces (Nfp,N,nf,d) = the reference's
+// grad_disu_fpts; detJ (Nu,N); JGinv component-major (d*d, Nu*N).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -47,8 +49,9 @@ struct StageArgs {
   double* div;           // div_tconf_upts output (RESID; ADVANCE if non-null)
   const double* uf_in;   // traces of the current state
   double* uf_out;        // traces written (TRACE_U, ADVANCE)
-  const double* gf_in;   // gradient traces (RESID/ADVANCE, viscous)
-  double* gf_out;        // gradient traces written (TRACE_GRAD)
+  const double* fv_in;   // viscous normal flux traces (RESID/ADVANCE, viscous)
+  double* fv_out;        // viscous normal flux traces written (TRACE_GRAD)
+  double* gf_out;        // optional gradient traces (TRACE_GRAD, may be null)
   const double* detj;
   const double* jg;
   const int* nbr;        // neighbour trace index per element flux point; bit 31: LDG switch -
@@ -114,27 +117,42 @@ __device__ __forceinline__ void line_of(int p, int dir, int& pos, int& base, int
   }
 }
 
-template <int P, int ND, int EPB, bool VISC, int MODE>
+template <int P, int ND, int EPB, int VIS, int MODE>
 struct SmemLayout {
   using T = TP<P, ND, EPB>;
   static constexpr int U = EPB * T::NF * T::NU;
   static constexpr int Q = (MODE == MODE_TRACE_U) ? 0 : EPB * T::NFP * T::NF;
-  static constexpr int G = (VISC && MODE != MODE_TRACE_U) ? EPB * T::NF * ND * T::NU : 0;
+  static constexpr int G = (MODE == MODE_TRACE_GRAD) ? EPB * T::NF * ND * T::NU : 0;
   static constexpr int F = (MODE == MODE_RESID || MODE == MODE_ADVANCE) ? EPB * T::NF * ND * T::NU : 0;
   static constexpr int TOTAL = U + Q + G + F;
 };
 
-template <int P, int ND, int EPB, bool VISC, int MODE>
+// Interpolate NF fields from LDS rows (stride NU apart) to the end of a line.
+template <int n, int NF, int NU>
+__device__ __forceinline__ void interp_line(const double* rows, const double* lw, int base, int st, double* out) {
+#pragma unroll
+  for (int k = 0; k < NF; k++) {
+    double v = 0.0;
+#pragma unroll
+    for (int m = 0; m < n; m++) v += lw[m] * rows[k * NU + base + m * st];
+    out[k] = v;
+  }
+}
+
+// VIS: 0 Euler, 1 Navier-Stokes with constant viscosity, 2 with Sutherland's law.
+template <int P, int ND, int EPB, int VIS, int MODE>
 __global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK))
 fr_tensor_kernel(const StageArgs a) {
+  constexpr bool VISC = VIS != 0;
+  constexpr bool SUTH = VIS == 2;
   using T = TP<P, ND, EPB>;
-  using S = SmemLayout<P, ND, EPB, VISC, MODE>;
+  using S = SmemLayout<P, ND, EPB, VIS, MODE>;
   constexpr int n = T::n, NU = T::NU, NFP = T::NFP, NF = T::NF, BLOCK = T::BLOCK;
   __shared__ double smem[S::TOTAL];
-  double* s_u = smem;                 // [EPB][NF][NU]
-  double* s_q = s_u + S::U;           // [EPB][NFP][NF]   delta (LDG) then jump
-  double* s_g = s_q + S::Q;           // [EPB][NF*ND][NU] physical gradient
-  double* s_f = s_g + S::G;           // [EPB][NF*ND][NU] transformed flux
+  double* s_u = smem;        // [EPB][NF][NU]
+  double* s_q = s_u + S::U;  // [EPB][NFP][NF]   LDG delta, then interface jumps
+  double* s_g = s_q + S::Q;  // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD)
+  double* s_f = s_g + S::G;  // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE)
 
   const int N = a.N;
   const long NN = (long)N;
@@ -146,56 +164,46 @@ fr_tensor_kernel(const StageArgs a) {
   const Ops1D& ops = a.ops;
 
   // ---- load u ----
-  double u_own[NF];
   if (active) {
 #pragma unroll
-    for (int k = 0; k < NF; k++) {
-      u_own[k] = a.u[p + NU * (e + NN * k)];
-      s_u[(el * NF + k) * NU + p] = u_own[k];
-    }
+    for (int k = 0; k < NF; k++) s_u[(el * NF + k) * NU + p] = a.u[p + NU * (e + NN * k)];
   }
   __syncthreads();
 
-  // ---- traces of u only (start of the first stage) ----
   if constexpr (MODE == MODE_TRACE_U) {
+    // ---- traces of u only (start of the first stage) ----
+#pragma unroll 1
     for (int task = tid; task < EPB * NFP; task += BLOCK) {
       const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
       if (te >= N) continue;
       int dir, side, base;
       face_line<P, ND>(fp, dir, side, base);
-      const int st = T::stride(dir);
-      const double* lw = side ? ops.lR : ops.lL;
+      double uo[NF];
+      interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
 #pragma unroll
-      for (int k = 0; k < NF; k++) {
-        double v = 0.0;
-#pragma unroll
-        for (int m = 0; m < n; m++) v += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
-        a.uf_out[fp + NFP * (te + NN * k)] = v;
-      }
+      for (int k = 0; k < NF; k++) a.uf_out[fp + NFP * (te + NN * k)] = uo[k];
     }
     return;
   } else {
     // ---- LDG common solution -> delta = u_c - u_own at each flux point
     //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346) ----
     if constexpr (VISC) {
+#pragma unroll 1
       for (int task = tid; task < EPB * NFP; task += BLOCK) {
         const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
         if (te >= N) continue;
         int dir, side, base;
         face_line<P, ND>(fp, dir, side, base);
-        const int st = T::stride(dir);
-        const double* lw = side ? ops.lR : ops.lL;
         int nb = a.nbr[fp + NFP * te];
         const double pen = nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact;
         nb &= 0x7fffffff;
+        double uo[NF];
+        interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-          double uo = 0.0;
-#pragma unroll
-          for (int m = 0; m < n; m++) uo += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
           const double un = a.uf_in[nb + NFP * NN * k];
-          const double uc = 0.5 * (uo + un) - pen * (uo - un);
-          s_q[(tl * NFP + fp) * NF + k] = uc - uo;
+          const double uc = 0.5 * (uo[k] + un) - pen * (uo[k] - un);
+          s_q[(tl * NFP + fp) * NF + k] = uc - uo[k];
         }
       }
       __syncthreads();
@@ -209,36 +217,54 @@ fr_tensor_kernel(const StageArgs a) {
       for (int l = 0; l < ND; l++)
 #pragma unroll
         for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * NU + pe];
+      double u_own[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) u_own[k] = s_u[(el * NF + k) * NU + p];
       ViscT<ND> vt;
       if constexpr (VISC) {
         const double inv_detj = 1.0 / a.detj[pe];
         int pos[ND], lb[ND], fL[ND], fR[ND];
 #pragma unroll
         for (int d = 0; d < ND; d++) line_of<P, ND>(p, d, pos[d], lb[d], fL[d], fR[d]);
-        double gp[NF][ND];
+        // physical gradient column by column: d/dx_m = sum_d (JGinv(d,m)/J) d/dr_d
+        // (reference gradient + LDG correction, src/eles.cpp:1667-1870)
+        double gr[NF][ND];
 #pragma unroll
-        for (int k = 0; k < NF; k++) {
-          double gr[ND];
+        for (int k = 0; k < NF; k++)
 #pragma unroll
           for (int d = 0; d < ND; d++) {
             const int st = T::stride(d);
+            double r = 0.0;
+#pragma unroll
+            for (int q = 0; q < n; q++) r += ops.D[pos[d]][q] * s_u[(el * NF + k) * NU + lb[d] + q * st];
+            r += ops.gL[pos[d]] * s_q[(el * NFP + fL[d]) * NF + k] + ops.gR[pos[d]] * s_q[(el * NFP + fR[d]) * NF + k];
+            gr[k][d] = r;
+          }
+        double Ji[ND][ND];
+#pragma unroll
+        for (int d = 0; d < ND; d++)
+#pragma unroll
+          for (int m = 0; m < ND; m++) Ji[d][m] = Jg[d][m] * inv_detj;
+        auto gcol = [&](int m, double* gc) {
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
             double v = 0.0;
 #pragma unroll
-            for (int m = 0; m < n; m++) v += ops.D[pos[d]][m] * s_u[(el * NF + k) * NU + lb[d] + m * st];
-            v += ops.gL[pos[d]] * s_q[(el * NFP + fL[d]) * NF + k] + ops.gR[pos[d]] * s_q[(el * NFP + fR[d]) * NF + k];
-            gr[d] = v;
+            for (int d = 0; d < ND; d++) v += gr[k][d] * Ji[d][m];
+            gc[k] = v;
           }
-          // reference -> physical (src/eles.cpp:1815-1870): d/dx_m = sum_d (JGinv(d,m)/J) d/dr_d
+        };
+        if constexpr (MODE == MODE_TRACE_GRAD) {
 #pragma unroll
           for (int m = 0; m < ND; m++) {
-            double v = 0.0;
+            double gc[NF];
+            gcol(m, gc);
 #pragma unroll
-            for (int d = 0; d < ND; d++) v += gr[d] * (Jg[d][m] * inv_detj);
-            gp[k][m] = v;
-            s_g[(el * NF * ND + k * ND + m) * NU + p] = v;
+            for (int k = 0; k < NF; k++) s_g[(el * NF * ND + k * ND + m) * NU + p] = gc[k];
           }
+        } else {
+          visc_tensor_cols<ND, SUTH>(u_own, gcol, a.ph, vt);
         }
-        if constexpr (MODE != MODE_TRACE_GRAD) visc_tensor<ND>(u_own, gp, a.ph, vt);
       }
       if constexpr (MODE != MODE_TRACE_GRAD) {
         // transformed flux tdisf(k,l) = sum_m JGinv(l,m) F(k,m)  (src/eles.cpp:1397-1405, 2437-2444)
@@ -260,7 +286,9 @@ fr_tensor_kernel(const StageArgs a) {
     __syncthreads();
 
     if constexpr (MODE == MODE_TRACE_GRAD) {
-      // ---- corrected gradient at the flux points (extrapolate_corrected_gradient) ----
+      // ---- viscous normal flux F_v(u_f, grad u_f).n at the flux points
+      //      (the interface-side viscous flux of src/int_inters.cpp:392-404) ----
+#pragma unroll 1
       for (int task = tid; task < EPB * NFP; task += BLOCK) {
         const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
         if (te >= N) continue;
@@ -268,78 +296,34 @@ fr_tensor_kernel(const StageArgs a) {
         face_line<P, ND>(fp, dir, side, base);
         const int st = T::stride(dir);
         const double* lw = side ? ops.lR : ops.lL;
+        const long fe = fp + NFP * (long)te;
+        double uo[NF];
+        interp_line<n, NF, NU>(s_u + tl * NF * NU, lw, base, st, uo);
+        double nrm[ND];
 #pragma unroll
-        for (int k = 0; k < NF; k++)
+        for (int l = 0; l < ND; l++) nrm[l] = a.fgeo[l * NN * NFP + fe];
+        double* gout = a.gf_out;
+        ViscT<ND> t;
+        visc_tensor_cols<ND, SUTH>(uo, [&](int c, double* gc) {
 #pragma unroll
-          for (int x = 0; x < ND; x++) {
-            double v = 0.0;
+          for (int k = 0; k < NF; k++) {
+            double w = 0.0;
 #pragma unroll
-            for (int m = 0; m < n; m++) v += lw[m] * s_g[(tl * NF * ND + k * ND + x) * NU + base + m * st];
-            a.gf_out[fp + NFP * (te + NN * (k + NF * x))] = v;
+            for (int q = 0; q < n; q++) w += lw[q] * s_g[(tl * NF * ND + k * ND + c) * NU + base + q * st];
+            gc[k] = w;
+            if (gout) gout[fe + NFP * NN * (k + NF * c)] = w;
           }
+        }, a.ph, t);
+        double fa[NF];
+        visc_flux_dot<ND>(t, nrm, fa);
+#pragma unroll
+        for (int k = 0; k < NF; k++) a.fv_out[fe + NFP * NN * k] = fa[k];
       }
       return;
     } else {
-      // ---- volume divergence (calculate_divergence) ----
-      double div[NF];
-#pragma unroll
-      for (int k = 0; k < NF; k++) div[k] = 0.0;
-      if (active) {
-#pragma unroll
-        for (int d = 0; d < ND; d++) {
-          int pos, lb, fpL, fpR;
-          line_of<P, ND>(p, d, pos, lb, fpL, fpR);
-          const int st = T::stride(d);
-#pragma unroll
-          for (int k = 0; k < NF; k++) {
-            double v = 0.0;
-#pragma unroll
-            for (int m = 0; m < n; m++) v += ops.D[pos][m] * s_f[(el * NF * ND + k * ND + d) * NU + lb + m * st];
-            div[k] += v;
-          }
-        }
-      }
-
-      // ---- own-side viscous normal flux at the flux points, F_v(u_f, grad u_f).n,
-      //      kept in s_q until the interface loop below consumes it ----
-      if constexpr (VISC) {
-        for (int task = tid; task < EPB * NFP; task += BLOCK) {
-          const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
-          if (te >= N) continue;
-          int dir, side, base;
-          face_line<P, ND>(fp, dir, side, base);
-          const int st = T::stride(dir);
-          const double* lw = side ? ops.lR : ops.lL;
-          const long fe = fp + NFP * (long)te;
-          double nrm[ND];
-#pragma unroll
-          for (int l = 0; l < ND; l++) nrm[l] = a.fgeo[l * NN * NFP + fe];
-          double uo[NF], g[NF][ND];
-#pragma unroll
-          for (int k = 0; k < NF; k++) {
-            double v = 0.0;
-#pragma unroll
-            for (int m = 0; m < n; m++) v += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
-            uo[k] = v;
-#pragma unroll
-            for (int c = 0; c < ND; c++) {
-              double w = 0.0;
-#pragma unroll
-              for (int q = 0; q < n; q++) w += lw[q] * s_g[(tl * NF * ND + k * ND + c) * NU + base + q * st];
-              g[k][c] = w;
-            }
-          }
-          ViscT<ND> t;
-          visc_tensor<ND>(uo, g, a.ph, t);
-          double fa[NF];
-          visc_flux_dot<ND>(t, nrm, fa);
-#pragma unroll
-          for (int k = 0; k < NF; k++) s_q[(tl * NFP + fp) * NF + k] = fa[k];
-        }
-      }
-
       // ---- interface fluxes and jumps at the flux points
-      //      (common_invFlux/common_viscFlux, extrapolate_totalFlux, daxpy) ----
+      //      (common_invFlux / common_viscFlux, extrapolate_totalFlux, daxpy) ----
+#pragma unroll 1
       for (int task = tid; task < EPB * NFP; task += BLOCK) {
         const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
         if (te >= N) continue;
@@ -356,14 +340,9 @@ fr_tensor_kernel(const StageArgs a) {
         for (int l = 0; l < ND; l++) nrm[l] = a.fgeo[l * NN * NFP + fe];
         const double tdA = a.fgeo[ND * NN * NFP + fe];
         double uo[NF], un[NF], fn[NF];
+        interp_line<n, NF, NU>(s_u + tl * NF * NU, lw, base, st, uo);
 #pragma unroll
-        for (int k = 0; k < NF; k++) {
-          double v = 0.0;
-#pragma unroll
-          for (int m = 0; m < n; m++) v += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
-          uo[k] = v;
-          un[k] = a.uf_in[nb + NFP * NN * k];
-        }
+        for (int k = 0; k < NF; k++) un[k] = a.uf_in[nb + NFP * NN * k];
         rusanov<ND>(uo, un, nrm, a.ph.gamma, fn);
         double jump[NF];
 #pragma unroll
@@ -372,22 +351,13 @@ fr_tensor_kernel(const StageArgs a) {
           double nn = 0.0;
 #pragma unroll
           for (int l = 0; l < ND; l++) nn += nrm[l] * nrm[l];
-          double fa[NF], fb[NF];
 #pragma unroll
-          for (int k = 0; k < NF; k++) fa[k] = s_q[(tl * NFP + fp) * NF + k];
-          {
-            double g[NF][ND];
-#pragma unroll
-            for (int k = 0; k < NF; k++)
-#pragma unroll
-              for (int m = 0; m < ND; m++) g[k][m] = a.gf_in[nb + NFP * NN * (k + NF * m)];
-            ViscT<ND> t;
-            visc_tensor<ND>(un, g, a.ph, t);
-            visc_flux_dot<ND>(t, nrm, fb);
+          for (int k = 0; k < NF; k++) {
+            // own F_v.n and the neighbour's, whose normal is the negated own normal
+            const double fa = a.fv_in[fe + NFP * NN * k];
+            const double fb = -a.fv_in[nb + NFP * NN * k];
+            jump[k] += ldg_normal<ND>(fa, fb, uo[k], un[k], nn, pen, a.ph.tau) * tdA;
           }
-#pragma unroll
-          for (int k = 0; k < NF; k++)
-            jump[k] += ldg_normal<ND>(fa[k], fb[k], uo[k], un[k], nn, pen, a.ph.tau) * tdA;
         }
         // minus the normal transformed discontinuous flux (opp_1 = opp_0 o tnorm)
         const double sgn = side ? 1.0 : -1.0;
@@ -401,15 +371,24 @@ fr_tensor_kernel(const StageArgs a) {
       }
       __syncthreads();
 
-      // ---- correction lift (calculate_corrected_divergence) and output ----
+      // ---- divergence of the transformed flux (calculate_divergence) plus the
+      //      correction lift (calculate_corrected_divergence), then output ----
       if (active) {
+        double div[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) div[k] = 0.0;
 #pragma unroll
         for (int d = 0; d < ND; d++) {
           int pos, lb, fpL, fpR;
           line_of<P, ND>(p, d, pos, lb, fpL, fpR);
+          const int st = T::stride(d);
 #pragma unroll
-          for (int k = 0; k < NF; k++)
-            div[k] += ops.gR[pos] * s_q[(el * NFP + fpR) * NF + k] - ops.gL[pos] * s_q[(el * NFP + fpL) * NF + k];
+          for (int k = 0; k < NF; k++) {
+            double v = 0.0;
+#pragma unroll
+            for (int m = 0; m < n; m++) v += ops.D[pos][m] * s_f[(el * NF * ND + k * ND + d) * NU + lb + m * st];
+            div[k] += v + (ops.gR[pos] * s_q[(el * NFP + fpR) * NF + k] - ops.gL[pos] * s_q[(el * NFP + fpL) * NF + k]);
+          }
         }
         const long pe = p + NU * (long)e;
         if (MODE == MODE_RESID || a.div) {
@@ -445,20 +424,16 @@ fr_tensor_kernel(const StageArgs a) {
       if constexpr (MODE == MODE_ADVANCE) {
         __syncthreads();
         // ---- traces of the new state for the next stage ----
+#pragma unroll 1
         for (int task = tid; task < EPB * NFP; task += BLOCK) {
           const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
           if (te >= N) continue;
           int dir, side, base;
           face_line<P, ND>(fp, dir, side, base);
-          const int st = T::stride(dir);
-          const double* lw = side ? ops.lR : ops.lL;
+          double uo[NF];
+          interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
 #pragma unroll
-          for (int k = 0; k < NF; k++) {
-            double v = 0.0;
-#pragma unroll
-            for (int m = 0; m < n; m++) v += lw[m] * s_u[(tl * NF + k) * NU + base + m * st];
-            a.uf_out[fp + NFP * (te + NN * k)] = v;
-          }
+          for (int k = 0; k < NF; k++) a.uf_out[fp + NFP * (te + NN * k)] = uo[k];
         }
       }
     }

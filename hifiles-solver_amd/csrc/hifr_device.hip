@@ -21,7 +21,7 @@ struct hifr_ctx {
   Ops1D ops;
   bool have_ops, have_metrics, have_faces, have_state;
   // device buffers
-  double *u, *res, *div, *uf[2], *gf, *detj, *jg, *fgeo;
+  double *u, *res, *div, *uf[2], *fv, *gf, *detj, *jg, *fgeo;
   int* nbr;
   double* red;  // reduction scratch
   int cur;      // which uf buffer holds the traces of the current state
@@ -59,25 +59,31 @@ struct Cfg {
   static constexpr int EPB = hifr::TP<P, ND, 1>::NU >= 128 ? 1 : 128 / hifr::TP<P, ND, 1>::NU;
 };
 
-template <int P, int ND, bool VISC, int MODE>
+template <int P, int ND, int VIS, int MODE>
 hipError_t launch_one(const StageArgs& a, hipStream_t s) {
   constexpr int EPB = Cfg<P, ND>::EPB;
   using T = hifr::TP<P, ND, EPB>;
   const int grid = (a.N + EPB - 1) / EPB;
-  hipLaunchKernelGGL((hifr::fr_tensor_kernel<P, ND, EPB, VISC, MODE>), dim3(grid), dim3(T::BLOCK), 0, s, a);
+  hipLaunchKernelGGL((hifr::fr_tensor_kernel<P, ND, EPB, VIS, MODE>), dim3(grid), dim3(T::BLOCK), 0, s, a);
   return hipGetLastError();
 }
 
+// vis: 0 Euler, 1 Navier-Stokes with constant viscosity, 2 Sutherland viscosity
 template <int P, int ND>
-hipError_t launch_mode(int mode, bool visc, const StageArgs& a, hipStream_t s) {
+hipError_t launch_mode(int mode, int vis, const StageArgs& a, hipStream_t s) {
   using namespace hifr;
   switch (mode) {
-    case MODE_TRACE_U: return launch_one<P, ND, false, MODE_TRACE_U>(a, s);
-    case MODE_TRACE_GRAD: return launch_one<P, ND, true, MODE_TRACE_GRAD>(a, s);
+    case MODE_TRACE_U: return launch_one<P, ND, 0, MODE_TRACE_U>(a, s);
+    case MODE_TRACE_GRAD:
+      return vis == 2 ? launch_one<P, ND, 2, MODE_TRACE_GRAD>(a, s) : launch_one<P, ND, 1, MODE_TRACE_GRAD>(a, s);
     case MODE_RESID:
-      return visc ? launch_one<P, ND, true, MODE_RESID>(a, s) : launch_one<P, ND, false, MODE_RESID>(a, s);
+      return vis == 2 ? launch_one<P, ND, 2, MODE_RESID>(a, s)
+           : vis == 1 ? launch_one<P, ND, 1, MODE_RESID>(a, s)
+                      : launch_one<P, ND, 0, MODE_RESID>(a, s);
     default:
-      return visc ? launch_one<P, ND, true, MODE_ADVANCE>(a, s) : launch_one<P, ND, false, MODE_ADVANCE>(a, s);
+      return vis == 2 ? launch_one<P, ND, 2, MODE_ADVANCE>(a, s)
+           : vis == 1 ? launch_one<P, ND, 1, MODE_ADVANCE>(a, s)
+                      : launch_one<P, ND, 0, MODE_ADVANCE>(a, s);
   }
 }
 
@@ -88,7 +94,7 @@ bool order_supported(int ele_type, int P) {
 }
 
 hipError_t launch(hifr_ctx* c, int mode, const StageArgs& a) {
-  const bool v = c->cfg.viscous != 0;
+  const int v = c->cfg.viscous == 0 ? 0 : (c->cfg.fix_vis == 1.0 ? 1 : 2);
   const int P = c->cfg.order;
   if (c->nd == 3) {
     switch (P) {
@@ -152,10 +158,13 @@ hifr_status trace_u(hifr_ctx* c) {
   return timed_launch(c, hifr::MODE_TRACE_U, 0, a);
 }
 
-hifr_status trace_grad(hifr_ctx* c) {
+// write_grad: also store the corrected gradient at the flux points (for
+// hifr_download; the residual itself only needs the viscous normal fluxes)
+hifr_status trace_grad(hifr_ctx* c, bool write_grad) {
   StageArgs a = base_args(c);
   a.uf_in = c->uf[c->cur];
-  a.gf_out = c->gf;
+  a.fv_out = c->fv;
+  a.gf_out = write_grad ? c->gf : nullptr;
   return timed_launch(c, hifr::MODE_TRACE_GRAD, 1, a);
 }
 
@@ -240,6 +249,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
       {(void**)&c->div, sizeof(double) * nu * N * nf},
       {(void**)&c->uf[0], sizeof(double) * nfp * N * nf},
       {(void**)&c->uf[1], sizeof(double) * nfp * N * nf},
+      {(void**)&c->fv, cfg->viscous ? sizeof(double) * nfp * N * nf : 8},
       {(void**)&c->gf, cfg->viscous ? sizeof(double) * nfp * N * nf * nd : 8},
       {(void**)&c->detj, sizeof(double) * nu * N},
       {(void**)&c->jg, sizeof(double) * nd * nd * nu * N},
@@ -264,7 +274,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
 
 hifr_status hifr_destroy(hifr_ctx* c) {
   if (!c) return HIFR_OK;
-  void* ps[] = {c->u, c->res, c->div, c->uf[0], c->uf[1], c->gf, c->detj, c->jg, c->fgeo, c->nbr, c->red};
+  void* ps[] = {c->u, c->res, c->div, c->uf[0], c->uf[1], c->fv, c->gf, c->detj, c->jg, c->fgeo, c->nbr, c->red};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -436,12 +446,12 @@ hifr_status hifr_residual(hifr_ctx* c) {
   hifr_status s = check_ready(c);
   if (s != HIFR_OK) return s;
   if (c->cfg.viscous) {
-    s = trace_grad(c);
+    s = trace_grad(c, true);
     if (s != HIFR_OK) return s;
   }
   StageArgs a = base_args(c);
   a.uf_in = c->uf[c->cur];
-  a.gf_in = c->gf;
+  a.fv_in = c->fv;
   a.div = c->div;
   return timed_launch(c, hifr::MODE_RESID, 2, a);
 }
@@ -454,13 +464,13 @@ static hifr_status advance_impl(hifr_ctx* c, int stage, int adv_type, double dt,
   if (adv_type != 0 && adv_type != 3) return HIFR_ERR_UNSUPPORTED;
   if (adv_type == 3 && (stage < 0 || stage > 4)) return HIFR_ERR_ARG;
   if (c->cfg.viscous) {
-    hifr_status s = trace_grad(c);
+    hifr_status s = trace_grad(c, false);
     if (s != HIFR_OK) return s;
   }
   StageArgs a = base_args(c);
   a.uf_in = c->uf[c->cur];
   a.uf_out = c->uf[1 - c->cur];
-  a.gf_in = c->gf;
+  a.fv_in = c->fv;
   a.div = write_div ? c->div : nullptr;
   a.adv_type = adv_type;
   a.dt = dt;
@@ -612,4 +622,46 @@ extern "C" hifr_status hifr_norm_residual(hifr_ctx* c, int norm_type, double* ou
     out[k] = v;
   }
   return nan ? HIFR_ERR_NAN : HIFR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// streaming copy used to calibrate byte counters
+
+namespace {
+
+__global__ void __launch_bounds__(256) stream_copy_kernel(double* __restrict__ dst, const double* __restrict__ src, long n) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256L) dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" hifr_status hifr_calibrate_copy(long n, int reps, double* ms) {
+  if (n <= 0 || reps <= 0 || !ms) return HIFR_ERR_ARG;
+  double *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, sizeof(double) * n));
+  if (hipMalloc(&b, sizeof(double) * n) != hipSuccess) {
+    (void)hipFree(a);
+    return HIFR_ERR_DEVICE;
+  }
+  hifr_status st = HIFR_OK;
+  hipEvent_t e0, e1;
+  if (hipMemset(a, 0, sizeof(double) * n) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess) {
+    st = HIFR_ERR_DEVICE;
+  } else {
+    const int grid = 256 * 8 * 4;
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(grid), dim3(256), 0, 0, b, a, n);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; r++) hipLaunchKernelGGL(stream_copy_kernel, dim3(grid), dim3(256), 0, 0, b, a, n);
+    (void)hipEventRecord(e1, 0);
+    if (hipEventSynchronize(e1) != hipSuccess) st = HIFR_ERR_DEVICE;
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms = t / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return st;
 }

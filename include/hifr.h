@@ -44,7 +44,8 @@ enum {
   HIFR_RK_REGISTER = 1,     /* disu_upts(1)     (Nu,N,nf)      */
   HIFR_DIV_TCONF_UPTS = 2,  /* div_tconf_upts(0)(Nu,N,nf)     after hifr_residual */
   HIFR_DISU_FPTS = 3,       /* disu_fpts        (Nfp,N,nf)     current traces */
-  HIFR_GRAD_DISU_FPTS = 4   /* grad_disu_fpts   (Nfp,N,nf,d)   after hifr_residual (viscous) */
+  HIFR_GRAD_DISU_FPTS = 4   /* grad_disu_fpts   (Nfp,N,nf,d)   after hifr_residual (viscous; not
+                                                                 refreshed by hifr_advance/hifr_run) */
 };
 
 /* Physical and numerical parameters after non-dimensionalisation
@@ -123,6 +124,12 @@ hifr_status hifr_synchronize(hifr_ctx* ctx);
  * each bracketed by its own events.  Arrays may be NULL. */
 hifr_status hifr_run_timed(hifr_ctx* ctx, int n_steps, int adv_type, double dt, double* total_ms,
                            double* kernel_ms, int* kernel_launches);
+
+/* Measurement aid (no counterpart in the reference): allocates two buffers of
+ * n_doubles, runs a plain streaming copy kernel `reps` times on the default
+ * stream and returns the average kernel time in *ms_per_copy.  Used to calibrate
+ * HBM byte counters and to state the achievable copy bandwidth. */
+hifr_status hifr_calibrate_copy(long n_doubles, int reps, double* ms_per_copy);
 
 /* Sizes for callers that allocate host buffers. */
 hifr_status hifr_sizes(const hifr_ctx* ctx, int* n_upts, int* n_fpts, int* n_fields, int* n_dims);
