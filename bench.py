@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Throughput of the FR residual + RK45 update on one MI355X.
+"""Throughput of the FR residual + RK45 update on one MI355X, FP64 and mixed precision.
 
 Workload (the headline configuration): 3-D compressible Navier-Stokes,
 Taylor-Green initial field (Mach 0.1, Re 1600, Pr 0.71) plus a seeded 1e-3
@@ -10,7 +10,10 @@ DOF x stages per second.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 
-Prints one JSON line.  `roofline` is for the dominant kernel (flux, divergence
+Prints one JSON line.  `value` is the FP64 path; `mixed` holds the same
+measurement of the mixed-precision path (FP32 arithmetic and traces, FP64 state
+and RK update) with its error against the FP64 state after the same steps.
+`roofline` is for the dominant kernel (flux, divergence
 and RK update, one launch per stage): algorithmic bytes per launch / its average
 duration from HIP events recorded around every launch in the timed region.
 `cpu_baseline` is the oracle (the C restatement of the reference's dense-operator
@@ -37,8 +40,9 @@ HBM_PEAK_GBS = 8000.0
 TWO_PI = 6.2831853071795862
 
 
-def deck(order, n_steps):
-    return hifr.deck_text(viscous=1, order=order, vcjh_scheme_hexa=1, dt=1e-3, adv_type=3, tau=1.0, pen_fact=0.5,
+def deck(order, n_steps, metrics=0, precision=0):
+    return hifr.deck_text(hifr_metrics=metrics, hifr_precision=precision,
+                          viscous=1, order=order, vcjh_scheme_hexa=1, dt=1e-3, adv_type=3, tau=1.0, pen_fact=0.5,
                           gamma=1.4, prandtl=0.71, Mach_free_stream=0.1, nx_free_stream=1.0,
                           Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, fix_vis=1,
                           dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI, n_steps=n_steps)
@@ -50,27 +54,36 @@ def tgv_state(pos, seed=1234):
     return u * (1.0 + frmesh.seeded_perturbation(u.shape, 1e-3, seed))
 
 
-def algorithmic_bytes(order, nd=3):
+def algorithmic_bytes(order, nd=3, per_element=False, mixed=False):
     """HBM bytes per element that the two kernels of a Navier-Stokes stage must
-    move (FP64 values, int32 neighbour table), counting every array once:
+    move, counting every array once:
       gradient-trace kernel: read u, neighbour u_f, JGinv, detJ, face normals,
                              neighbour index; write the viscous normal flux
                              traces F_v.n (nf per flux point);
       flux/RK kernel:        read u, RK register, neighbour u_f, own and neighbour
                              F_v.n, JGinv, detJ, face normal + tdA, neighbour
-                             index; write u, RK register, next u_f."""
+                             index; write u, RK register, next u_f.
+    u, RK register, detJ, normals and tdA are FP64; traces and JGinv are FP64, or
+    FP32 on the mixed path; the neighbour index is int32.  Metrics are per
+    solution point / flux point, or with per_element (every element affine) one
+    JGinv + detJ per element and one normal + tdA per face."""
     n = order + 1
     nu, nfp, nf = n ** nd, 2 * nd * n ** (nd - 1), nd + 2
-    grad = 8 * (nu * nf + nfp * nf + nu * nd * nd + nu + nfp * nd + nfp * nf) + 4 * nfp
-    rk = 8 * (4 * nu * nf + nfp * nf + 2 * nfp * nf + nu * nd * nd + nu + nfp * (nd + 1) + nfp * nf) + 4 * nfp
+    t = 4 if mixed else 8
+    nv = 1 if per_element else nu
+    vol = t * nd * nd * nv + 8 * nv
+    fpts = 2 * nd if per_element else nfp
+    grad = 8 * nu * nf + t * nfp * nf + vol + 8 * fpts * nd + t * nfp * nf + 4 * nfp
+    rk = 8 * 4 * nu * nf + t * nfp * nf + t * 2 * nfp * nf + vol + 8 * fpts * (nd + 1) + t * nfp * nf + 4 * nfp
     return grad, rk
 
 
 # HBM traffic per element of each stage kernel at 32^3, P=3, from rocprofv3 --pmc
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the factors read off a
-# 1 GiB copy kernel (profiles/fr_stage_tgv32_p3_fp64.md).
-MEASURED_TRAFFIC = {3: {"grad": (227408.0 * 2.0 + 122880.0) * 1024 / 32768,
-                        "rk": (388554.0 * 2.0 + 297642.7) * 1024 / 32768}}
+# 1 GiB copy kernel (profiles/fr_stage_tgv32_p3_fp64.md); keyed by (order,
+# per-element metrics).
+MEASURED_TRAFFIC = {(3, False): {"grad": (227408.0 * 2.0 + 122880.0) * 1024 / 32768,
+                                 "rk": (388554.0 * 2.0 + 297642.7) * 1024 / 32768}}
 
 
 def survey_model_bytes(order, nd=3):
@@ -113,6 +126,8 @@ def main():
     ap.add_argument("--n", type=int, default=32)
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--per-point-metrics", action="store_true",
+                    help="keep per-point metrics on the device even though the mesh is affine")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy kernel of MIB MiB (byte-counter calibration) and exit")
     args = ap.parse_args()
@@ -133,8 +148,57 @@ def main():
                           "GBs": 2 * nd * 8 / (ms.value * 1e-3) / 1e9}), flush=True)
         return
     n, order = args.n, args.order
+    fp64 = measure(args, "fp64")
+    mixed = measure(args, "mixed")
+    s, r = fp64["solver"], fp64
+    l2, mx = rel_errors(mixed["solver"].get("disu_upts"), s.get("disu_upts"))
+    traffic = MEASURED_TRAFFIC.get((order, r["per_element"])) if n == 32 else None
+    rec = {
+        "metric": METRIC, "value": round(r["value"], 4), "unit": "GDOF-stage/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3-D Navier-Stokes Taylor-Green (Mach 0.1, Re 1600), %d^3 periodic hexes, P=%d, "
+                               "Rusanov+LDG, RK45 dt=1e-3, FP64" % (n, order),
+                   "n_eles": s.n_eles, "order": order, "dof": r["dof"], "stages_per_step": 5,
+                   "metrics": "per element (affine elements)" if r["per_element"] else "per point"},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic["rk"] * s.n_eles) if traffic else None,
+                     "kernel": "fr_tensor_kernel ADVANCE (flux, divergence, RK update, next traces)",
+                     "bytes_per_launch": r["rk_b"] * s.n_eles, "bytes_per_element": r["rk_b"],
+                     "avg_launch_ms": round(r["avg_rk_ms"], 5)},
+        "stage": stage_block(r, order, s, traffic),
+        "mixed": {"value": round(mixed["value"], 4), "unit": "GDOF-stage/s",
+                  "speedup_vs_fp64": round(mixed["value"] / r["value"], 3),
+                  "ms_per_step": round(mixed["wall"] * 1e3 / args.steps, 4),
+                  "rel_l2_err_vs_fp64": [float("%.3e" % v) for v in l2],
+                  "rel_max_err_vs_fp64": [float("%.3e" % v) for v in mx],
+                  "err_after_steps": args.warmup + args.steps,
+                  "roofline": {"achieved": round(mixed["achieved"], 1), "frac": round(mixed["achieved"] / HBM_PEAK_GBS, 4),
+                               "bytes_per_element": mixed["rk_b"], "avg_launch_ms": round(mixed["avg_rk_ms"], 5)},
+                  "stage": stage_block(mixed, order, mixed["solver"], None)},
+        "residual_L1": [float("%.6e" % v) for v in r["norms"]],
+        "setup_s": round(r["setup"], 2),
+    }
+    if not args.no_cpu_baseline:
+        rec["cpu_baseline"] = cpu_baseline(s, order)
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
+def rel_errors(a, b):
+    """Relative L2 and max error per conserved variable of a against b."""
+    d = (a - b).reshape(-1, a.shape[-1])
+    ref = b.reshape(-1, b.shape[-1])
+    return np.sqrt((d ** 2).sum(0) / (ref ** 2).sum(0)), np.abs(d).max(0) / np.abs(ref).max(0)
+
+
+def measure(args, precision):
+    n, order = args.n, args.order
     t_setup = time.perf_counter()
-    s = hifr.Solver.box(deck(order, args.steps), hifr.HEX, (n, n, n), (TWO_PI,) * 3, (0.0,) * 3)
+    s = hifr.Solver.box(deck(order, args.steps, int(args.per_point_metrics), int(precision == "mixed")),
+                        hifr.HEX, (n, n, n), (TWO_PI,) * 3, (0.0,) * 3)
     s.set_state(tgv_state(s.get("pos_upts")))
     t_setup = time.perf_counter() - t_setup
     s.run(args.warmup)
@@ -145,47 +209,29 @@ def main():
     wall = time.perf_counter() - t0
     norms = s.norms(1)
     if not np.all(np.isfinite(norms)):
-        raise SystemExit("non-finite residual")
+        raise SystemExit("non-finite residual (%s)" % precision)
     dof = s.nu * s.n_eles * s.nf
-    stages = 5 * args.steps
-    value = dof * stages / wall / 1e9
-    grad_b, rk_b = algorithmic_bytes(order)
+    per_element = s.metrics_per_element()
+    grad_b, rk_b = algorithmic_bytes(order, per_element=per_element, mixed=precision == "mixed")
     avg_rk_ms = kms[2] / kl[2]
     avg_grad_ms = kms[1] / kl[1] if kl[1] else 0.0
-    achieved = rk_b * s.n_eles / (avg_rk_ms * 1e-3) / 1e9
-    stage_bytes = (grad_b + rk_b) * s.n_eles
-    stage_ms = avg_rk_ms + avg_grad_ms
-    traffic = MEASURED_TRAFFIC.get(order) if n == 32 else None
-    rec = {
-        "metric": METRIC, "value": round(value, 4), "unit": "GDOF-stage/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "3-D Navier-Stokes Taylor-Green (Mach 0.1, Re 1600), %d^3 periodic hexes, P=%d, "
-                               "Rusanov+LDG, RK45 dt=1e-3, FP64" % (n, order),
-                   "n_eles": s.n_eles, "order": order, "dof": dof, "stages_per_step": 5},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": round(traffic["rk"] * s.n_eles) if traffic else None,
-                     "kernel": "fr_tensor_kernel ADVANCE (flux, divergence, RK update, next traces)",
-                     "bytes_per_launch": rk_b * s.n_eles, "bytes_per_element": rk_b,
-                     "avg_launch_ms": round(avg_rk_ms, 5)},
-        "stage": {"avg_grad_kernel_ms": round(avg_grad_ms, 5), "avg_rk_kernel_ms": round(avg_rk_ms, 5),
-                  "grad_kernel_bytes_per_element": grad_b,
-                  "grad_kernel_traffic": round(traffic["grad"] * s.n_eles) if traffic else None,
-                  "algorithmic_bytes": stage_bytes,
-                  "achieved_GBs": round(stage_bytes / (stage_ms * 1e-3) / 1e9, 1),
-                  "three_phase_model_bytes_per_dof": round(survey_model_bytes(order) / s.nu / s.nf, 2),
-                  "three_phase_model_GBs": round(survey_model_bytes(order) * s.n_eles / (stage_ms * 1e-3) / 1e9, 1),
-                  "event_total_ms": round(total_ms, 3)},
-        "residual_L1": [float("%.6e" % v) for v in norms],
-        "setup_s": round(t_setup, 2),
-    }
-    if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(s, order)
-    else:
-        rec["cpu_baseline"] = None
-    print(json.dumps(rec), flush=True)
+    return {"solver": s, "value": dof * 5 * args.steps / wall / 1e9, "wall": wall, "dof": dof, "norms": norms,
+            "per_element": per_element, "grad_b": grad_b, "rk_b": rk_b, "avg_rk_ms": avg_rk_ms,
+            "avg_grad_ms": avg_grad_ms, "achieved": rk_b * s.n_eles / (avg_rk_ms * 1e-3) / 1e9,
+            "total_ms": total_ms, "setup": t_setup}
 
+
+def stage_block(r, order, s, traffic):
+    stage_ms = r["avg_rk_ms"] + r["avg_grad_ms"]
+    stage_bytes = (r["grad_b"] + r["rk_b"]) * s.n_eles
+    return {"avg_grad_kernel_ms": round(r["avg_grad_ms"], 5), "avg_rk_kernel_ms": round(r["avg_rk_ms"], 5),
+            "grad_kernel_bytes_per_element": r["grad_b"],
+            "grad_kernel_traffic": round(traffic["grad"] * s.n_eles) if traffic else None,
+            "algorithmic_bytes": stage_bytes,
+            "achieved_GBs": round(stage_bytes / (stage_ms * 1e-3) / 1e9, 1),
+            "three_phase_model_bytes_per_dof": round(survey_model_bytes(order) / s.nu / s.nf, 2),
+            "three_phase_model_GBs": round(survey_model_bytes(order) * s.n_eles / (stage_ms * 1e-3) / 1e9, 1),
+            "event_total_ms": round(r["total_ms"], 3)}
 
 if __name__ == "__main__":
     main()

@@ -1,0 +1,17 @@
+// Kernel dispatch of the element-block FR kernels: (dimension, order, mode,
+// viscosity law, metric storage) -> one template instantiation.  The FP64 and
+// the mixed-precision instantiations live in separate translation units
+// (kernels_fp64.hip, kernels_mixed.hip) that include fr_launch_impl.hpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fr_tensor_kernels.hpp"
+
+namespace hifr {
+
+// vis: 0 Euler, 1 Navier-Stokes with constant viscosity, 2 Sutherland viscosity;
+// aff: metrics per element / per face
+hipError_t launch_fp64(int nd, int P, int mode, int vis, bool aff, const StageArgsT<double>& a, hipStream_t s);
+hipError_t launch_mixed(int nd, int P, int mode, int vis, bool aff, const StageArgsT<float>& a, hipStream_t s);
+
+}  // namespace hifr

@@ -2,6 +2,13 @@
 // fluxes, the Rusanov interface flux and the LDG common solution / viscous flux.
 // Formulas follow src/flux.cpp:36-431 and src/inters.cpp:311-365, 550-678.
 //
+// Every function is templated on the arithmetic type T (double for the FP64
+// path, float for the mixed-precision path).  The mixed path works with the
+// deviation du = u - u_ref from a constant reference state at rest (density
+// rho_r, energy E_r): the *_dev variants return F(u) - F(u_ref), written so that
+// no O(p_ref) term is formed and then cancelled.  A constant flux has zero
+// divergence and zero interface jump, so the residual is unchanged.
+//
 // Interface functions work from the point of view of one element ("own" side,
 // outward normal n) and only ever need normal components f.n of the fluxes, which
 // keeps the register footprint of the face kernels small.  Both elements sharing
@@ -16,23 +23,60 @@ struct Phys {
   double gamma, prandtl, mu_inf, rt_inf, c_sth, fix_vis, tau, pen_fact, const_src;
 };
 
+// Reference state at rest of the mixed-precision path (rho_r, E_r; p_r and the
+// total enthalpy H_r = E_r + p_r follow).
+template <class T>
+struct RefState {
+  T rho, E, p, H;
+};
+
 // Euler flux projected on an arbitrary vector w: out[k] = sum_m f(k,m) w_m
 // (src/flux.cpp:36-128).
-template <int ND>
-__device__ __forceinline__ void inv_flux_dot(const double* u, const double* w, double gamma, double* out) {
-  const double ir = 1.0 / u[0];
-  double vw = 0.0, ke = 0.0;
+template <int ND, class T>
+__device__ __forceinline__ void inv_flux_dot(const T* u, const T* w, T gamma, T* out) {
+  const T ir = T(1) / u[0];
+  T vw = 0, ke = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
-    const double v = u[1 + i] * ir;
+    const T v = u[1 + i] * ir;
     vw += v * w[i];
     ke += v * v;
   }
-  const double p = (gamma - 1.0) * (u[ND + 1] - 0.5 * u[0] * ke);
+  const T p = (gamma - T(1)) * (u[ND + 1] - T(0.5) * u[0] * ke);
   out[0] = u[0] * vw;
 #pragma unroll
   for (int i = 0; i < ND; i++) out[1 + i] = u[1 + i] * vw + p * w[i];
   out[ND + 1] = vw * (u[ND + 1] + p);
+}
+
+// Deviation form: du = u - u_ref (u_ref at rest); out = (F(u) - F(u_ref)) . w.
+// p - p_r = (gamma-1)(dE - rho|v|^2/2); the energy flux is v.w (H_r + dE + p - p_r).
+template <int ND, class T>
+__device__ __forceinline__ void inv_flux_dot_dev(const T* du, const T* w, T gamma, const RefState<T>& r, T* out) {
+  const T ir = T(1) / (r.rho + du[0]);
+  T vw = 0, mv = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    const T v = du[1 + i] * ir;
+    vw += v * w[i];
+    mv += du[1 + i] * v;
+  }
+  const T dp = (gamma - T(1)) * (du[ND + 1] - T(0.5) * mv);
+  out[0] = du[1] * w[0];
+#pragma unroll
+  for (int i = 1; i < ND; i++) out[0] += du[1 + i] * w[i];
+#pragma unroll
+  for (int i = 0; i < ND; i++) out[1 + i] = du[1 + i] * vw + dp * w[i];
+  out[ND + 1] = vw * (r.H + (du[ND + 1] + dp));
+}
+
+// Full state from a deviation (for the viscous terms of the mixed path).
+template <int ND, class T>
+__device__ __forceinline__ void add_ref(const T* du, const RefState<T>& r, T* u) {
+  u[0] = r.rho + du[0];
+#pragma unroll
+  for (int i = 0; i < ND; i++) u[1 + i] = du[1 + i];
+  u[ND + 1] = r.E + du[ND + 1];
 }
 
 // Dynamic viscosity.  SUTH = false: constant viscosity mu_inf (fix_vis = 1, every
@@ -40,115 +84,70 @@ __device__ __forceinline__ void inv_flux_dot(const double* u, const double* w, d
 // (src/flux.cpp:308-310), with rt^1.5 formed as rt*sqrt(rt), within an ulp of pow.
 // The choice is a template parameter so that the constant-viscosity kernels do not
 // carry the register footprint of the Sutherland branch.
-template <bool SUTH>
-__device__ __forceinline__ double viscosity(const Phys& ph, double inte) {
+template <bool SUTH, class T>
+__device__ __forceinline__ T viscosity(const Phys& ph, T inte) {
   if constexpr (!SUTH) {
-    return ph.mu_inf;
+    return T(ph.mu_inf);
   } else {
-    const double rt = (ph.gamma - 1.0) * inte / ph.rt_inf;
-    const double mu = ph.mu_inf * (rt * sqrt(rt)) * (1.0 + ph.c_sth) / (rt + ph.c_sth);
-    return mu + ph.fix_vis * (ph.mu_inf - mu);
+    const T rt = (T(ph.gamma) - T(1)) * inte / T(ph.rt_inf);
+    const T mu = T(ph.mu_inf) * (rt * sqrt(rt)) * (T(1) + T(ph.c_sth)) / (rt + T(ph.c_sth));
+    return mu + T(ph.fix_vis) * (T(ph.mu_inf) - mu);
   }
 }
 
 // Laminar viscous stress tau and energy flux column of the Navier-Stokes viscous
 // flux (src/flux.cpp:263-431): F_v(1+i, m) = -tau(i,m), F_v(nf-1, m) = qe(m).
-template <int ND>
+template <int ND, class T = double>
 struct ViscT {
-  double tau[ND][ND];
-  double qe[ND];
+  T tau[ND][ND];
+  T qe[ND];
 };
 
-// g(k, m) = d u_k / d x_m (physical gradient of the conservative state)
-template <int ND, bool SUTH>
-__device__ __forceinline__ void visc_tensor(const double* u, const double (*g)[ND], const Phys& ph, ViscT<ND>& t) {
-  const double rho = u[0], irho = 1.0 / rho;
-  double v[ND];
-  double ke = 0.0;
-#pragma unroll
-  for (int i = 0; i < ND; i++) {
-    v[i] = u[1 + i] * irho;
-    ke += v[i] * v[i];
-  }
-  const double inte = u[ND + 1] * irho - 0.5 * ke;
-  const double mu = viscosity<SUTH>(ph, inte);
-  double dv[ND][ND];
-  double de[ND];
-#pragma unroll
-  for (int m = 0; m < ND; m++) {
-    double rdv = 0.0;
-#pragma unroll
-    for (int i = 0; i < ND; i++) {
-      dv[i][m] = (g[1 + i][m] - g[0][m] * v[i]) * irho;
-      rdv += v[i] * dv[i][m];
-    }
-    const double dke = 0.5 * ke * g[0][m] + rho * rdv;
-    de[m] = (g[ND + 1][m] - dke - g[0][m] * inte) * irho;
-  }
-  double diag = 0.0;
-#pragma unroll
-  for (int i = 0; i < ND; i++) diag += dv[i][i];
-  diag = diag / 3.0;
-#pragma unroll
-  for (int i = 0; i < ND; i++)
-#pragma unroll
-    for (int j = 0; j < ND; j++)
-      t.tau[i][j] = (i == j) ? 2.0 * mu * (dv[i][i] - diag) : mu * (dv[i][j] + dv[j][i]);
-  const double kq = mu / ph.prandtl * ph.gamma;
-#pragma unroll
-  for (int m = 0; m < ND; m++) {
-    double w = 0.0;
-#pragma unroll
-    for (int i = 0; i < ND; i++) w += v[i] * t.tau[i][m];
-    t.qe[m] = -(w + kq * de[m]);
-  }
-}
-
-
-// Same as visc_tensor with the gradient supplied one column (d/dx_m of all
-// fields) at a time by gcol(m, out[NF]); keeps only one column live.
-template <int ND, bool SUTH, class GCol>
-__device__ __forceinline__ void visc_tensor_cols(const double* u, GCol&& gcol, const Phys& ph, ViscT<ND>& t) {
+// Viscous stress and heat-flux column from the state u and the physical gradient
+// g(k, m) = d u_k / d x_m, supplied one column (d/dx_m of all fields) at a time by
+// gcol(m, out[NF]) so that only one column is live.
+template <int ND, bool SUTH, class T, class GCol>
+__device__ __forceinline__ void visc_tensor_cols(const T* u, GCol&& gcol, const Phys& ph, ViscT<ND, T>& t) {
   constexpr int NF = ND + 2;
-  const double rho = u[0], irho = 1.0 / rho;
-  double v[ND];
-  double ke = 0.0;
+  const T rho = u[0], irho = T(1) / rho;
+  T v[ND];
+  T ke = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
     v[i] = u[1 + i] * irho;
     ke += v[i] * v[i];
   }
-  const double inte = u[ND + 1] * irho - 0.5 * ke;
-  const double mu = viscosity<SUTH>(ph, inte);
-  double dv[ND][ND];
-  double de[ND];
+  const T inte = u[ND + 1] * irho - T(0.5) * ke;
+  const T mu = viscosity<SUTH, T>(ph, inte);
+  T dv[ND][ND];
+  T de[ND];
 #pragma unroll
   for (int m = 0; m < ND; m++) {
-    double g[NF];
+    T g[NF];
     gcol(m, g);
-    double rdv = 0.0;
+    T rdv = 0;
 #pragma unroll
     for (int i = 0; i < ND; i++) {
       dv[i][m] = (g[1 + i] - g[0] * v[i]) * irho;
       rdv += v[i] * dv[i][m];
     }
-    const double dke = 0.5 * ke * g[0] + rho * rdv;
+    const T dke = T(0.5) * ke * g[0] + rho * rdv;
     de[m] = (g[ND + 1] - dke - g[0] * inte) * irho;
     __builtin_amdgcn_sched_barrier(0);
   }
-  double diag = 0.0;
+  T diag = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) diag += dv[i][i];
-  diag = diag / 3.0;
+  diag = diag / T(3);
 #pragma unroll
   for (int i = 0; i < ND; i++)
 #pragma unroll
     for (int j = 0; j < ND; j++)
-      t.tau[i][j] = (i == j) ? 2.0 * mu * (dv[i][i] - diag) : mu * (dv[i][j] + dv[j][i]);
-  const double kq = mu / ph.prandtl * ph.gamma;
+      t.tau[i][j] = (i == j) ? T(2) * mu * (dv[i][i] - diag) : mu * (dv[i][j] + dv[j][i]);
+  const T kq = mu / T(ph.prandtl) * T(ph.gamma);
 #pragma unroll
   for (int m = 0; m < ND; m++) {
-    double w = 0.0;
+    T w = 0;
 #pragma unroll
     for (int i = 0; i < ND; i++) w += v[i] * t.tau[i][m];
     t.qe[m] = -(w + kq * de[m]);
@@ -156,57 +155,83 @@ __device__ __forceinline__ void visc_tensor_cols(const double* u, GCol&& gcol, c
 }
 
 // Viscous flux projected on w: out[k] = sum_m F_v(k,m) w_m
-template <int ND>
-__device__ __forceinline__ void visc_flux_dot(const ViscT<ND>& t, const double* w, double* out) {
-  out[0] = 0.0;
+template <int ND, class T>
+__device__ __forceinline__ void visc_flux_dot(const ViscT<ND, T>& t, const T* w, T* out) {
+  out[0] = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
-    double s = 0.0;
+    T s = 0;
 #pragma unroll
     for (int m = 0; m < ND; m++) s += t.tau[i][m] * w[m];
     out[1 + i] = -s;
   }
-  double s = 0.0;
+  T s = 0;
 #pragma unroll
   for (int m = 0; m < ND; m++) s += t.qe[m] * w[m];
   out[ND + 1] = s;
 }
 
 // Rusanov common normal flux for outward normal n (src/inters.cpp:311-365, static mesh).
-template <int ND>
-__device__ __forceinline__ void rusanov(const double* uo, const double* un, const double* n, double gamma,
-                                        double* fn) {
+template <int ND, class T>
+__device__ __forceinline__ void rusanov(const T* uo, const T* un, const T* n, T gamma, T* fn) {
   constexpr int NF = ND + 2;
-  double fo[NF], fb[NF];
-  inv_flux_dot<ND>(uo, n, gamma, fo);
-  inv_flux_dot<ND>(un, n, gamma, fb);
-  double vno = 0.0, vnn = 0.0, ko = 0.0, kn = 0.0;
-  const double iro = 1.0 / uo[0], irn = 1.0 / un[0];
+  T fo[NF], fb[NF];
+  inv_flux_dot<ND, T>(uo, n, gamma, fo);
+  inv_flux_dot<ND, T>(un, n, gamma, fb);
+  T vno = 0, vnn = 0, ko = 0, kn = 0;
+  const T iro = T(1) / uo[0], irn = T(1) / un[0];
 #pragma unroll
   for (int i = 0; i < ND; i++) {
-    const double a = uo[1 + i] * iro, b = un[1 + i] * irn;
+    const T a = uo[1 + i] * iro, b = un[1 + i] * irn;
     vno += a * n[i];
     vnn += b * n[i];
     ko += a * a;
     kn += b * b;
   }
-  const double po = (gamma - 1.0) * (uo[ND + 1] - 0.5 * uo[0] * ko);
-  const double pn = (gamma - 1.0) * (un[ND + 1] - 0.5 * un[0] * kn);
-  const double vn_av = fabs(0.5 * (vno + vnn));
-  const double c_av = sqrt(gamma * (po + pn) / (uo[0] + un[0]));
-  const double eig = fabs(vn_av + c_av);
+  const T po = (gamma - T(1)) * (uo[ND + 1] - T(0.5) * uo[0] * ko);
+  const T pn = (gamma - T(1)) * (un[ND + 1] - T(0.5) * un[0] * kn);
+  const T vn_av = fabs(T(0.5) * (vno + vnn));
+  const T c_av = sqrt(gamma * (po + pn) / (uo[0] + un[0]));
+  const T eig = fabs(vn_av + c_av);
 #pragma unroll
-  for (int k = 0; k < NF; k++) fn[k] = 0.5 * ((fo[k] + fb[k]) - eig * (un[k] - uo[k]));
+  for (int k = 0; k < NF; k++) fn[k] = T(0.5) * ((fo[k] + fb[k]) - eig * (un[k] - uo[k]));
+}
+
+// Rusanov flux of the mixed path on deviations: (F_c - F(u_ref)) . n.
+template <int ND, class T>
+__device__ __forceinline__ void rusanov_dev(const T* duo, const T* dun, const T* n, T gamma, const RefState<T>& r,
+                                            T* fn) {
+  constexpr int NF = ND + 2;
+  T fo[NF], fb[NF];
+  inv_flux_dot_dev<ND, T>(duo, n, gamma, r, fo);
+  inv_flux_dot_dev<ND, T>(dun, n, gamma, r, fb);
+  const T ro = r.rho + duo[0], rn = r.rho + dun[0];
+  const T iro = T(1) / ro, irn = T(1) / rn;
+  T vno = 0, vnn = 0, mvo = 0, mvn = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    const T a = duo[1 + i] * iro, b = dun[1 + i] * irn;
+    vno += a * n[i];
+    vnn += b * n[i];
+    mvo += duo[1 + i] * a;
+    mvn += dun[1 + i] * b;
+  }
+  const T po = r.p + (gamma - T(1)) * (duo[ND + 1] - T(0.5) * mvo);
+  const T pn = r.p + (gamma - T(1)) * (dun[ND + 1] - T(0.5) * mvn);
+  const T vn_av = fabs(T(0.5) * (vno + vnn));
+  const T c_av = sqrt(gamma * (po + pn) / (ro + rn));
+  const T eig = fabs(vn_av + c_av);
+#pragma unroll
+  for (int k = 0; k < NF; k++) fn[k] = T(0.5) * ((fo[k] + fb[k]) - eig * (dun[k] - duo[k]));
 }
 
 // LDG viscous common normal flux, interior form (flux_spec 0, src/inters.cpp:550-641):
 //   f_c = 1/2 (f_o + f_n) + pen n [n.(f_o - f_n)] + tau n (u_o - u_n),  fn = f_c . n
 // written with the normal components a = f_o.n, b = f_n.n and nn = n.n:
 //   fn = 1/2 (a + b) + (pen (a - b) + tau (u_o - u_n)) nn
-template <int ND>
-__device__ __forceinline__ double ldg_normal(double a, double b, double uo, double un, double nn, double pen,
-                                             double tau) {
-  return 0.5 * (a + b) + (pen * (a - b) + tau * (uo - un)) * nn;
+template <class T>
+__device__ __forceinline__ T ldg_normal(T a, T b, T uo, T un, T nn, T pen, T tau) {
+  return T(0.5) * (a + b) + (pen * (a - b) + tau * (uo - un)) * nn;
 }
 
 }  // namespace hifr

@@ -23,9 +23,17 @@
 //
 // Layouts (first index fastest): u, res, div (Nu,N,nf); u_f and the viscous normal
 // flux traces (Nfp,N,nf); optional gradient traces (Nfp,N,nf,d) = the reference's
-// grad_disu_fpts; detJ (Nu,N); JGinv component-major (d*d, Nu*N).
+// grad_disu_fpts.  Metrics, per point: detJ (Nu,N), JGinv component-major
+// (d*d, Nu*N), face geometry (d+1, Nfp*N); per element when every element is
+// affine (AFF): detJ (N), JGinv (d*d, N), face geometry (d+1, 2d*N).
+//
+// MIX selects the mixed-precision variant: arithmetic, LDS, JGinv and the traces
+// in FP32 on the deviation u - u_ref (see fr_physics.hpp); the state u, the RK
+// register, detJ, the face normals / tdA and the RK update stay in FP64.
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "fr_physics.hpp"
 
@@ -34,33 +42,40 @@ namespace hifr {
 constexpr int MAXN1 = 8;
 
 // 1-D factors of the element operators (Lagrange basis on the solution points).
-struct Ops1D {
-  double lL[MAXN1], lR[MAXN1];  // l_m(-1), l_m(+1): interpolation to the faces
-  double D[MAXN1][MAXN1];       // D[a][m] = l'_m(xi_a): differentiation
-  double gL[MAXN1], gR[MAXN1];  // g_L'(xi_a), g_R'(xi_a): VCJH correction derivatives
+template <class R>
+struct Ops1DT {
+  R lL[MAXN1], lR[MAXN1];  // l_m(-1), l_m(+1): interpolation to the faces
+  R D[MAXN1][MAXN1];       // D[a][m] = l'_m(xi_a): differentiation
+  R gL[MAXN1], gR[MAXN1];  // g_L'(xi_a), g_R'(xi_a): VCJH correction derivatives
 };
+using Ops1D = Ops1DT<double>;
 
 enum { MODE_TRACE_U = 0, MODE_TRACE_GRAD = 1, MODE_RESID = 2, MODE_ADVANCE = 3 };
 
-struct StageArgs {
+// R: arithmetic / trace type (double, or float on the mixed path)
+template <class R>
+struct StageArgsT {
   int N;
   double* u;             // state (read; written by ADVANCE)
   double* res;           // RK register (ADVANCE)
   double* div;           // div_tconf_upts output (RESID; ADVANCE if non-null)
-  const double* uf_in;   // traces of the current state
-  double* uf_out;        // traces written (TRACE_U, ADVANCE)
-  const double* fv_in;   // viscous normal flux traces (RESID/ADVANCE, viscous)
-  double* fv_out;        // viscous normal flux traces written (TRACE_GRAD)
-  double* gf_out;        // optional gradient traces (TRACE_GRAD, may be null)
-  const double* detj;
-  const double* jg;
+  const R* uf_in;        // traces of the current state (deviations on the mixed path)
+  R* uf_out;             // traces written (TRACE_U, ADVANCE)
+  const R* fv_in;        // viscous normal flux traces (RESID/ADVANCE, viscous)
+  R* fv_out;             // viscous normal flux traces written (TRACE_GRAD)
+  R* gf_out;             // optional gradient traces (TRACE_GRAD, may be null)
+  const double* detj;    // per point, or per element (AFF)
+  const R* jg;           // per point, or per element (AFF)
   const int* nbr;        // neighbour trace index per element flux point; bit 31: LDG switch -
-  const double* fgeo;    // own-side outward normal (d) and tdA, component-major
+  const double* fgeo;    // own-side outward normal (d) and tdA, component-major; per flux
+                         // point, or per face (AFF)
   double rk_a, rk_b, dt;
   int adv_type;          // 3 RK45 stage, 0 forward Euler
-  Ops1D ops;
+  Ops1DT<R> ops;
   Phys ph;
+  RefState<double> ref;  // mixed path: reference state of the deviations
 };
+using StageArgs = StageArgsT<double>;
 
 template <int P, int ND, int EPB>
 struct TP {
@@ -117,7 +132,7 @@ __device__ __forceinline__ void line_of(int p, int dir, int& pos, int& base, int
   }
 }
 
-template <int P, int ND, int EPB, int VIS, int MODE>
+template <int P, int ND, int EPB, int MODE>
 struct SmemLayout {
   using T = TP<P, ND, EPB>;
   static constexpr int U = EPB * T::NF * T::NU;
@@ -128,11 +143,11 @@ struct SmemLayout {
 };
 
 // Interpolate NF fields from LDS rows (stride NU apart) to the end of a line.
-template <int n, int NF, int NU>
-__device__ __forceinline__ void interp_line(const double* rows, const double* lw, int base, int st, double* out) {
+template <int n, int NF, int NU, class R>
+__device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base, int st, R* out) {
 #pragma unroll
   for (int k = 0; k < NF; k++) {
-    double v = 0.0;
+    R v = 0;
 #pragma unroll
     for (int m = 0; m < n; m++) v += lw[m] * rows[k * NU + base + m * st];
     out[k] = v;
@@ -140,19 +155,40 @@ __device__ __forceinline__ void interp_line(const double* rows, const double* lw
 }
 
 // VIS: 0 Euler, 1 Navier-Stokes with constant viscosity, 2 with Sutherland's law.
-template <int P, int ND, int EPB, int VIS, int MODE>
+// AFF: metrics stored per element / per face (every element affine).
+// MIX: mixed precision (FP32 arithmetic on deviations, FP64 state and update).
+template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX>
 __global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK))
-fr_tensor_kernel(const StageArgs a) {
+fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
+  using R = std::conditional_t<MIX, float, double>;
   constexpr bool VISC = VIS != 0;
   constexpr bool SUTH = VIS == 2;
   using T = TP<P, ND, EPB>;
-  using S = SmemLayout<P, ND, EPB, VIS, MODE>;
+  using S = SmemLayout<P, ND, EPB, MODE>;
   constexpr int n = T::n, NU = T::NU, NFP = T::NFP, NF = T::NF, BLOCK = T::BLOCK;
-  __shared__ double smem[S::TOTAL];
-  double* s_u = smem;        // [EPB][NF][NU]
-  double* s_q = s_u + S::U;  // [EPB][NFP][NF]   LDG delta, then interface jumps
-  double* s_g = s_q + S::Q;  // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD)
-  double* s_f = s_g + S::G;  // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE)
+  constexpr int NFPF = T::NFPF, NFACE = 2 * ND;
+  __shared__ R smem[S::TOTAL];
+  R* s_u = smem;        // [EPB][NF][NU]  u, or u - u_ref (MIX)
+  R* s_q = s_u + S::U;  // [EPB][NFP][NF]   LDG delta, then interface jumps
+  R* s_g = s_q + S::Q;  // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD)
+  R* s_f = s_g + S::G;  // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE)
+  const R gam = R(a.ph.gamma);
+  RefState<R> ref;
+  ref.rho = R(a.ref.rho);
+  ref.E = R(a.ref.E);
+  ref.p = R(a.ref.p);
+  ref.H = R(a.ref.H);
+  // reference value of field k (zero momentum); only used by MIX
+  auto ref_of = [&](int k) -> double { return k == 0 ? a.ref.rho : (k == NF - 1 ? a.ref.E : 0.0); };
+  // full state from what the LDS / traces hold
+  auto full = [&](const R* w, R* out) {
+    if constexpr (MIX) {
+      add_ref<ND, R>(w, ref, out);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; k++) out[k] = w[k];
+    }
+  };
 
   const int N = a.N;
   const long NN = (long)N;
@@ -161,12 +197,20 @@ fr_tensor_kernel(const StageArgs a) {
   const int p = tid - el * NU;
   const int e = blockIdx.x * EPB + el;
   const bool active = (tid < EPB * NU) && (e < N);
-  const Ops1D& ops = a.ops;
+  const Ops1DT<R>& ops = a.ops;
+  // metric index of (point p of element e) and of (flux point fp of element te)
+  auto vol_idx = [&](long pe, int ee) -> long { return AFF ? (long)ee : pe; };
+  auto geo_idx = [&](int fpt, int ee) -> long { return AFF ? (long)(fpt / NFPF) + NFACE * (long)ee
+                                                             : (long)fpt + NFP * (long)ee; };
+  constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
   // ---- load u ----
   if (active) {
 #pragma unroll
-    for (int k = 0; k < NF; k++) s_u[(el * NF + k) * NU + p] = a.u[p + NU * (e + NN * k)];
+    for (int k = 0; k < NF; k++) {
+      const double v = a.u[p + NU * (e + NN * k)];
+      s_u[(el * NF + k) * NU + p] = MIX ? R(v - ref_of(k)) : R(v);
+    }
   }
   __syncthreads();
 
@@ -178,7 +222,7 @@ fr_tensor_kernel(const StageArgs a) {
       if (te >= N) continue;
       int dir, side, base;
       face_line<P, ND>(fp, dir, side, base);
-      double uo[NF];
+      R uo[NF];
       interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
 #pragma unroll
       for (int k = 0; k < NF; k++) a.uf_out[fp + NFP * (te + NN * k)] = uo[k];
@@ -195,14 +239,14 @@ fr_tensor_kernel(const StageArgs a) {
         int dir, side, base;
         face_line<P, ND>(fp, dir, side, base);
         int nb = a.nbr[fp + NFP * te];
-        const double pen = nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact;
+        const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
         nb &= 0x7fffffff;
-        double uo[NF];
+        R uo[NF];
         interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-          const double un = a.uf_in[nb + NFP * NN * k];
-          const double uc = 0.5 * (uo[k] + un) - pen * (uo[k] - un);
+          const R un = a.uf_in[nb + NFP * NN * k];
+          const R uc = R(0.5) * (uo[k] + un) - pen * (uo[k] - un);
           s_q[(tl * NFP + fp) * NF + k] = uc - uo[k];
         }
       }
@@ -212,43 +256,45 @@ fr_tensor_kernel(const StageArgs a) {
     // ---- per solution point: corrected physical gradient, flux, transformed flux ----
     if (active) {
       const long pe = p + NU * (long)e;
-      double Jg[ND][ND];
+      const long vi = vol_idx(pe, e);
+      constexpr long VOL = AFF ? 1 : NU;
+      R Jg[ND][ND];
 #pragma unroll
       for (int l = 0; l < ND; l++)
 #pragma unroll
-        for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * NU + pe];
-      double u_own[NF];
+        for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * VOL + vi];
+      R u_own[NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) u_own[k] = s_u[(el * NF + k) * NU + p];
-      ViscT<ND> vt;
+      ViscT<ND, R> vt;
       if constexpr (VISC) {
-        const double inv_detj = 1.0 / a.detj[pe];
+        const R inv_detj = R(1.0 / a.detj[vi]);
         int pos[ND], lb[ND], fL[ND], fR[ND];
 #pragma unroll
         for (int d = 0; d < ND; d++) line_of<P, ND>(p, d, pos[d], lb[d], fL[d], fR[d]);
         // physical gradient column by column: d/dx_m = sum_d (JGinv(d,m)/J) d/dr_d
         // (reference gradient + LDG correction, src/eles.cpp:1667-1870)
-        double gr[NF][ND];
+        R gr[NF][ND];
 #pragma unroll
         for (int k = 0; k < NF; k++)
 #pragma unroll
           for (int d = 0; d < ND; d++) {
             const int st = T::stride(d);
-            double r = 0.0;
+            R r = 0;
 #pragma unroll
             for (int q = 0; q < n; q++) r += ops.D[pos[d]][q] * s_u[(el * NF + k) * NU + lb[d] + q * st];
             r += ops.gL[pos[d]] * s_q[(el * NFP + fL[d]) * NF + k] + ops.gR[pos[d]] * s_q[(el * NFP + fR[d]) * NF + k];
             gr[k][d] = r;
           }
-        double Ji[ND][ND];
+        R Ji[ND][ND];
 #pragma unroll
         for (int d = 0; d < ND; d++)
 #pragma unroll
           for (int m = 0; m < ND; m++) Ji[d][m] = Jg[d][m] * inv_detj;
-        auto gcol = [&](int m, double* gc) {
+        auto gcol = [&](int m, R* gc) {
 #pragma unroll
           for (int k = 0; k < NF; k++) {
-            double v = 0.0;
+            R v = 0;
 #pragma unroll
             for (int d = 0; d < ND; d++) v += gr[k][d] * Ji[d][m];
             gc[k] = v;
@@ -257,24 +303,27 @@ fr_tensor_kernel(const StageArgs a) {
         if constexpr (MODE == MODE_TRACE_GRAD) {
 #pragma unroll
           for (int m = 0; m < ND; m++) {
-            double gc[NF];
+            R gc[NF];
             gcol(m, gc);
 #pragma unroll
             for (int k = 0; k < NF; k++) s_g[(el * NF * ND + k * ND + m) * NU + p] = gc[k];
           }
         } else {
-          visc_tensor_cols<ND, SUTH>(u_own, gcol, a.ph, vt);
+          R uu[NF];
+          full(u_own, uu);
+          visc_tensor_cols<ND, SUTH, R>(uu, gcol, a.ph, vt);
         }
       }
       if constexpr (MODE != MODE_TRACE_GRAD) {
         // transformed flux tdisf(k,l) = sum_m JGinv(l,m) F(k,m)  (src/eles.cpp:1397-1405, 2437-2444)
 #pragma unroll
         for (int l = 0; l < ND; l++) {
-          double fi[NF];
-          inv_flux_dot<ND>(u_own, Jg[l], a.ph.gamma, fi);
+          R fi[NF];
+          if constexpr (MIX) inv_flux_dot_dev<ND, R>(u_own, Jg[l], gam, ref, fi);
+          else inv_flux_dot<ND, R>(u_own, Jg[l], gam, fi);
           if constexpr (VISC) {
-            double fv[NF];
-            visc_flux_dot<ND>(vt, Jg[l], fv);
+            R fv[NF];
+            visc_flux_dot<ND, R>(vt, Jg[l], fv);
 #pragma unroll
             for (int k = 0; k < NF; k++) fi[k] += fv[k];
           }
@@ -295,27 +344,30 @@ fr_tensor_kernel(const StageArgs a) {
         int dir, side, base;
         face_line<P, ND>(fp, dir, side, base);
         const int st = T::stride(dir);
-        const double* lw = side ? ops.lR : ops.lL;
+        const R* lw = side ? ops.lR : ops.lL;
         const long fe = fp + NFP * (long)te;
-        double uo[NF];
+        R uo[NF];
         interp_line<n, NF, NU>(s_u + tl * NF * NU, lw, base, st, uo);
-        double nrm[ND];
+        const long gi = geo_idx(fp, te);
+        R nrm[ND];
 #pragma unroll
-        for (int l = 0; l < ND; l++) nrm[l] = a.fgeo[l * NN * NFP + fe];
-        double* gout = a.gf_out;
-        ViscT<ND> t;
-        visc_tensor_cols<ND, SUTH>(uo, [&](int c, double* gc) {
+        for (int l = 0; l < ND; l++) nrm[l] = R(a.fgeo[l * NN * GEO_PER_ELE + gi]);
+        R* gout = a.gf_out;
+        ViscT<ND, R> t;
+        R uu[NF];
+        full(uo, uu);
+        visc_tensor_cols<ND, SUTH, R>(uu, [&](int c, R* gc) {
 #pragma unroll
           for (int k = 0; k < NF; k++) {
-            double w = 0.0;
+            R w = 0;
 #pragma unroll
             for (int q = 0; q < n; q++) w += lw[q] * s_g[(tl * NF * ND + k * ND + c) * NU + base + q * st];
             gc[k] = w;
             if (gout) gout[fe + NFP * NN * (k + NF * c)] = w;
           }
         }, a.ph, t);
-        double fa[NF];
-        visc_flux_dot<ND>(t, nrm, fa);
+        R fa[NF];
+        visc_flux_dot<ND, R>(t, nrm, fa);
 #pragma unroll
         for (int k = 0; k < NF; k++) a.fv_out[fe + NFP * NN * k] = fa[k];
       }
@@ -330,40 +382,42 @@ fr_tensor_kernel(const StageArgs a) {
         int dir, side, base;
         face_line<P, ND>(fp, dir, side, base);
         const int st = T::stride(dir);
-        const double* lw = side ? ops.lR : ops.lL;
+        const R* lw = side ? ops.lR : ops.lL;
         const long fe = fp + NFP * (long)te;
         int nb = a.nbr[fe];
-        const double pen = nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact;
+        const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
         nb &= 0x7fffffff;
-        double nrm[ND];
+        const long gi = geo_idx(fp, te);
+        R nrm[ND];
 #pragma unroll
-        for (int l = 0; l < ND; l++) nrm[l] = a.fgeo[l * NN * NFP + fe];
-        const double tdA = a.fgeo[ND * NN * NFP + fe];
-        double uo[NF], un[NF], fn[NF];
+        for (int l = 0; l < ND; l++) nrm[l] = R(a.fgeo[l * NN * GEO_PER_ELE + gi]);
+        const R tdA = R(a.fgeo[ND * NN * GEO_PER_ELE + gi]);
+        R uo[NF], un[NF], fn[NF];
         interp_line<n, NF, NU>(s_u + tl * NF * NU, lw, base, st, uo);
 #pragma unroll
         for (int k = 0; k < NF; k++) un[k] = a.uf_in[nb + NFP * NN * k];
-        rusanov<ND>(uo, un, nrm, a.ph.gamma, fn);
-        double jump[NF];
+        if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
+        else rusanov<ND, R>(uo, un, nrm, gam, fn);
+        R jump[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) jump[k] = fn[k] * tdA;
         if constexpr (VISC) {
-          double nn = 0.0;
+          R nn = 0;
 #pragma unroll
           for (int l = 0; l < ND; l++) nn += nrm[l] * nrm[l];
 #pragma unroll
           for (int k = 0; k < NF; k++) {
             // own F_v.n and the neighbour's, whose normal is the negated own normal
-            const double fa = a.fv_in[fe + NFP * NN * k];
-            const double fb = -a.fv_in[nb + NFP * NN * k];
-            jump[k] += ldg_normal<ND>(fa, fb, uo[k], un[k], nn, pen, a.ph.tau) * tdA;
+            const R fa = a.fv_in[fe + NFP * NN * k];
+            const R fb = -a.fv_in[nb + NFP * NN * k];
+            jump[k] += ldg_normal<R>(fa, fb, uo[k], un[k], nn, pen, R(a.ph.tau)) * tdA;
           }
         }
         // minus the normal transformed discontinuous flux (opp_1 = opp_0 o tnorm)
-        const double sgn = side ? 1.0 : -1.0;
+        const R sgn = side ? R(1) : R(-1);
 #pragma unroll
         for (int k = 0; k < NF; k++) {
-          double v = 0.0;
+          R v = 0;
 #pragma unroll
           for (int m = 0; m < n; m++) v += lw[m] * s_f[(tl * NF * ND + k * ND + dir) * NU + base + m * st];
           s_q[(tl * NFP + fp) * NF + k] = jump[k] - sgn * v;
@@ -374,9 +428,9 @@ fr_tensor_kernel(const StageArgs a) {
       // ---- divergence of the transformed flux (calculate_divergence) plus the
       //      correction lift (calculate_corrected_divergence), then output ----
       if (active) {
-        double div[NF];
+        R div[NF];
 #pragma unroll
-        for (int k = 0; k < NF; k++) div[k] = 0.0;
+        for (int k = 0; k < NF; k++) div[k] = 0;
 #pragma unroll
         for (int d = 0; d < ND; d++) {
           int pos, lb, fpL, fpR;
@@ -384,7 +438,7 @@ fr_tensor_kernel(const StageArgs a) {
           const int st = T::stride(d);
 #pragma unroll
           for (int k = 0; k < NF; k++) {
-            double v = 0.0;
+            R v = 0;
 #pragma unroll
             for (int m = 0; m < n; m++) v += ops.D[pos][m] * s_f[(el * NF * ND + k * ND + d) * NU + lb + m * st];
             div[k] += v + (ops.gR[pos] * s_q[(el * NFP + fpR) * NF + k] - ops.gL[pos] * s_q[(el * NFP + fpL) * NF + k]);
@@ -393,30 +447,42 @@ fr_tensor_kernel(const StageArgs a) {
         const long pe = p + NU * (long)e;
         if (MODE == MODE_RESID || a.div) {
 #pragma unroll
-          for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = div[k];
+          for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = double(div[k]);
         }
         if constexpr (MODE == MODE_ADVANCE) {
-          const double detj = a.detj[pe];
+          const double detj = a.detj[vol_idx(pe, e)];
           if (a.adv_type == 3) {
             // low-storage RK45 stage (src/eles.cpp:1155-1178)
 #pragma unroll
             for (int k = 0; k < NF; k++) {
               const long idx = pe + NU * NN * k;
-              const double rhs = -div[k] / detj + a.ph.const_src;
+              const double rhs = -double(div[k]) / detj + a.ph.const_src;
               const double r = a.rk_a * a.res[idx] + a.dt * rhs;
               a.res[idx] = r;
-              const double unew = s_u[(el * NF + k) * NU + p] + a.rk_b * r;
-              a.u[idx] = unew;
-              s_u[(el * NF + k) * NU + p] = unew;
+              if constexpr (MIX) {
+                const double unew = a.u[idx] + a.rk_b * r;
+                a.u[idx] = unew;
+                s_u[(el * NF + k) * NU + p] = R(unew - ref_of(k));
+              } else {
+                const double unew = s_u[(el * NF + k) * NU + p] + a.rk_b * r;
+                a.u[idx] = unew;
+                s_u[(el * NF + k) * NU + p] = unew;
+              }
             }
           } else {
             // forward Euler (src/eles.cpp:1067-1079)
 #pragma unroll
             for (int k = 0; k < NF; k++) {
               const long idx = pe + NU * NN * k;
-              const double unew = s_u[(el * NF + k) * NU + p] - a.dt * (div[k] / detj - a.ph.const_src);
-              a.u[idx] = unew;
-              s_u[(el * NF + k) * NU + p] = unew;
+              if constexpr (MIX) {
+                const double unew = a.u[idx] - a.dt * (double(div[k]) / detj - a.ph.const_src);
+                a.u[idx] = unew;
+                s_u[(el * NF + k) * NU + p] = R(unew - ref_of(k));
+              } else {
+                const double unew = s_u[(el * NF + k) * NU + p] - a.dt * (div[k] / detj - a.ph.const_src);
+                a.u[idx] = unew;
+                s_u[(el * NF + k) * NU + p] = unew;
+              }
             }
           }
         }
@@ -430,7 +496,7 @@ fr_tensor_kernel(const StageArgs a) {
           if (te >= N) continue;
           int dir, side, base;
           face_line<P, ND>(fp, dir, side, base);
-          double uo[NF];
+          R uo[NF];
           interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
 #pragma unroll
           for (int k = 0; k < NF; k++) a.uf_out[fp + NFP * (te + NN * k)] = uo[k];

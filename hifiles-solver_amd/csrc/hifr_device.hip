@@ -9,19 +9,25 @@
 #include <vector>
 
 #include "../../include/hifr.h"
-#include "fr_tensor_kernels.hpp"
+#include "fr_launch.hpp"
 
 using hifr::Ops1D;
-using hifr::StageArgs;
+using hifr::StageArgsT;
 
 struct hifr_ctx {
   hifr_config cfg;
   int nd, nf, n1, nu, nfp, nfpf, N;
   hipStream_t stream;
   Ops1D ops;
+  hifr::Ops1DT<float> opsf;
   bool have_ops, have_metrics, have_faces, have_state;
+  bool affine;  // metrics kept per element / per face
+  bool mixed;   // HIFR_MIXED: FP32 traces / JGinv / arithmetic on deviations from ref
+  size_t tsz;   // bytes of a trace / JGinv value (8, or 4 when mixed)
+  hifr::RefState<double> ref;
   // device buffers
-  double *u, *res, *div, *uf[2], *fv, *gf, *detj, *jg, *fgeo;
+  double *u, *res, *div, *detj, *fgeo;
+  void *uf[2], *fv, *gf, *jg;  // element type double, or float when mixed
   int* nbr;
   double* red;  // reduction scratch
   int cur;      // which uf buffer holds the traces of the current state
@@ -50,82 +56,27 @@ hifr_status dev_err(hipError_t e, const char* what) {
     if (_s != HIFR_OK) return _s;                      \
   } while (0)
 
-// ---------------------------------------------------------------------------
-// launch helpers
-
-template <int P, int ND>
-struct Cfg {
-  // elements per workgroup: about 128 threads per block
-  static constexpr int EPB = hifr::TP<P, ND, 1>::NU >= 128 ? 1 : 128 / hifr::TP<P, ND, 1>::NU;
-};
-
-template <int P, int ND, int VIS, int MODE>
-hipError_t launch_one(const StageArgs& a, hipStream_t s) {
-  constexpr int EPB = Cfg<P, ND>::EPB;
-  using T = hifr::TP<P, ND, EPB>;
-  const int grid = (a.N + EPB - 1) / EPB;
-  hipLaunchKernelGGL((hifr::fr_tensor_kernel<P, ND, EPB, VIS, MODE>), dim3(grid), dim3(T::BLOCK), 0, s, a);
-  return hipGetLastError();
-}
-
-// vis: 0 Euler, 1 Navier-Stokes with constant viscosity, 2 Sutherland viscosity
-template <int P, int ND>
-hipError_t launch_mode(int mode, int vis, const StageArgs& a, hipStream_t s) {
-  using namespace hifr;
-  switch (mode) {
-    case MODE_TRACE_U: return launch_one<P, ND, 0, MODE_TRACE_U>(a, s);
-    case MODE_TRACE_GRAD:
-      return vis == 2 ? launch_one<P, ND, 2, MODE_TRACE_GRAD>(a, s) : launch_one<P, ND, 1, MODE_TRACE_GRAD>(a, s);
-    case MODE_RESID:
-      return vis == 2 ? launch_one<P, ND, 2, MODE_RESID>(a, s)
-           : vis == 1 ? launch_one<P, ND, 1, MODE_RESID>(a, s)
-                      : launch_one<P, ND, 0, MODE_RESID>(a, s);
-    default:
-      return vis == 2 ? launch_one<P, ND, 2, MODE_ADVANCE>(a, s)
-           : vis == 1 ? launch_one<P, ND, 1, MODE_ADVANCE>(a, s)
-                      : launch_one<P, ND, 0, MODE_ADVANCE>(a, s);
-  }
-}
-
 bool order_supported(int ele_type, int P) {
   if (ele_type == HIFR_HEX) return P >= 1 && P <= 5;
   if (ele_type == HIFR_QUAD) return P >= 1 && P <= 6;
   return false;
 }
 
-hipError_t launch(hifr_ctx* c, int mode, const StageArgs& a) {
-  const int v = c->cfg.viscous == 0 ? 0 : (c->cfg.fix_vis == 1.0 ? 1 : 2);
-  const int P = c->cfg.order;
-  if (c->nd == 3) {
-    switch (P) {
-      case 1: return launch_mode<1, 3>(mode, v, a, c->stream);
-      case 2: return launch_mode<2, 3>(mode, v, a, c->stream);
-      case 3: return launch_mode<3, 3>(mode, v, a, c->stream);
-      case 4: return launch_mode<4, 3>(mode, v, a, c->stream);
-      default: return launch_mode<5, 3>(mode, v, a, c->stream);
-    }
-  }
-  switch (P) {
-    case 1: return launch_mode<1, 2>(mode, v, a, c->stream);
-    case 2: return launch_mode<2, 2>(mode, v, a, c->stream);
-    case 3: return launch_mode<3, 2>(mode, v, a, c->stream);
-    case 4: return launch_mode<4, 2>(mode, v, a, c->stream);
-    case 5: return launch_mode<5, 2>(mode, v, a, c->stream);
-    default: return launch_mode<6, 2>(mode, v, a, c->stream);
-  }
-}
+int vis_kind(const hifr_ctx* c) { return c->cfg.viscous == 0 ? 0 : (c->cfg.fix_vis == 1.0 ? 1 : 2); }
 
-StageArgs base_args(hifr_ctx* c) {
-  StageArgs a;
+template <class R>
+StageArgsT<R> base_args(hifr_ctx* c) {
+  StageArgsT<R> a;
   std::memset(&a, 0, sizeof(a));
   a.N = c->N;
   a.u = c->u;
   a.res = c->res;
   a.detj = c->detj;
-  a.jg = c->jg;
+  a.jg = (const R*)c->jg;
   a.nbr = c->nbr;
   a.fgeo = c->fgeo;
-  a.ops = c->ops;
+  if constexpr (std::is_same_v<R, float>) a.ops = c->opsf;
+  else a.ops = c->ops;
   a.ph.gamma = c->cfg.gamma;
   a.ph.prandtl = c->cfg.prandtl;
   a.ph.mu_inf = c->cfg.mu_inf;
@@ -135,16 +86,48 @@ StageArgs base_args(hifr_ctx* c) {
   a.ph.tau = c->cfg.tau;
   a.ph.pen_fact = c->cfg.pen_fact;
   a.ph.const_src = c->cfg.const_src;
+  a.ref = c->ref;
   return a;
 }
 
-hifr_status timed_launch(hifr_ctx* c, int mode, int kind, const StageArgs& a) {
+// Buffers of one stage launch, filled by the callers below.
+struct StageIO {
+  int uf_in = -1, uf_out = -1;  // trace buffer indices
+  bool fv_in = false, fv_out = false, gf_out = false, div = false;
+  int adv_type = 0;
+  double rk_a = 0, rk_b = 0, dt = 0;
+};
+
+template <class R>
+hipError_t launch_io(hifr_ctx* c, int mode, const StageIO& io) {
+  StageArgsT<R> a = base_args<R>(c);
+  a.uf_in = io.uf_in >= 0 ? (const R*)c->uf[io.uf_in] : nullptr;
+  a.uf_out = io.uf_out >= 0 ? (R*)c->uf[io.uf_out] : nullptr;
+  a.fv_in = io.fv_in ? (const R*)c->fv : nullptr;
+  a.fv_out = io.fv_out ? (R*)c->fv : nullptr;
+  a.gf_out = io.gf_out ? (R*)c->gf : nullptr;
+  a.div = io.div ? c->div : nullptr;
+  a.adv_type = io.adv_type;
+  a.rk_a = io.rk_a;
+  a.rk_b = io.rk_b;
+  a.dt = io.dt;
+  if constexpr (std::is_same_v<R, float>)
+    return hifr::launch_mixed(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, a, c->stream);
+  else
+    return hifr::launch_fp64(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, a, c->stream);
+}
+
+hipError_t launch(hifr_ctx* c, int mode, const StageIO& io) {
+  return c->mixed ? launch_io<float>(c, mode, io) : launch_io<double>(c, mode, io);
+}
+
+hifr_status timed_launch(hifr_ctx* c, int mode, int kind, const StageIO& io) {
   if (c->timing) {
     if (c->ev_used + 2 > c->ev.size()) return HIFR_ERR_ARG;
     HIP_TRY(hipEventRecord(c->ev[c->ev_used], c->stream));
     c->ev_kind[c->ev_used / 2] = kind;
   }
-  HIP_TRY(launch(c, mode, a));
+  HIP_TRY(launch(c, mode, io));
   if (c->timing) {
     HIP_TRY(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
     c->ev_used += 2;
@@ -153,19 +136,19 @@ hifr_status timed_launch(hifr_ctx* c, int mode, int kind, const StageArgs& a) {
 }
 
 hifr_status trace_u(hifr_ctx* c) {
-  StageArgs a = base_args(c);
-  a.uf_out = c->uf[c->cur];
-  return timed_launch(c, hifr::MODE_TRACE_U, 0, a);
+  StageIO io;
+  io.uf_out = c->cur;
+  return timed_launch(c, hifr::MODE_TRACE_U, 0, io);
 }
 
 // write_grad: also store the corrected gradient at the flux points (for
 // hifr_download; the residual itself only needs the viscous normal fluxes)
 hifr_status trace_grad(hifr_ctx* c, bool write_grad) {
-  StageArgs a = base_args(c);
-  a.uf_in = c->uf[c->cur];
-  a.fv_out = c->fv;
-  a.gf_out = write_grad ? c->gf : nullptr;
-  return timed_launch(c, hifr::MODE_TRACE_GRAD, 1, a);
+  StageIO io;
+  io.uf_in = c->cur;
+  io.fv_out = true;
+  io.gf_out = write_grad;
+  return timed_launch(c, hifr::MODE_TRACE_GRAD, 1, io);
 }
 
 hifr_status check_ready(hifr_ctx* c) {
@@ -226,7 +209,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   if (!order_supported(cfg->ele_type, cfg->order)) return HIFR_ERR_UNSUPPORTED;
   if (cfg->n_eles <= 0) return HIFR_ERR_ARG;
   if (cfg->riemann_solve_type != 0 || cfg->vis_riemann_solve_type != 0) return HIFR_ERR_UNSUPPORTED;
-  if (cfg->precision != HIFR_FP64) return HIFR_ERR_UNSUPPORTED;
+  if (cfg->precision != HIFR_FP64 && cfg->precision != HIFR_MIXED) return HIFR_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HIFR_ERR_DEVICE;
   hifr_ctx* c = new hifr_ctx();
@@ -242,22 +225,25 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   c->cur = 0;
   c->timing = false;
   c->ev_used = 0;
-  const size_t N = c->N, nu = c->nu, nfp = c->nfp, nf = c->nf, nd = c->nd;
+  c->mixed = cfg->precision == HIFR_MIXED;
+  c->tsz = c->mixed ? sizeof(float) : sizeof(double);
+  c->ref = hifr::RefState<double>{0.0, 0.0, 0.0, 0.0};
+  const size_t N = c->N, nu = c->nu, nfp = c->nfp, nf = c->nf, nd = c->nd, tsz = c->tsz;
   struct { void** p; size_t bytes; } allocs[] = {
       {(void**)&c->u, sizeof(double) * nu * N * nf},
       {(void**)&c->res, sizeof(double) * nu * N * nf},
       {(void**)&c->div, sizeof(double) * nu * N * nf},
-      {(void**)&c->uf[0], sizeof(double) * nfp * N * nf},
-      {(void**)&c->uf[1], sizeof(double) * nfp * N * nf},
-      {(void**)&c->fv, cfg->viscous ? sizeof(double) * nfp * N * nf : 8},
-      {(void**)&c->gf, cfg->viscous ? sizeof(double) * nfp * N * nf * nd : 8},
-      {(void**)&c->detj, sizeof(double) * nu * N},
-      {(void**)&c->jg, sizeof(double) * nd * nd * nu * N},
-      {(void**)&c->fgeo, sizeof(double) * (nd + 1) * nfp * N},
+      {&c->uf[0], tsz * nfp * N * nf},
+      {&c->uf[1], tsz * nfp * N * nf},
+      {&c->fv, cfg->viscous ? tsz * nfp * N * nf : 8},
+      {&c->gf, cfg->viscous ? tsz * nfp * N * nf * nd : 8},
       {(void**)&c->nbr, sizeof(int) * nfp * N},
       {(void**)&c->red, sizeof(double) * 4096},
   };
   for (auto& al : allocs) *al.p = nullptr;
+  c->detj = c->fgeo = nullptr;  // sized by hifr_set_metrics / hifr_set_interfaces
+  c->jg = nullptr;
+  c->affine = false;
   for (auto& al : allocs) {
     if (hipMalloc(al.p, al.bytes) != hipSuccess) {
       hifr_destroy(c);
@@ -349,34 +335,114 @@ hifr_status hifr_set_operators(hifr_ctx* c, const double* opp_0, const double* o
     return HIFR_ERR_NOT_TENSOR;
   }
   c->ops = o;
+  hifr::Ops1DT<float>& f = c->opsf;
+  for (int a = 0; a < hifr::MAXN1; a++) {
+    f.lL[a] = float(o.lL[a]);
+    f.lR[a] = float(o.lR[a]);
+    f.gL[a] = float(o.gL[a]);
+    f.gR[a] = float(o.gR[a]);
+    for (int m = 0; m < hifr::MAXN1; m++) f.D[a][m] = float(o.D[a][m]);
+  }
   c->have_ops = true;
   return HIFR_OK;
 }
+
+}  // extern "C"
+
+namespace {
+
+// True when every element's volume metrics are the same at all its solution points
+// and every face's normal and tdA the same at all its flux points, to `rel`
+// relative to the largest magnitude of the quantity in that element / face.
+bool all_affine(const hifr_ctx* c, const double* detjac, const double* JGinv, const double* tdA,
+                const double* norm, double rel) {
+  const size_t N = c->N, nu = c->nu, nfp = c->nfp, nd = c->nd, nfpf = c->nfpf;
+  for (size_t e = 0; e < N; e++) {
+    const double* J = JGinv + nd * nd * nu * e;
+    double mx = 0.0, dev = 0.0;
+    for (size_t p = 0; p < nu; p++)
+      for (size_t k = 0; k < nd * nd; k++) {
+        mx = std::fmax(mx, std::fabs(J[k + nd * nd * p]));
+        dev = std::fmax(dev, std::fabs(J[k + nd * nd * p] - J[k]));
+      }
+    if (!(dev <= rel * mx)) return false;
+    const double* dj = detjac + nu * e;
+    for (size_t p = 0; p < nu; p++)
+      if (!(std::fabs(dj[p] - dj[0]) <= rel * std::fabs(dj[0]))) return false;
+    for (size_t f = 0; f < 2 * nd; f++) {
+      const size_t f0 = f * nfpf + nfp * e;
+      for (size_t q = 0; q < nfpf; q++) {
+        const size_t fe = f0 + q;
+        if (!(std::fabs(tdA[fe] - tdA[f0]) <= rel * std::fabs(tdA[f0]))) return false;
+        for (size_t m = 0; m < nd; m++)
+          if (!(std::fabs(norm[fe + nfp * N * m] - norm[f0 + nfp * N * m]) <= rel)) return false;
+      }
+    }
+  }
+  return true;
+}
+
+template <class V>
+hifr_status realloc_dev(V** p, size_t bytes) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  HIP_TRY(hipMalloc((void**)p, bytes));
+  return HIFR_OK;
+}
+
+}  // namespace
+
+extern "C" {
 
 hifr_status hifr_set_metrics(hifr_ctx* c, const double* detjac, const double* JGinv, const double* tdA,
                              const double* norm) {
   if (!c || !detjac || !JGinv || !tdA || !norm) return HIFR_ERR_ARG;
   const size_t N = c->N, nu = c->nu, nfp = c->nfp, nd = c->nd;
-  HIP_TRY(hipMemcpy(c->detj, detjac, sizeof(double) * nu * N, hipMemcpyHostToDevice));
+  c->have_metrics = false;
+  c->have_faces = false;
+  c->affine = c->cfg.metric_storage == HIFR_METRICS_AUTO && all_affine(c, detjac, JGinv, tdA, norm, 1e-13);
+  // per element: the values at the first solution point; per point: as given
+  const size_t nv = c->affine ? N : nu * N, stride = c->affine ? nu : 1;
+  std::vector<double> dj(nv), jg(nd * nd * nv);
   // JGinv(l,m,j,i) (first index fastest) -> component-major jg[(l*nd+m)][j + nu*i]
-  std::vector<double> jg(nd * nd * nu * N);
-  for (size_t pe = 0; pe < nu * N; pe++)
+  for (size_t v = 0; v < nv; v++) {
+    const size_t pe = v * stride;
+    dj[v] = detjac[pe];
     for (size_t l = 0; l < nd; l++)
-      for (size_t m = 0; m < nd; m++) jg[(l * nd + m) * nu * N + pe] = JGinv[l + nd * m + nd * nd * pe];
-  HIP_TRY(hipMemcpy(c->jg, jg.data(), sizeof(double) * jg.size(), hipMemcpyHostToDevice));
+      for (size_t m = 0; m < nd; m++) jg[(l * nd + m) * nv + v] = JGinv[l + nd * m + nd * nd * pe];
+  }
+  hifr_status st;
+  if ((st = realloc_dev(&c->detj, sizeof(double) * nv)) != HIFR_OK) return st;
+  if ((st = realloc_dev(&c->jg, c->tsz * nd * nd * nv)) != HIFR_OK) return st;
+  HIP_TRY(hipMemcpy(c->detj, dj.data(), sizeof(double) * nv, hipMemcpyHostToDevice));
+  if (c->mixed) {
+    std::vector<float> jf(jg.begin(), jg.end());
+    HIP_TRY(hipMemcpy(c->jg, jf.data(), sizeof(float) * jf.size(), hipMemcpyHostToDevice));
+  } else {
+    HIP_TRY(hipMemcpy(c->jg, jg.data(), sizeof(double) * jg.size(), hipMemcpyHostToDevice));
+  }
   c->h_tdA.assign(tdA, tdA + nfp * N);
   c->h_norm.assign(norm, norm + nfp * N * nd);
   c->have_metrics = true;
-  c->have_faces = false;
+  return HIFR_OK;
+}
+
+hifr_status hifr_metrics_per_element(const hifr_ctx* c, int* per_element) {
+  if (!c || !per_element) return HIFR_ERR_ARG;
+  if (!c->have_metrics) return HIFR_ERR_STATE;
+  *per_element = c->affine ? 1 : 0;
   return HIFR_OK;
 }
 
 hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, const int32_t* right) {
   if (!c || n_pts < 0 || (n_pts > 0 && (!left || !right))) return HIFR_ERR_ARG;
   if (!c->have_metrics) return HIFR_ERR_STATE;
-  const size_t N = c->N, nfp = c->nfp, nd = c->nd, tot = nfp * N;
+  const size_t N = c->N, nfp = c->nfp, nd = c->nd, tot = nfp * N, nfpf = c->nfpf;
+  // face geometry per flux point, or per face when the metrics are per element
+  const size_t gtot = c->affine ? 2 * nd * N : tot;
+  auto gidx = [&](size_t fe) { return c->affine ? fe / nfpf : fe; };
   std::vector<int> nbr(tot, -1);
-  std::vector<double> geo((nd + 1) * tot, 0.0);
+  std::vector<double> geo((nd + 1) * gtot, 0.0);
   std::vector<char> seen(tot, 0);
   for (int q = 0; q < n_pts; q++) {
     const long L = left[q], R = right[q];
@@ -389,12 +455,15 @@ hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, con
     const bool neg_l = sw < 0.;
     nbr[L] = (int)R | (neg_l ? (int)0x80000000u : 0);
     nbr[R] = (int)L | (!neg_l ? (int)0x80000000u : 0);
+    // one normal (the left one) serves both sides of the point, as in the
+    // reference; per-face storage takes the face's first visited point
+    const size_t gl = gidx(L), gr = gidx(R);
     for (size_t m = 0; m < nd; m++) {
-      geo[m * tot + L] = nl[m];
-      geo[m * tot + R] = -nl[m];
+      geo[m * gtot + gl] = nl[m];
+      geo[m * gtot + gr] = -nl[m];
     }
-    geo[nd * tot + L] = c->h_tdA[L];
-    geo[nd * tot + R] = c->h_tdA[R];
+    geo[nd * gtot + gl] = c->h_tdA[L];
+    geo[nd * gtot + gr] = c->h_tdA[R];
   }
   for (size_t i = 0; i < tot; i++)
     if (!seen[i]) {
@@ -403,6 +472,8 @@ hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, con
       return HIFR_ERR_UNSUPPORTED;
     }
   HIP_TRY(hipMemcpy(c->nbr, nbr.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
+  hifr_status st = realloc_dev(&c->fgeo, sizeof(double) * geo.size());
+  if (st != HIFR_OK) return st;
   HIP_TRY(hipMemcpy(c->fgeo, geo.data(), sizeof(double) * geo.size(), hipMemcpyHostToDevice));
   c->have_faces = true;
   return HIFR_OK;
@@ -412,6 +483,20 @@ hifr_status hifr_upload_state(hifr_ctx* c, const double* u) {
   if (!c || !u) return HIFR_ERR_ARG;
   if (!c->have_ops) return HIFR_ERR_STATE;
   const size_t n = (size_t)c->nu * c->N * c->nf;
+  if (c->mixed) {
+    // reference state at rest for the FP32 deviations: the mean density and
+    // total energy of the uploaded state
+    const size_t np = (size_t)c->nu * c->N;
+    double rho = 0.0, E = 0.0;
+    for (size_t i = 0; i < np; i++) {
+      rho += u[i];
+      E += u[i + np * (c->nf - 1)];
+    }
+    rho /= (double)np;
+    E /= (double)np;
+    const double p = (c->cfg.gamma - 1.0) * E;
+    c->ref = hifr::RefState<double>{rho, E, p, E + p};
+  }
   HIP_TRY(hipMemcpyAsync(c->u, u, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemsetAsync(c->res, 0, sizeof(double) * n, c->stream));
   c->cur = 0;
@@ -425,20 +510,35 @@ hifr_status hifr_upload_state(hifr_ctx* c, const double* u) {
 hifr_status hifr_download(hifr_ctx* c, int what, double* host) {
   if (!c || !host) return HIFR_ERR_ARG;
   const size_t N = c->N, nu = c->nu, nfp = c->nfp, nf = c->nf, nd = c->nd;
-  const double* src = nullptr;
-  size_t n = 0;
+  const void* src = nullptr;
+  size_t n = 0, esz = sizeof(double);
   switch (what) {
     case HIFR_DISU_UPTS: src = c->u; n = nu * N * nf; break;
     case HIFR_RK_REGISTER: src = c->res; n = nu * N * nf; break;
     case HIFR_DIV_TCONF_UPTS: src = c->div; n = nu * N * nf; break;
-    case HIFR_DISU_FPTS: src = c->uf[c->cur]; n = nfp * N * nf; break;
+    case HIFR_DISU_FPTS: src = c->uf[c->cur]; n = nfp * N * nf; esz = c->tsz; break;
     case HIFR_GRAD_DISU_FPTS:
       if (!c->cfg.viscous) return HIFR_ERR_ARG;
-      src = c->gf; n = nfp * N * nf * nd; break;
+      src = c->gf; n = nfp * N * nf * nd; esz = c->tsz; break;
     default: return HIFR_ERR_ARG;
   }
-  HIP_TRY(hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  if (esz == sizeof(double)) {
+    HIP_TRY(hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return HIFR_OK;
+  }
+  // mixed path: FP32 buffers; state traces are deviations from the reference state
+  std::vector<float> tmp(n);
+  HIP_TRY(hipMemcpyAsync(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < n; i++) host[i] = tmp[i];
+  if (what == HIFR_DISU_FPTS) {
+    const size_t np = nfp * N;
+    for (size_t i = 0; i < np; i++) {
+      host[i] += c->ref.rho;
+      host[i + np * (nf - 1)] += c->ref.E;
+    }
+  }
   return HIFR_OK;
 }
 
@@ -449,11 +549,11 @@ hifr_status hifr_residual(hifr_ctx* c) {
     s = trace_grad(c, true);
     if (s != HIFR_OK) return s;
   }
-  StageArgs a = base_args(c);
-  a.uf_in = c->uf[c->cur];
-  a.fv_in = c->fv;
-  a.div = c->div;
-  return timed_launch(c, hifr::MODE_RESID, 2, a);
+  StageIO io;
+  io.uf_in = c->cur;
+  io.fv_in = c->cfg.viscous != 0;
+  io.div = true;
+  return timed_launch(c, hifr::MODE_RESID, 2, io);
 }
 
 static hifr_status advance_impl(hifr_ctx* c, int stage, int adv_type, double dt, bool write_div) {
@@ -467,16 +567,16 @@ static hifr_status advance_impl(hifr_ctx* c, int stage, int adv_type, double dt,
     hifr_status s = trace_grad(c, false);
     if (s != HIFR_OK) return s;
   }
-  StageArgs a = base_args(c);
-  a.uf_in = c->uf[c->cur];
-  a.uf_out = c->uf[1 - c->cur];
-  a.fv_in = c->fv;
-  a.div = write_div ? c->div : nullptr;
-  a.adv_type = adv_type;
-  a.dt = dt;
-  a.rk_a = adv_type == 3 ? rk4a[stage] : 0.0;
-  a.rk_b = adv_type == 3 ? rk4b[stage] : 0.0;
-  hifr_status s = timed_launch(c, hifr::MODE_ADVANCE, 2, a);
+  StageIO io;
+  io.uf_in = c->cur;
+  io.uf_out = 1 - c->cur;
+  io.fv_in = c->cfg.viscous != 0;
+  io.div = write_div;
+  io.adv_type = adv_type;
+  io.dt = dt;
+  io.rk_a = adv_type == 3 ? rk4a[stage] : 0.0;
+  io.rk_b = adv_type == 3 ? rk4b[stage] : 0.0;
+  hifr_status s = timed_launch(c, hifr::MODE_ADVANCE, 2, io);
   if (s != HIFR_OK) return s;
   c->cur = 1 - c->cur;
   return HIFR_OK;
@@ -563,13 +663,15 @@ hifr_status hifr_run_timed(hifr_ctx* c, int n_steps, int adv_type, double dt, do
 namespace {
 
 template <int NF>
+// detj holds one value per point (pts_per_det = 1) or per element (pts_per_det = Nu)
 __global__ void __launch_bounds__(256) norm_partial_kernel(const double* __restrict__ div, const double* __restrict__ detj,
-                                                           long npts, int norm_type, double const_src, double* out) {
+                                                           long npts, int pts_per_det, int norm_type, double const_src,
+                                                           double* out) {
   __shared__ double sm[NF][256];
   double acc[NF];
   for (int k = 0; k < NF; k++) acc[k] = 0.0;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < npts; i += (long)gridDim.x * 256L) {
-    const double inv = 1.0 / detj[i];
+    const double inv = 1.0 / detj[i / pts_per_det];
     for (int k = 0; k < NF; k++) {
       const double r = div[i + npts * k] * inv - const_src;
       if (norm_type == 0) acc[k] = fmax(acc[k], fabs(r));
@@ -598,13 +700,14 @@ extern "C" hifr_status hifr_norm_residual(hifr_ctx* c, int norm_type, double* ou
   if (s != HIFR_OK) return s;
   if (!out || norm_type < 0 || norm_type > 2) return HIFR_ERR_ARG;
   const long npts = (long)c->nu * c->N;
+  const int ppd = c->affine ? c->nu : 1;  // detj entries: per element or per point
   const int nb = 512;
   if (c->nf == 5)
-    hipLaunchKernelGGL(norm_partial_kernel<5>, dim3(nb), dim3(256), 0, c->stream, c->div, c->detj, npts, norm_type,
-                       c->cfg.const_src, c->red);
+    hipLaunchKernelGGL(norm_partial_kernel<5>, dim3(nb), dim3(256), 0, c->stream, c->div, c->detj, npts, ppd,
+                       norm_type, c->cfg.const_src, c->red);
   else
-    hipLaunchKernelGGL(norm_partial_kernel<4>, dim3(nb), dim3(256), 0, c->stream, c->div, c->detj, npts, norm_type,
-                       c->cfg.const_src, c->red);
+    hipLaunchKernelGGL(norm_partial_kernel<4>, dim3(nb), dim3(256), 0, c->stream, c->div, c->detj, npts, ppd,
+                       norm_type, c->cfg.const_src, c->red);
   HIP_TRY(hipGetLastError());
   std::vector<double> part((size_t)nb * c->nf);
   HIP_TRY(hipMemcpyAsync(part.data(), c->red, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));

@@ -187,6 +187,14 @@ class Solver:
     def ctx(self):
         return lib().hifr_solver_ctx(self._h)
 
+    def metrics_per_element(self):
+        """True when the device keeps one metric set per element (all elements affine)."""
+        flag = C.c_int(-1)
+        rc = lib().hifr_metrics_per_element(C.c_void_p(self.ctx()), C.byref(flag))
+        if rc != 0:
+            raise RuntimeError("hifr_metrics_per_element failed (%d)" % rc)
+        return bool(flag.value)
+
     def dt(self):
         return lib().hifr_solver_dt(self._h)
 

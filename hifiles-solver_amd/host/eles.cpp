@@ -240,7 +240,7 @@ std::string eles::mv_all_cpu_gpu(const input& in) {
   c.viscous = in.viscous;
   c.riemann_solve_type = in.riemann_solve_type;
   c.vis_riemann_solve_type = in.vis_riemann_solve_type;
-  c.precision = HIFR_FP64;
+  c.precision = in.precision == 1 ? HIFR_MIXED : HIFR_FP64;
   c.gamma = in.gamma;
   c.prandtl = in.prandtl;
   c.mu_inf = in.mu_inf;
@@ -250,6 +250,7 @@ std::string eles::mv_all_cpu_gpu(const input& in) {
   c.tau = in.tau;
   c.pen_fact = in.pen_fact;
   c.const_src = in.const_src;
+  c.metric_storage = in.metric_storage;
   if (ctx) { hifr_destroy(ctx); ctx = nullptr; }
   hifr_status s = hifr_create(&c, &ctx);
   if (s != HIFR_OK) return std::string("hifr_create: ") + hifr_status_string(s);

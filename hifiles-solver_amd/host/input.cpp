@@ -132,6 +132,8 @@ std::string input::read_input_file(const std::string& path) {
   d.get("eta_hexa", eta_hexa);
   d.get("const_src", const_src);
   d.get("hifr_host_only", host_only);
+  d.get("hifr_metrics", metric_storage);
+  d.get("hifr_precision", precision);
   return "";
 }
 

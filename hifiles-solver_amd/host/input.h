@@ -36,6 +36,12 @@ struct input {
   // Not a reference key: hifr_host_only 1 builds the host-side element data
   // (operators, metrics, interfaces) without creating the device context.
   int host_only = 0;
+  // Not a reference key: hifr_metrics 1 keeps per-point metrics on the device even
+  // when every element is affine (HIFR_METRICS_PER_POINT).
+  int metric_storage = 0;
+  // Not a reference key: hifr_precision 1 selects the mixed-precision kernels
+  // (HIFR_MIXED); 0 is FP64.
+  int precision = 0;
   // derived (setup_params)
   double mu_inf = 0., rt_inf = 0., c_sth = 0., R_ref = 0., prandtl_t = 0., T_ref = 0., L_ref = 0.;
   double uvw_ref = 0., rho_ref = 0., p_ref = 0., mu_ref = 0., time_ref = 0.;

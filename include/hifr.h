@@ -35,8 +35,12 @@ typedef enum {
 /* Element types use the reference's numbering (src/eles_*.cpp setup_ele_type_specific). */
 enum { HIFR_QUAD = 1, HIFR_HEX = 4 };
 
-/* Arithmetic of the residual kernels. */
-enum { HIFR_FP64 = 0 };
+/* Arithmetic of the residual kernels.  HIFR_MIXED (no counterpart in the
+ * reference): state, RK register, detJ, face normals / tdA and the RK update in
+ * FP64; arithmetic, LDS, JGinv and the face traces in FP32, on the deviation of
+ * the state from a constant reference at rest (the mean density and energy of the
+ * uploaded state), so that FP32 never holds the O(p_ref) part of the fluxes. */
+enum { HIFR_FP64 = 0, HIFR_MIXED = 1 };
 
 /* What hifr_download can copy out (reference layouts). */
 enum {
@@ -57,11 +61,20 @@ typedef struct {
   int viscous;               /* 0 Euler, 1 Navier-Stokes (LDG) */
   int riemann_solve_type;    /* 0 Rusanov (only inviscid flux built) */
   int vis_riemann_solve_type;/* 0 LDG */
-  int precision;             /* HIFR_FP64 */
+  int precision;             /* HIFR_FP64 or HIFR_MIXED */
   double gamma, prandtl, mu_inf, rt_inf, c_sth, fix_vis;
   double tau, pen_fact;      /* LDG penalty and switch */
   double const_src;
+  int metric_storage;        /* HIFR_METRICS_AUTO or HIFR_METRICS_PER_POINT */
 } hifr_config;
+
+/* How the context stores the metrics it is given (no counterpart in the reference,
+ * which always keeps them per point).  AUTO: when every element is affine (its
+ * JGinv and detJ are the same at all solution points, and each face's normal and
+ * tdA the same at all its flux points, to 1e-13 relative), one value per element
+ * and per face is kept and read by the kernels; otherwise, and with PER_POINT,
+ * the per-point arrays are used as given. */
+enum { HIFR_METRICS_AUTO = 0, HIFR_METRICS_PER_POINT = 1 };
 
 typedef struct hifr_ctx hifr_ctx;
 
@@ -89,6 +102,9 @@ hifr_status hifr_set_operators(hifr_ctx* ctx, const double* opp_0, const double*
  * detjac_upts (Nu,N), JGinv_upts (d,d,Nu,N), tdA_fpts (Nfp,N), norm_fpts (Nfp,N,d). */
 hifr_status hifr_set_metrics(hifr_ctx* ctx, const double* detjac_upts, const double* JGinv_upts,
                              const double* tdA_fpts, const double* norm_fpts);
+
+/* 1 if the context keeps one metric set per element (see HIFR_METRICS_AUTO), else 0. */
+hifr_status hifr_metrics_per_element(const hifr_ctx* ctx, int* per_element);
 
 /* Interior and periodic interfaces: for each of n_pts interface points, the
  * element flux-point index (fpt + Nfp*ele) on the left and on the right.  This is

@@ -28,11 +28,18 @@ def _pair(g, **over):
     return s, o
 
 
+@pytest.mark.parametrize("metrics", [0, 1], ids=["auto", "per_point"])
 @pytest.mark.parametrize("name", cases.golden_names("residual"))
-def test_residual_matches_reference_and_oracle(require_gpu, name):
+def test_residual_matches_reference_and_oracle(require_gpu, name, metrics):
     g = cases.load_golden(name)
-    s, o = _pair(g)
-    print("case", name, flush=True)
+    s, o = _pair(g, hifr_metrics=metrics)
+    print("case", name, "metrics", metrics, flush=True)
+    # box meshes are affine: auto keeps one metric set per element; meshes with
+    # perturbed interior vertices have non-affine elements and keep per-point metrics
+    J = s.get("JGinv_upts")
+    affine = np.abs(J - J[:, :, :1, :]).max() < 1e-13 * np.abs(J).max()
+    assert affine == (name in ("hex_p3_ns_tgv_2x2x2", "hex_p4_ns_2x2x2", "quad_p3_euler_vortex_4x4"))
+    assert s.metrics_per_element() == (metrics == 0 and affine)
     s.set_state(g["u0"])
     div = s.residual()
     J = s.get("detjac_upts")[:, :, None]

@@ -52,11 +52,12 @@ def test_isentropic_vortex_200_steps(require_gpu):
     assert err.max() < 1.5e-2, err
 
 
-def _tgv_solver(n=32, order=3, host_only=False):
+def _tgv_solver(n=32, order=3, host_only=False, metrics=0):
     deck = hifr.deck_text(viscous=1, order=order, vcjh_scheme_hexa=1, dt=1e-3, adv_type=3, tau=1.0, pen_fact=0.5,
                           gamma=1.4, prandtl=0.71, Mach_free_stream=0.1, nx_free_stream=1.0,
                           Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, fix_vis=1,
-                          dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI, hifr_host_only=int(host_only))
+                          dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI, hifr_host_only=int(host_only),
+                          hifr_metrics=metrics)
     return hifr.Solver.box(deck, hifr.HEX, (n, n, n), (TWO_PI,) * 3, (0.0,) * 3), deck
 
 
@@ -86,8 +87,20 @@ def test_tgv_32cubed_free_stream_and_conservation(require_gpu):
     assert np.all(np.abs(total) < 1e-12 * scale + 1e-12), (total, scale)
 
 
-def test_tgv_32cubed_against_oracle(require_gpu):
-    s, _ = _tgv_solver()
+# Residual tolerance per conserved variable, relative to max|R| of that variable.
+# Per-point metrics are the reference's own arrays: 1e-12.  Per-element metrics
+# (the box is affine) replace the reference's per-point JGinv, whose values differ
+# between the points of one element by rounding (~1e-16 relative); at Mach 0.1 the
+# momentum flux carries the mean pressure p0 = 71.4, so that rounding enters the
+# reference's momentum divergence as ~p0 * |D| * 1e-16 ~ 5e-14 absolute, i.e.
+# ~1e-11 of the momentum residual (max ~5e-3).  Density and energy stay at 1e-12.
+RESID_TOL = {1: [1e-12] * 5, 0: [1e-12, 3e-11, 3e-11, 3e-11, 1e-12]}
+
+
+@pytest.mark.parametrize("metrics", [1, 0], ids=["per_point", "per_element"])
+def test_tgv_32cubed_against_oracle(require_gpu, metrics):
+    s, _ = _tgv_solver(metrics=metrics)
+    assert s.metrics_per_element() == (metrics == 0)
     xv = s.get("xv").reshape(-1, 3)
     c2v = s.get("c2v").reshape(-1, 8).astype(np.int32)
     od = po.default_deck(equation=0, viscous=1, order=3, adv_type=3, dt=1e-3, tau=1.0, pen_fact=0.5,
@@ -99,10 +112,16 @@ def test_tgv_32cubed_against_oracle(require_gpu):
     s.set_state(u)
     o.set_state(u)
     o.residual()
-    assert cases.field_rel_err(s.residual(), o.get("div_tconf_upts")).max() < 1e-12
+    err = cases.field_rel_err(s.residual(), o.get("div_tconf_upts"))
+    print("metrics", metrics, "residual field errors", err)
+    assert np.all(err < RESID_TOL[metrics]), err
     s.set_state(u)
     o.set_state(u)
     o.zero_rk_register()
     s.run(1)
     o.run_steps(1)
-    assert cases.field_rel_err(s.get("disu_upts"), o.get("disu_upts")).max() < 1e-12
+    # the z-momentum starts at zero (w = 0), so after one step its scale is
+    # dt * |R|: the state carries the residual's tolerance
+    err = cases.field_rel_err(s.get("disu_upts"), o.get("disu_upts"))
+    print("metrics", metrics, "state field errors", err)
+    assert np.all(err < RESID_TOL[metrics]), err
