@@ -14,59 +14,71 @@ struct Cfg {
 template <bool MIX>
 using Args = StageArgsT<std::conditional_t<MIX, float, double>>;
 
-template <int P, int ND, int VIS, int MODE, bool AFF, bool MIX>
-hipError_t launch_aff(const Args<MIX>& a, hipStream_t s) {
+template <int P, int ND, int VIS, int MODE, bool AFF, bool MIX, int LES>
+hipError_t launch_les(const Args<MIX>& a, hipStream_t s) {
   constexpr int EPB = Cfg<P, ND>::EPB;
   using T = TP<P, ND, EPB>;
   const int grid = (a.N + EPB - 1) / EPB;
-  hipLaunchKernelGGL((fr_tensor_kernel<P, ND, EPB, VIS, MODE, AFF, MIX>), dim3(grid), dim3(T::BLOCK), 0, s, a);
+  hipLaunchKernelGGL((fr_tensor_kernel<P, ND, EPB, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
   return hipGetLastError();
+}
+
+// LES variants exist for the constant-viscosity Navier-Stokes kernels only; the
+// context rejects other combinations before launching
+template <int P, int ND, int VIS, int MODE, bool AFF, bool MIX>
+hipError_t launch_aff(const Args<MIX>& a, hipStream_t s, int les) {
+  if constexpr (VIS == 1 && MODE != MODE_TRACE_U) {
+    if (les == 1) return launch_les<P, ND, VIS, MODE, AFF, MIX, 1>(a, s);
+    if (les == 2) return launch_les<P, ND, VIS, MODE, AFF, MIX, 2>(a, s);
+  }
+  if (les != 0) return hipErrorInvalidValue;
+  return launch_les<P, ND, VIS, MODE, AFF, MIX, 0>(a, s);
 }
 
 // the trace kernel reads no metrics: one instantiation
 template <int P, int ND, int VIS, int MODE, bool MIX>
-hipError_t launch_one(const Args<MIX>& a, hipStream_t s, bool aff) {
-  if constexpr (MODE == MODE_TRACE_U) return launch_aff<P, ND, VIS, MODE, false, MIX>(a, s);
-  return aff ? launch_aff<P, ND, VIS, MODE, true, MIX>(a, s) : launch_aff<P, ND, VIS, MODE, false, MIX>(a, s);
+hipError_t launch_one(const Args<MIX>& a, hipStream_t s, bool aff, int les) {
+  if constexpr (MODE == MODE_TRACE_U) return launch_aff<P, ND, VIS, MODE, false, MIX>(a, s, 0);
+  return aff ? launch_aff<P, ND, VIS, MODE, true, MIX>(a, s, les) : launch_aff<P, ND, VIS, MODE, false, MIX>(a, s, les);
 }
 
 template <int P, int ND, bool MIX>
-hipError_t launch_mode(int mode, int vis, const Args<MIX>& a, hipStream_t s, bool f) {
+hipError_t launch_mode(int mode, int vis, const Args<MIX>& a, hipStream_t s, bool f, int les) {
   switch (mode) {
-    case MODE_TRACE_U: return launch_one<P, ND, 0, MODE_TRACE_U, MIX>(a, s, f);
+    case MODE_TRACE_U: return launch_one<P, ND, 0, MODE_TRACE_U, MIX>(a, s, f, les);
     case MODE_TRACE_GRAD:
-      return vis == 2 ? launch_one<P, ND, 2, MODE_TRACE_GRAD, MIX>(a, s, f)
-                      : launch_one<P, ND, 1, MODE_TRACE_GRAD, MIX>(a, s, f);
+      return vis == 2 ? launch_one<P, ND, 2, MODE_TRACE_GRAD, MIX>(a, s, f, les)
+                      : launch_one<P, ND, 1, MODE_TRACE_GRAD, MIX>(a, s, f, les);
     case MODE_RESID:
-      return vis == 2 ? launch_one<P, ND, 2, MODE_RESID, MIX>(a, s, f)
-           : vis == 1 ? launch_one<P, ND, 1, MODE_RESID, MIX>(a, s, f)
-                      : launch_one<P, ND, 0, MODE_RESID, MIX>(a, s, f);
+      return vis == 2 ? launch_one<P, ND, 2, MODE_RESID, MIX>(a, s, f, les)
+           : vis == 1 ? launch_one<P, ND, 1, MODE_RESID, MIX>(a, s, f, les)
+                      : launch_one<P, ND, 0, MODE_RESID, MIX>(a, s, f, les);
     default:
-      return vis == 2 ? launch_one<P, ND, 2, MODE_ADVANCE, MIX>(a, s, f)
-           : vis == 1 ? launch_one<P, ND, 1, MODE_ADVANCE, MIX>(a, s, f)
-                      : launch_one<P, ND, 0, MODE_ADVANCE, MIX>(a, s, f);
+      return vis == 2 ? launch_one<P, ND, 2, MODE_ADVANCE, MIX>(a, s, f, les)
+           : vis == 1 ? launch_one<P, ND, 1, MODE_ADVANCE, MIX>(a, s, f, les)
+                      : launch_one<P, ND, 0, MODE_ADVANCE, MIX>(a, s, f, les);
   }
 }
 
 // hexes P = 1..5, quads P = 1..6 (callers check the order first)
 template <bool MIX>
-hipError_t launch(int nd, int P, int mode, int vis, bool aff, const Args<MIX>& a, hipStream_t s) {
+hipError_t launch(int nd, int P, int mode, int vis, bool aff, int les, const Args<MIX>& a, hipStream_t s) {
   if (nd == 3) {
     switch (P) {
-      case 1: return launch_mode<1, 3, MIX>(mode, vis, a, s, aff);
-      case 2: return launch_mode<2, 3, MIX>(mode, vis, a, s, aff);
-      case 3: return launch_mode<3, 3, MIX>(mode, vis, a, s, aff);
-      case 4: return launch_mode<4, 3, MIX>(mode, vis, a, s, aff);
-      default: return launch_mode<5, 3, MIX>(mode, vis, a, s, aff);
+      case 1: return launch_mode<1, 3, MIX>(mode, vis, a, s, aff, les);
+      case 2: return launch_mode<2, 3, MIX>(mode, vis, a, s, aff, les);
+      case 3: return launch_mode<3, 3, MIX>(mode, vis, a, s, aff, les);
+      case 4: return launch_mode<4, 3, MIX>(mode, vis, a, s, aff, les);
+      default: return launch_mode<5, 3, MIX>(mode, vis, a, s, aff, les);
     }
   }
   switch (P) {
-    case 1: return launch_mode<1, 2, MIX>(mode, vis, a, s, aff);
-    case 2: return launch_mode<2, 2, MIX>(mode, vis, a, s, aff);
-    case 3: return launch_mode<3, 2, MIX>(mode, vis, a, s, aff);
-    case 4: return launch_mode<4, 2, MIX>(mode, vis, a, s, aff);
-    case 5: return launch_mode<5, 2, MIX>(mode, vis, a, s, aff);
-    default: return launch_mode<6, 2, MIX>(mode, vis, a, s, aff);
+    case 1: return launch_mode<1, 2, MIX>(mode, vis, a, s, aff, les);
+    case 2: return launch_mode<2, 2, MIX>(mode, vis, a, s, aff, les);
+    case 3: return launch_mode<3, 2, MIX>(mode, vis, a, s, aff, les);
+    case 4: return launch_mode<4, 2, MIX>(mode, vis, a, s, aff, les);
+    case 5: return launch_mode<5, 2, MIX>(mode, vis, a, s, aff, les);
+    default: return launch_mode<6, 2, MIX>(mode, vis, a, s, aff, les);
   }
 }
 

@@ -154,6 +154,96 @@ __device__ __forceinline__ void visc_tensor_cols(const T* u, GCol&& gcol, const 
   }
 }
 
+// Eddy-viscosity SGS flux at a solution point, as the reference CPU build
+// computes it (src/eles.cpp:2456-2778, no wall model): MODEL 1 Smagorinsky
+// (C_s = 0.1), 2 WALE (C_w = 0.5, eps = 1e-12), turbulent Prandtl number 0.5,
+// filter width delta = filter_ratio * (detJ * vref)^(1/d) / (P+1) with the
+// reference-element volume vref (4 quads, 8 hexes).  Two index slips of that code
+// are kept: its "velocity" is u_i = U_i / rho for i < d (density / rho in slot 0,
+// src/eles.cpp:2483) and du(i,j) takes the density derivative along x_j
+// (src/eles.cpp:2656).  u: conservative state; g[k][m] = d U_k / d x_m.
+// Output: momentum flux M[i][j] = F_sgs(1+i, j), energy flux E[j] = F_sgs(d+1, j);
+// the density flux is zero.
+template <int ND, int MODEL, class T>
+__device__ __forceinline__ void sgs_flux(const T* u, const T (*g)[ND], T detj_vref, T filter_ratio, int order,
+                                         T gamma, T (*M)[ND], T* E) {
+  constexpr int NF = ND + 2;
+  const T rho = u[0];
+  T uu[ND];
+  T ke = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    uu[i] = u[i] / rho;
+    ke += T(0.5) * (uu[i] * uu[i]);
+  }
+  const T inte = u[NF - 1] / rho - ke;
+  const T delta = filter_ratio * pow(detj_vref, T(1) / T(ND)) / (T(order) + T(1));
+  T du[ND][ND], de[ND], S[ND][ND];
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    T dke = ke * g[0][i];
+#pragma unroll
+    for (int j = 0; j < ND; j++) {
+      du[i][j] = (g[j + 1][i] - uu[j] * g[0][j]) / rho;
+      dke += rho * uu[j] * du[i][j];
+    }
+    de[i] = (g[NF - 1][i] - dke - g[0][i] * inte) / rho;
+  }
+  T diag = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+#pragma unroll
+    for (int j = 0; j < ND; j++) S[i][j] = (du[i][j] + du[j][i]) / T(2);
+    diag += S[i][i] / T(3);
+  }
+#pragma unroll
+  for (int i = 0; i < ND; i++) S[i][i] -= diag;
+  T mu_t;
+  if constexpr (MODEL == 1) {
+    T smod = 0;
+#pragma unroll
+    for (int i = 0; i < ND; i++)
+#pragma unroll
+      for (int j = 0; j < ND; j++) smod += T(2) * S[i][j] * S[i][j];
+    smod = sqrt(smod);
+    const T cs = T(0.1);
+    mu_t = rho * cs * cs * delta * delta * smod;
+  } else {
+    T num = 0, den = 0, d2 = 0;
+    T Sq[ND][ND];
+#pragma unroll
+    for (int i = 0; i < ND; i++)
+#pragma unroll
+      for (int j = 0; j < ND; j++) {
+        T q = 0;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) q += (du[i][k] * du[k][j] + du[j][k] * du[k][i]) / T(2);
+        Sq[i][j] = q;
+        d2 += du[i][j] * du[j][i] / T(3);
+      }
+#pragma unroll
+    for (int i = 0; i < ND; i++) Sq[i][i] -= d2;
+#pragma unroll
+    for (int i = 0; i < ND; i++)
+#pragma unroll
+      for (int j = 0; j < ND; j++) {
+        num += Sq[i][j] * Sq[i][j];
+        den += S[i][j] * S[i][j];
+      }
+    den = pow(den, T(2.5)) + pow(num, T(1.25));
+    num = pow(num, T(1.5));
+    const T cw = T(0.5);
+    mu_t = rho * cw * cw * delta * delta * num / (den + T(1.e-12));
+  }
+  const T ek = -gamma * mu_t / T(0.5);
+#pragma unroll
+  for (int j = 0; j < ND; j++) {
+    E[j] = ek * de[j];
+#pragma unroll
+    for (int i = 0; i < ND; i++) M[i][j] = T(-2) * mu_t * S[i][j];
+  }
+}
+
 // Viscous flux projected on w: out[k] = sum_m F_v(k,m) w_m
 template <int ND, class T>
 __device__ __forceinline__ void visc_flux_dot(const ViscT<ND, T>& t, const T* w, T* out) {

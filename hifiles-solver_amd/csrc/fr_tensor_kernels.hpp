@@ -74,6 +74,7 @@ struct StageArgsT {
   Ops1DT<R> ops;
   Phys ph;
   RefState<double> ref;  // mixed path: reference state of the deviations
+  double filter_ratio;   // LES filter width factor
 };
 using StageArgs = StageArgsT<double>;
 
@@ -132,14 +133,16 @@ __device__ __forceinline__ void line_of(int p, int dir, int& pos, int& base, int
   }
 }
 
-template <int P, int ND, int EPB, int MODE>
+template <int P, int ND, int EPB, int MODE, int LES>
 struct SmemLayout {
   using T = TP<P, ND, EPB>;
   static constexpr int U = EPB * T::NF * T::NU;
   static constexpr int Q = (MODE == MODE_TRACE_U) ? 0 : EPB * T::NFP * T::NF;
   static constexpr int G = (MODE == MODE_TRACE_GRAD) ? EPB * T::NF * ND * T::NU : 0;
   static constexpr int F = (MODE == MODE_RESID || MODE == MODE_ADVANCE) ? EPB * T::NF * ND * T::NU : 0;
-  static constexpr int TOTAL = U + Q + G + F;
+  // LES: transformed SGS flux at the solution points (extrapolated to the faces)
+  static constexpr int SG = (MODE == MODE_TRACE_GRAD && LES) ? EPB * T::NF * ND * T::NU : 0;
+  static constexpr int TOTAL = U + Q + G + F + SG;
 };
 
 // Interpolate NF fields from LDS rows (stride NU apart) to the end of a line.
@@ -157,14 +160,16 @@ __device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base
 // VIS: 0 Euler, 1 Navier-Stokes with constant viscosity, 2 with Sutherland's law.
 // AFF: metrics stored per element / per face (every element affine).
 // MIX: mixed precision (FP32 arithmetic on deviations, FP64 state and update).
-template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX>
+// LES: 0 none, 1 Smagorinsky, 2 WALE eddy viscosity (viscous kernels only).
+template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES>
 __global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK))
 fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
   constexpr bool VISC = VIS != 0;
   constexpr bool SUTH = VIS == 2;
+  static_assert(LES == 0 || VISC, "LES needs the viscous kernels");
   using T = TP<P, ND, EPB>;
-  using S = SmemLayout<P, ND, EPB, MODE>;
+  using S = SmemLayout<P, ND, EPB, MODE, LES>;
   constexpr int n = T::n, NU = T::NU, NFP = T::NFP, NF = T::NF, BLOCK = T::BLOCK;
   constexpr int NFPF = T::NFPF, NFACE = 2 * ND;
   __shared__ R smem[S::TOTAL];
@@ -172,6 +177,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   R* s_q = s_u + S::U;  // [EPB][NFP][NF]   LDG delta, then interface jumps
   R* s_g = s_q + S::Q;  // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD)
   R* s_f = s_g + S::G;  // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE)
+  R* s_s = s_f + S::F;  // [EPB][NF*ND][NU] transformed SGS flux (TRACE_GRAD, LES)
   const R gam = R(a.ph.gamma);
   RefState<R> ref;
   ref.rho = R(a.ref.rho);
@@ -300,6 +306,23 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
             gc[k] = v;
           }
         };
+        // SGS eddy-viscosity flux (src/eles.cpp:2369-2417): physical momentum flux
+        // M and energy flux E from the full corrected gradient
+        R sgM[ND][ND], sgE[ND];
+        if constexpr (LES != 0) {
+          R gf[NF][ND];
+#pragma unroll
+          for (int m = 0; m < ND; m++) {
+            R gc[NF];
+            gcol(m, gc);
+#pragma unroll
+            for (int k = 0; k < NF; k++) gf[k][m] = gc[k];
+          }
+          R uu[NF];
+          full(u_own, uu);
+          const R vref = ND == 3 ? R(8) : R(4);
+          sgs_flux<ND, LES, R>(uu, gf, R(a.detj[vi]) * vref, R(a.filter_ratio), P, gam, sgM, sgE);
+        }
         if constexpr (MODE == MODE_TRACE_GRAD) {
 #pragma unroll
           for (int m = 0; m < ND; m++) {
@@ -308,10 +331,37 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
             for (int k = 0; k < NF; k++) s_g[(el * NF * ND + k * ND + m) * NU + p] = gc[k];
           }
+          if constexpr (LES != 0) {
+            // transformed SGS flux sgsf(k,l) = sum_m JGinv(l,m) F_sgs(k,m)
+#pragma unroll
+            for (int l = 0; l < ND; l++) {
+              s_s[(el * NF * ND + 0 * ND + l) * NU + p] = R(0);
+#pragma unroll
+              for (int i = 0; i < ND; i++) {
+                R v = 0;
+#pragma unroll
+                for (int m = 0; m < ND; m++) v += Jg[l][m] * sgM[i][m];
+                s_s[(el * NF * ND + (1 + i) * ND + l) * NU + p] = v;
+              }
+              R v = 0;
+#pragma unroll
+              for (int m = 0; m < ND; m++) v += Jg[l][m] * sgE[m];
+              s_s[(el * NF * ND + (NF - 1) * ND + l) * NU + p] = v;
+            }
+          }
         } else {
           R uu[NF];
           full(u_own, uu);
           visc_tensor_cols<ND, SUTH, R>(uu, gcol, a.ph, vt);
+          if constexpr (LES != 0) {
+            // F_v(1+i, j) = -tau(i, j), F_v(d+1, j) = qe(j): add the SGS flux
+#pragma unroll
+            for (int i = 0; i < ND; i++)
+#pragma unroll
+              for (int j = 0; j < ND; j++) vt.tau[i][j] -= sgM[i][j];
+#pragma unroll
+            for (int j = 0; j < ND; j++) vt.qe[j] += sgE[j];
+          }
         }
       }
       if constexpr (MODE != MODE_TRACE_GRAD) {
@@ -368,6 +418,22 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         }, a.ph, t);
         R fa[NF];
         visc_flux_dot<ND, R>(t, nrm, fa);
+        if constexpr (LES != 0) {
+          // plus the extrapolated transformed SGS flux dotted with the physical
+          // normal, as the reference adds it (src/int_inters.cpp:415-427)
+#pragma unroll
+          for (int k = 1; k < NF; k++) {
+            R acc = 0;
+#pragma unroll
+            for (int l = 0; l < ND; l++) {
+              R w = 0;
+#pragma unroll
+              for (int q = 0; q < n; q++) w += lw[q] * s_s[(tl * NF * ND + k * ND + l) * NU + base + q * st];
+              acc += nrm[l] * w;
+            }
+            fa[k] += acc;
+          }
+        }
 #pragma unroll
         for (int k = 0; k < NF; k++) a.fv_out[fe + NFP * NN * k] = fa[k];
       }

@@ -64,6 +64,9 @@ bool order_supported(int ele_type, int P) {
 
 int vis_kind(const hifr_ctx* c) { return c->cfg.viscous == 0 ? 0 : (c->cfg.fix_vis == 1.0 ? 1 : 2); }
 
+// 0 no LES, 1 Smagorinsky, 2 WALE
+int les_kind(const hifr_ctx* c) { return c->cfg.les ? c->cfg.sgs_model + 1 : 0; }
+
 template <class R>
 StageArgsT<R> base_args(hifr_ctx* c) {
   StageArgsT<R> a;
@@ -87,6 +90,7 @@ StageArgsT<R> base_args(hifr_ctx* c) {
   a.ph.pen_fact = c->cfg.pen_fact;
   a.ph.const_src = c->cfg.const_src;
   a.ref = c->ref;
+  a.filter_ratio = c->cfg.filter_ratio;
   return a;
 }
 
@@ -112,9 +116,9 @@ hipError_t launch_io(hifr_ctx* c, int mode, const StageIO& io) {
   a.rk_b = io.rk_b;
   a.dt = io.dt;
   if constexpr (std::is_same_v<R, float>)
-    return hifr::launch_mixed(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, a, c->stream);
+    return hifr::launch_mixed(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, les_kind(c), a, c->stream);
   else
-    return hifr::launch_fp64(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, a, c->stream);
+    return hifr::launch_fp64(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, les_kind(c), a, c->stream);
 }
 
 hipError_t launch(hifr_ctx* c, int mode, const StageIO& io) {
@@ -210,6 +214,8 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   if (cfg->n_eles <= 0) return HIFR_ERR_ARG;
   if (cfg->riemann_solve_type != 0 || cfg->vis_riemann_solve_type != 0) return HIFR_ERR_UNSUPPORTED;
   if (cfg->precision != HIFR_FP64 && cfg->precision != HIFR_MIXED) return HIFR_ERR_UNSUPPORTED;
+  if (cfg->les && (!cfg->viscous || cfg->fix_vis != 1.0 || cfg->sgs_model < 0 || cfg->sgs_model > 1))
+    return HIFR_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HIFR_ERR_DEVICE;
   hifr_ctx* c = new hifr_ctx();

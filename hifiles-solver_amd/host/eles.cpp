@@ -251,6 +251,9 @@ std::string eles::mv_all_cpu_gpu(const input& in) {
   c.pen_fact = in.pen_fact;
   c.const_src = in.const_src;
   c.metric_storage = in.metric_storage;
+  c.les = in.LES;
+  c.sgs_model = in.SGS_model;
+  c.filter_ratio = in.filter_ratio;
   if (ctx) { hifr_destroy(ctx); ctx = nullptr; }
   hifr_status s = hifr_create(&c, &ctx);
   if (s != HIFR_OK) return std::string("hifr_create: ") + hifr_status_string(s);

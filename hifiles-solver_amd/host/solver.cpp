@@ -147,7 +147,13 @@ std::string setup_solver(hifr_solver* s) {
   input& in = s->in;
   if (in.equation != 0) return "only the Euler / Navier-Stokes equations are built";
   if (in.motion != 0) return "moving meshes are not built";
-  if (in.LES != 0 || in.turb_model != 0) return "LES / turbulence models are not built";
+  if (in.turb_model != 0) return "turbulence models are not built";
+  if (in.LES != 0) {
+    if (!in.viscous) return "LES needs viscous 1";
+    if (in.SGS_model != 0 && in.SGS_model != 1) return "only the Smagorinsky (0) and WALE (1) SGS models are built";
+    if (in.wall_model != 0) return "LES wall models are not built";
+    if (in.fix_vis != 1.0) return "LES is built for constant viscosity (fix_vis 1) only";
+  }
   if (in.dt_type != 0) return "only user-supplied time steps (dt_type 0) are built";
   in.setup_params();
   const mesh& m = s->msh;

@@ -66,6 +66,12 @@ typedef struct {
   double tau, pen_fact;      /* LDG penalty and switch */
   double const_src;
   int metric_storage;        /* HIFR_METRICS_AUTO or HIFR_METRICS_PER_POINT */
+  /* LES (src/eles.cpp:2321-2778, 3294-3368; src/int_inters.cpp:415-427): les 1 with
+   * sgs_model 0 Smagorinsky or 1 WALE, eddy viscosity at the solution points with
+   * filter width filter_ratio * vol^(1/d) / (P+1); requires viscous and constant
+   * viscosity (fix_vis 1).  No wall model. */
+  int les, sgs_model;
+  double filter_ratio;
 } hifr_config;
 
 /* How the context stores the metrics it is given (no counterpart in the reference,

@@ -167,6 +167,7 @@ struct or_solver {
   double *detjac_upts, *JGinv_upts, *tdA_fpts, *norm_fpts, *pos_upts, *pos_fpts;
   double *disu_upts[2], *div_tconf_upts, *disu_fpts, *tdisf_upts, *norm_tdisf_fpts,
       *norm_tconf_fpts, *delta_disu_fpts, *grad_disu_upts, *grad_disu_fpts, *src_upts;
+  double *sgsf_upts, *sgsf_fpts; /* LES: transformed SGS flux (Nu,N,nf,d), (Nfp,N,nf,d) */
   int n_inters;
   int *face_l, *face_r; /* (n_inters*nfpf) flux-point indices fpt + Nfp*ele */
 };
@@ -631,7 +632,8 @@ static int set_connectivity(or_solver* s, const int* c2v, const double* xv) {
 or_solver* or_create(const or_deck* deck, int ele_type, int n_verts, const double* xv, int n_eles,
                      const int* c2v) {
   if (ele_type != 1 && ele_type != 4) { fprintf(stderr, "fr_oracle: ele_type %d\n", ele_type); return NULL; }
-  if (deck->order < 0 || deck->order > 6 || deck->upts_type != 0 || deck->equation != 0 || deck->LES) {
+  if (deck->order < 0 || deck->order > 6 || deck->upts_type != 0 || deck->equation != 0 ||
+      (deck->LES && (!deck->viscous || deck->SGS_model < 0 || deck->SGS_model > 1))) {
     fprintf(stderr, "fr_oracle: unsupported deck\n");
     return NULL;
   }
@@ -675,6 +677,10 @@ or_solver* or_create(const or_deck* deck, int ele_type, int n_verts, const doubl
     s->grad_disu_upts = xcalloc(nuN * nf * nd, sizeof(double));
     s->grad_disu_fpts = xcalloc(nfpN * nf * nd, sizeof(double));
   }
+  if (deck->LES) {
+    s->sgsf_upts = xcalloc(nuN * nf * nd, sizeof(double));
+    s->sgsf_fpts = xcalloc(nfpN * nf * nd, sizeof(double));
+  }
   return s;
 }
 
@@ -687,6 +693,7 @@ void or_destroy(or_solver* s) {
   free(s->disu_upts[0]); free(s->disu_upts[1]); free(s->div_tconf_upts); free(s->src_upts);
   free(s->disu_fpts); free(s->tdisf_upts); free(s->norm_tdisf_fpts); free(s->norm_tconf_fpts);
   free(s->delta_disu_fpts); free(s->grad_disu_upts); free(s->grad_disu_fpts);
+  free(s->sgsf_upts); free(s->sgsf_fpts);
   free(s->face_l); free(s->face_r);
   free(s);
 }
@@ -723,6 +730,8 @@ long or_get(const or_solver* s, int what, double* out) {
     case 15: src = s->loc_upts; n = (long)nd * s->nu; break;
     case 16: src = s->tloc_fpts; n = (long)nd * s->nfp; break;
     case 17: src = s->pos_fpts; n = nfpN * nd; break;
+    case 18: src = s->sgsf_upts; n = src ? nuN * nf * nd : 0; break;
+    case 19: src = s->sgsf_fpts; n = src ? nfpN * nf * nd : 0; break;
     default: return -1;
   }
   if (out && n > 0) {
@@ -1011,23 +1020,115 @@ static void extrapolate_corrected_gradient(or_solver* s) {
               s->grad_disu_fpts + nfpN * s->nf * d);
 }
 
-/* src/eles.cpp:2321-2453 (static mesh, no LES / wall model) */
+/* Eddy-viscosity SGS flux at a solution point, src/eles.cpp:2456-2778 (CPU path,
+ * no wall model): sgs_model 0 Smagorinsky (C_s = 0.1), 1 WALE (C_w = 0.5,
+ * eps = 1e-12); turbulent Prandtl number 0.5.  Kept as the reference CPU build
+ * computes it, with its two index slips: the "velocity" is u(i) = temp_u(i)/rho for
+ * i < d (src/eles.cpp:2483: density/rho in slot 0, the momenta shifted by
+ * one), and du(i,j) = (d(rho u_j)/dx_i - u(j) d(rho)/dx_j)/rho takes the density
+ * derivative along x_j (src/eles.cpp:2656).  u (nf), g (nf,nd) col-major, sgsf
+ * (nf,nd) col-major physical flux. */
+static void calc_sgsf(const or_solver* s, const double* u, const double* g, double detjac, double* sgsf) {
+  int nd = s->nd, nf = s->nf;
+  double diag = 0.0, Smod = 0.0, ke = 0.0, Pr = 0.5;
+  double uu[3], drho[3], dene[3], dke[3], de[3], dmom[3][3], du[3][3], S[3][3];
+  double rho = u[0];
+  for (int i = 0; i < nd; i++) {
+    uu[i] = u[i] / rho;
+    ke += 0.5 * pow(uu[i], 2);
+  }
+  double inte = u[nf - 1] / rho - ke;
+  for (int k = 0; k < nf * nd; k++) sgsf[k] = 0.0;
+  /* calc_ele_vol: detjac * reference volume (quads 4, hexes 8) */
+  double vol = detjac * (nd == 3 ? 8. : 4.);
+  double delta = s->deck.filter_ratio * pow(vol, 1. / nd) / (s->order + 1.);
+  for (int i = 0; i < nd; i++) {
+    drho[i] = g[0 + nf * i];
+    dene[i] = g[nf - 1 + nf * i];
+    for (int j = 1; j < nf - 1; j++) dmom[i][j - 1] = g[j + nf * i];
+  }
+  for (int i = 0; i < nd; i++) {
+    dke[i] = ke * drho[i];
+    for (int j = 0; j < nd; j++) {
+      du[i][j] = (dmom[i][j] - uu[j] * drho[j]) / rho;
+      dke[i] += rho * uu[j] * du[i][j];
+    }
+    de[i] = (dene[i] - dke[i] - drho[i] * inte) / rho;
+  }
+  for (int i = 0; i < nd; i++) {
+    for (int j = 0; j < nd; j++) S[i][j] = (du[i][j] + du[j][i]) / 2.0;
+    diag += S[i][i] / 3.0;
+  }
+  for (int i = 0; i < nd; i++) S[i][i] -= diag;
+  for (int i = 0; i < nd; i++)
+    for (int j = 0; j < nd; j++) Smod += 2.0 * S[i][j] * S[i][j];
+  Smod = sqrt(Smod);
+  double mu_t, Cs;
+  if (s->deck.SGS_model == 0) {
+    Cs = 0.1;
+    mu_t = rho * Cs * Cs * delta * delta * Smod;
+  } else {
+    Cs = 0.5;
+    double num = 0.0, denom = 0.0, eps = 1.e-12, Sq[3][3];
+    diag = 0.0;
+    for (int i = 0; i < nd; i++)
+      for (int j = 0; j < nd; j++) {
+        Sq[i][j] = 0.0;
+        for (int k = 0; k < nd; ++k) Sq[i][j] += (du[i][k] * du[k][j] + du[j][k] * du[k][i]) / 2.0;
+        diag += du[i][j] * du[j][i] / 3.0;
+      }
+    for (int i = 0; i < nd; i++) Sq[i][i] -= diag;
+    for (int i = 0; i < nd; i++)
+      for (int j = 0; j < nd; j++) {
+        num += Sq[i][j] * Sq[i][j];
+        denom += S[i][j] * S[i][j];
+      }
+    denom = pow(denom, 2.5) + pow(num, 1.25);
+    num = pow(num, 1.5);
+    mu_t = rho * Cs * Cs * delta * delta * num / (denom + eps);
+  }
+  for (int j = 0; j < nd; j++) {
+    sgsf[0 + nf * j] = 0.0;
+    sgsf[nf - 1 + nf * j] = -1.0 * s->ph.gamma * mu_t / Pr * de[j];
+    for (int i = 1; i < nf - 1; i++) sgsf[i + nf * j] = -2.0 * mu_t * S[i - 1][j];
+  }
+}
+
+/* src/eles.cpp:2321-2453 (static mesh, no wall model) */
 static void evaluate_viscFlux(or_solver* s) {
   int nu = s->nu, N = s->n_eles, nf = s->nf, nd = s->nd;
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < N; i++) {
-    double u[5], g[15], f[15];
+    double u[5], g[15], f[15], sg[15];
     for (int j = 0; j < nu; j++) {
       for (int k = 0; k < nf; k++) {
         u[k] = UPT(s->disu_upts[0], j, i, k);
         for (int m = 0; m < nd; m++) g[k + nf * m] = UPT4(s->grad_disu_upts, j, i, k, m);
       }
       calc_visf(nd, &s->ph, u, g, f);
+      if (s->deck.LES) {
+        calc_sgsf(s, u, g, s->detjac_upts[j + (long)nu * i], sg);
+        for (int k = 0; k < nf; k++)
+          for (int l = 0; l < nd; l++) f[k + nf * l] += sg[k + nf * l];
+        for (int k = 0; k < nf; k++)
+          for (int l = 0; l < nd; l++) {
+            UPT4(s->sgsf_upts, j, i, k, l) = 0.0;
+            for (int m = 0; m < nd; m++) UPT4(s->sgsf_upts, j, i, k, l) += JG(l, m, j, i) * sg[k + nf * m];
+          }
+      }
       for (int k = 0; k < nf; k++)
         for (int l = 0; l < nd; l++)
           for (int m = 0; m < nd; m++) UPT4(s->tdisf_upts, j, i, k, l) += JG(l, m, j, i) * f[k + nf * m];
     }
   }
+}
+
+/* src/eles.cpp:3294-3368: SGS flux at the flux points, extrapolated (opp_0) */
+static void evaluate_sgsFlux(or_solver* s) {
+  long nuN = (long)s->nu * s->n_eles, nfpN = (long)s->nfp * s->n_eles;
+  for (int d = 0; d < s->nd; d++)
+    dgemm_ref(s->nfp, s->nf * s->n_eles, s->nu, 1.0, 0.0, s->opp_0, s->sgsf_upts + nuN * s->nf * d,
+              s->sgsf_fpts + nfpN * s->nf * d);
 }
 
 /* src/eles.cpp:1423-1512 */
@@ -1048,7 +1149,7 @@ static void calculate_divergence(or_solver* s) {
     dgemm_ref(s->nu, cols, s->nu, 1.0, 1.0, s->opp_2[d], s->tdisf_upts + nuN * s->nf * d, s->div_tconf_upts);
 }
 
-/* src/int_inters.cpp:359-470 (no LES) */
+/* src/int_inters.cpp:359-470 */
 static void common_viscFlux(or_solver* s) {
   int nf = s->nf, nd = s->nd, nfp = s->nfp, N = s->n_eles;
   long nfpN = (long)nfp * N, npts = (long)s->n_inters * s->nfpf;
@@ -1064,6 +1165,15 @@ static void common_viscFlux(or_solver* s) {
       }
     calc_visf(nd, &s->ph, ul, gl, fl);
     calc_visf(nd, &s->ph, ur, gr, fr);
+    if (s->deck.LES) {
+      /* the extrapolated (transformed) SGS flux is added to the physical viscous
+         flux as the reference does (src/int_inters.cpp:415-427) */
+      for (int m = 0; m < nd; m++)
+        for (int l = 0; l < nf; l++) {
+          fl[l + nf * m] += s->sgsf_fpts[a + nfpN * (l + (long)nf * m)];
+          fr[l + nf * m] += s->sgsf_fpts[b + nfpN * (l + (long)nf * m)];
+        }
+    }
     for (int m = 0; m < nd; m++) n[m] = s->norm_fpts[a + nfpN * m];
     ldg_flux(nd, ul, ur, fl, fr, n, fn, s->ph.tau, s->ph.pen_fact);
     double tdl = s->tdA_fpts[a], tdr = s->tdA_fpts[b];
@@ -1095,6 +1205,7 @@ void or_residual(or_solver* s) {
     extrapolate_corrected_gradient(s);
     evaluate_viscFlux(s);
   }
+  if (s->deck.LES) evaluate_sgsFlux(s);
   extrapolate_totalFlux(s);
   calculate_divergence(s);
   if (s->deck.viscous) common_viscFlux(s);

@@ -44,7 +44,7 @@ typedef struct {
   int upts_type;              /* 0 Gauss (only supported point set) */
   int vcjh_scheme;            /* 0 user eta, 1 DG, 2 SD, 3 Hu, 4 c+ */
   double eta;                 /* used when vcjh_scheme == 0 */
-  int LES, SGS_model;         /* LES=0 only in this restatement */
+  int LES, SGS_model;         /* LES 1 with SGS_model 0 Smagorinsky, 1 WALE (no wall model) */
   double filter_ratio;
   double const_src;
   double dx_cyclic, dy_cyclic, dz_cyclic;

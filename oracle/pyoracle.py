@@ -100,7 +100,7 @@ class Oracle:
             "grad_disu_upts": 4, "grad_disu_fpts": 5, "detjac_upts": 6, "JGinv_upts": 7,
             "tdA_fpts": 8, "norm_fpts": 9, "pos_upts": 10, "norm_tconf_fpts": 11,
             "opp_0": 12, "opp_3": 13, "opp_4": 14, "loc_upts": 15, "tloc_fpts": 16,
-            "pos_fpts": 17}
+            "pos_fpts": 17, "sgsf_upts": 18, "sgsf_fpts": 19}
 
     def __init__(self, deck, ele_type, xv, c2v, threads=None):
         L = lib()
@@ -139,7 +139,8 @@ class Oracle:
                 "JGinv_upts": (nd, nd, nu, N), "tdA_fpts": (nfp, N), "norm_fpts": (nfp, N, nd),
                 "pos_upts": (nu, N, nd), "norm_tconf_fpts": (nfp, N, nf), "opp_0": (nfp, nu),
                 "opp_3": (nu, nfp), "opp_4": (nu, nu, nd), "loc_upts": (nd, nu),
-                "tloc_fpts": (nd, nfp), "pos_fpts": (nfp, N, nd)}[name]
+                "tloc_fpts": (nd, nfp), "pos_fpts": (nfp, N, nd), "sgsf_upts": (nu, N, nf, nd),
+                "sgsf_fpts": (nfp, N, nf, nd)}[name]
 
     def get(self, name):
         shp = self.shape_of(name)

@@ -46,6 +46,9 @@ def product_deck(g, host_only=False, **over):
         kw["vcjh_scheme_hexa"] = int(g["vcjh"])
     else:
         kw["vcjh_scheme_quad"] = int(g["vcjh"])
+    if int(g.get("les", 0)):
+        kw.update(LES=1, SGS_model=int(g["sgs_model"]), filter_ratio=float(g["filter_ratio"]), filter_type=0,
+                  wall_model=0)
     if host_only:
         kw["hifr_host_only"] = 1
     kw.update(over)
@@ -58,7 +61,8 @@ def oracle_deck(g, **over):
     kw = dict(equation=0, viscous=int(g["viscous"]), order=int(g["order"]),
               adv_type=3 if str(g["kind"]) == "rk45" else 0, dt=float(g["dt"]), tau=float(g["tau"]),
               pen_fact=float(g["pen_fact"]), vcjh_scheme=int(g["vcjh"]), eta=0.0,
-              dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2])
+              dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2], LES=int(g.get("les", 0)),
+              SGS_model=int(g.get("sgs_model", 0)), filter_ratio=float(g.get("filter_ratio", 1.0)))
     kw.update(PHYS)
     kw.update(over)
     return po.default_deck(**kw)

@@ -61,6 +61,9 @@ def deck_text(c, restart_iter=0):
         "L_free_stream 1.", "T_free_stream 300", "T_wall 300",
         "rho_bound 1.", "u_bound 1.0", "v_bound 0.0", "w_bound 0.0", "p_bound 71.428571428571431",
     ]
+    if c.get("les"):
+        lines += ["LES 1", "filter_type 0", "filter_ratio %.17g" % c["filter_ratio"],
+                  "SGS_model %d" % c["sgs_model"], "wall_model 0", "wall_layer_thickness 0.0"]
     return "\n".join(lines) + "\n"
 
 
@@ -71,7 +74,8 @@ def oracle_deck(c):
         gamma=1.4, prandtl=0.72, Mach_free_stream=c.get("mach", 0.1), nx_free_stream=1.0,
         Re_free_stream=c.get("Re", 1600.0), L_free_stream=1.0, T_free_stream=300.0, T_wall=300.0,
         vcjh_scheme=c["vcjh"], eta=0.0, dx_cyclic=c["cyc"][0], dy_cyclic=c["cyc"][1],
-        dz_cyclic=c["cyc"][2])
+        dz_cyclic=c["cyc"][2], LES=int(c.get("les", 0)), SGS_model=c.get("sgs_model", 0),
+        filter_ratio=c.get("filter_ratio", 1.0))
 
 
 def write_restart(path, ele_type, loc1d, order, u0):
@@ -140,6 +144,19 @@ CASES = [
          pen_fact=0.5, state="vortex", noise=1e-3, seed=3, perturb=0.1, kind="residual"),
     dict(name="hex_p3_ns_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=2, viscous=1, tau=1.0,
          pen_fact=0.5, state="tgv", noise=1e-3, seed=1234, perturb=0.0, kind="rk45", steps=3, dt=1e-3),
+    # LES: WALE / Smagorinsky eddy viscosity (reference CPU path)
+    dict(name="hex_p3_les_wale_perturbed_3x3x3", ele_type=4, n=(3, 3, 3), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=21, perturb=0.1, kind="residual", les=1, sgs_model=1,
+         filter_ratio=2.0),
+    dict(name="hex_p2_les_smagorinsky_3x2x2", ele_type=4, n=(3, 2, 2), order=2, vcjh=2, viscous=1, tau=0.5,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=22, perturb=0.0, kind="residual", les=1, sgs_model=0,
+         filter_ratio=1.5),
+    dict(name="quad_p3_les_wale_perturbed_4x3", ele_type=1, n=(4, 3), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="vortex", noise=1e-3, seed=23, perturb=0.1, kind="residual", les=1, sgs_model=1,
+         filter_ratio=2.0),
+    dict(name="hex_p3_les_wale_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=24, perturb=0.0, kind="rk45", steps=3, dt=1e-3, les=1,
+         sgs_model=1, filter_ratio=2.0),
 ]
 
 
@@ -186,7 +203,8 @@ def main():
         rec = dict(xv=xv, conn_gambit=conn_g, ele_type=c["ele_type"], order=c["order"],
                    vcjh=c["vcjh"], viscous=c["viscous"], tau=c.get("tau", 0.0),
                    pen_fact=c.get("pen_fact", 0.5), cyc=np.array(c["cyc"]), dt=c["dt"],
-                   u0=u0, kind=c["kind"])
+                   u0=u0, kind=c["kind"], les=int(c.get("les", 0)), sgs_model=int(c.get("sgs_model", 0)),
+                   filter_ratio=float(c.get("filter_ratio", 1.0)))
         if c["kind"] == "residual":
             R = (u0 - u1) / c["dt"]
             rec["R"] = R

@@ -59,7 +59,8 @@ def _kinetic_energy(u, detj, w3):
     return float(np.einsum("j,je,je->", w3, detj, ke))
 
 
-@pytest.mark.parametrize("name", ["hex_p3_ns_tgv_2x2x2", "hex_p3_ns_perturbed_3x3x3", "hex_p4_ns_2x2x2"])
+@pytest.mark.parametrize("name", ["hex_p3_ns_tgv_2x2x2", "hex_p3_ns_perturbed_3x3x3", "hex_p4_ns_2x2x2",
+                                  "hex_p3_les_wale_perturbed_3x3x3"])
 def test_mixed_residual_close_to_oracle(require_gpu, name):
     g = cases.load_golden(name)
     ele_type, xv, c2v = cases.mesh_of(g)

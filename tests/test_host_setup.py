@@ -33,9 +33,13 @@ def test_host_setup_matches_oracle(name):
 def test_host_rejects_unsupported_decks():
     g = cases.load_golden("hex_p3_ns_tgv_2x2x2")
     ele_type, xv, c2v = cases.mesh_of(g)
-    with pytest.raises(RuntimeError, match="LES"):
-        hifr.Solver.from_arrays(cases.product_deck(g, host_only=True, LES=1, SGS_model=1, filter_ratio=1.0,
+    # LES is built for the Smagorinsky and WALE models without wall model
+    with pytest.raises(RuntimeError, match="SGS models"):
+        hifr.Solver.from_arrays(cases.product_deck(g, host_only=True, LES=1, SGS_model=2, filter_ratio=1.0,
                                                    wall_model=0), ele_type, xv, c2v)
+    with pytest.raises(RuntimeError, match="wall model"):
+        hifr.Solver.from_arrays(cases.product_deck(g, host_only=True, LES=1, SGS_model=1, filter_ratio=1.0,
+                                                   wall_model=1), ele_type, xv, c2v)
     with pytest.raises(RuntimeError, match="no neighbour"):
         hifr.Solver.from_arrays(cases.product_deck(g, host_only=True, dx_cyclic=float("inf")), ele_type, xv, c2v)
 
