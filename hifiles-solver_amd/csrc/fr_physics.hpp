@@ -21,7 +21,27 @@ namespace hifr {
 
 struct Phys {
   double gamma, prandtl, mu_inf, rt_inf, c_sth, fix_vis, tau, pen_fact, const_src;
+  double R_ref, T_wall;  // isothermal walls
 };
+
+// Ghost state of an isothermal no-slip wall on a static mesh (bctype 11,
+// src/bdy_inters.cpp:391-420, 668-697, 884-889): pressure extrapolated from the
+// trace ul, temperature T_wall, rho = p / (R_ref T_wall), velocity zero.
+template <int ND, class T>
+__device__ __forceinline__ void wall_ghost(const T* ul, T gamma, T R_ref, T T_wall, T* ur) {
+  T v_sq = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    const T v = ul[i + 1] / ul[0];
+    v_sq += v * v;
+  }
+  const T p = (gamma - T(1)) * (ul[ND + 1] - T(0.5) * ul[0] * v_sq);
+  const T rho = p / (R_ref * T_wall);
+  ur[0] = rho;
+#pragma unroll
+  for (int i = 0; i < ND; i++) ur[i + 1] = T(0);
+  ur[ND + 1] = p / (gamma - T(1));
+}
 
 // Reference state at rest of the mixed-precision path (rho_r, E_r; p_r and the
 // total enthalpy H_r = E_r + p_r follow).

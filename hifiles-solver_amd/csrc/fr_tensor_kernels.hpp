@@ -52,6 +52,9 @@ using Ops1D = Ops1DT<double>;
 
 enum { MODE_TRACE_U = 0, MODE_TRACE_GRAD = 1, MODE_RESID = 2, MODE_ADVANCE = 3 };
 
+// nbr entry of a flux point on a physical boundary (isothermal wall)
+constexpr int BDY_FLAG = 1 << 30;
+
 // R: arithmetic / trace type (double, or float on the mixed path)
 template <class R>
 struct StageArgsT {
@@ -66,7 +69,8 @@ struct StageArgsT {
   R* gf_out;             // optional gradient traces (TRACE_GRAD, may be null)
   const double* detj;    // per point, or per element (AFF)
   const R* jg;           // per point, or per element (AFF)
-  const int* nbr;        // neighbour trace index per element flux point; bit 31: LDG switch -
+  const int* nbr;        // neighbour trace index per element flux point; bit 31: LDG switch -;
+                         // boundary points: BDY_FLAG | bctype
   const double* fgeo;    // own-side outward normal (d) and tdA, component-major; per flux
                          // point, or per face (AFF)
   double rk_a, rk_b, dt;
@@ -195,6 +199,16 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       for (int k = 0; k < NF; k++) out[k] = w[k];
     }
   };
+  // wall ghost state in the representation of the traces (deviations on MIX)
+  auto ghost = [&](const R* wo, R* wr) {
+    R uo_full[NF];
+    full(wo, uo_full);
+    wall_ghost<ND, R>(uo_full, gam, R(a.ph.R_ref), R(a.ph.T_wall), wr);
+    if constexpr (MIX) {
+      wr[0] -= ref.rho;
+      wr[NF - 1] -= ref.E;
+    }
+  };
 
   const int N = a.N;
   const long NN = (long)N;
@@ -249,11 +263,19 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         nb &= 0x7fffffff;
         R uo[NF];
         interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
+        if (nb & BDY_FLAG) {
+          // wall: Dirichlet common solution u_c = (u_l + u_r)/2 (src/inters.cpp:664-668)
+          R ur[NF];
+          ghost(uo, ur);
 #pragma unroll
-        for (int k = 0; k < NF; k++) {
-          const R un = a.uf_in[nb + NFP * NN * k];
-          const R uc = R(0.5) * (uo[k] + un) - pen * (uo[k] - un);
-          s_q[(tl * NFP + fp) * NF + k] = uc - uo[k];
+          for (int k = 0; k < NF; k++) s_q[(tl * NFP + fp) * NF + k] = R(0.5) * (ur[k] + uo[k]) - uo[k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
+            const R un = a.uf_in[nb + NFP * NN * k];
+            const R uc = R(0.5) * (uo[k] + un) - pen * (uo[k] - un);
+            s_q[(tl * NFP + fp) * NF + k] = uc - uo[k];
+          }
         }
       }
       __syncthreads();
@@ -460,10 +482,27 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         const R tdA = R(a.fgeo[ND * NN * GEO_PER_ELE + gi]);
         R uo[NF], un[NF], fn[NF];
         interp_line<n, NF, NU>(s_u + tl * NF * NU, lw, base, st, uo);
+        const bool wall = (nb & BDY_FLAG) != 0;
+        if (wall) {
+          // isothermal wall: central flux between the trace and the ghost state
+          // (convective_flux_boundary, src/inters.cpp:368-387)
+          ghost(uo, un);
+          R fo[NF], fg[NF];
+          if constexpr (MIX) {
+            inv_flux_dot_dev<ND, R>(uo, nrm, gam, ref, fo);
+            inv_flux_dot_dev<ND, R>(un, nrm, gam, ref, fg);
+          } else {
+            inv_flux_dot<ND, R>(uo, nrm, gam, fo);
+            inv_flux_dot<ND, R>(un, nrm, gam, fg);
+          }
 #pragma unroll
-        for (int k = 0; k < NF; k++) un[k] = a.uf_in[nb + NFP * NN * k];
-        if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
-        else rusanov<ND, R>(uo, un, nrm, gam, fn);
+          for (int k = 0; k < NF; k++) fn[k] = R(0.5) * (fo[k] + fg[k]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < NF; k++) un[k] = a.uf_in[nb + NFP * NN * k];
+          if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
+          else rusanov<ND, R>(uo, un, nrm, gam, fn);
+        }
         R jump[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) jump[k] = fn[k] * tdA;
@@ -471,12 +510,22 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           R nn = 0;
 #pragma unroll
           for (int l = 0; l < ND; l++) nn += nrm[l] * nrm[l];
+          const R tau = R(a.ph.tau);
+          if (wall) {
+            // Dirichlet LDG flux f_c = f_l + tau n (u_l - u_r) (src/inters.cpp:605-619)
 #pragma unroll
-          for (int k = 0; k < NF; k++) {
-            // own F_v.n and the neighbour's, whose normal is the negated own normal
-            const R fa = a.fv_in[fe + NFP * NN * k];
-            const R fb = -a.fv_in[nb + NFP * NN * k];
-            jump[k] += ldg_normal<R>(fa, fb, uo[k], un[k], nn, pen, R(a.ph.tau)) * tdA;
+            for (int k = 0; k < NF; k++) {
+              const R fa = a.fv_in[fe + NFP * NN * k];
+              jump[k] += (fa + tau * nn * (uo[k] - un[k])) * tdA;
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < NF; k++) {
+              // own F_v.n and the neighbour's, whose normal is the negated own normal
+              const R fa = a.fv_in[fe + NFP * NN * k];
+              const R fb = -a.fv_in[nb + NFP * NN * k];
+              jump[k] += ldg_normal<R>(fa, fb, uo[k], un[k], nn, pen, tau) * tdA;
+            }
           }
         }
         // minus the normal transformed discontinuous flux (opp_1 = opp_0 o tnorm)

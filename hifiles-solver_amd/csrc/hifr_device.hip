@@ -33,6 +33,7 @@ struct hifr_ctx {
   int cur;      // which uf buffer holds the traces of the current state
   // host copies needed to build the interface tables
   std::vector<double> h_tdA, h_norm;
+  std::vector<int> h_bdy_fpt, h_bdy_type;  // physical boundary points (hifr_set_boundaries)
   // timing
   bool timing;
   std::vector<hipEvent_t> ev;
@@ -89,6 +90,8 @@ StageArgsT<R> base_args(hifr_ctx* c) {
   a.ph.tau = c->cfg.tau;
   a.ph.pen_fact = c->cfg.pen_fact;
   a.ph.const_src = c->cfg.const_src;
+  a.ph.R_ref = c->cfg.R_ref;
+  a.ph.T_wall = c->cfg.T_wall;
   a.ref = c->ref;
   a.filter_ratio = c->cfg.filter_ratio;
   return a;
@@ -440,6 +443,19 @@ hifr_status hifr_metrics_per_element(const hifr_ctx* c, int* per_element) {
   return HIFR_OK;
 }
 
+hifr_status hifr_set_boundaries(hifr_ctx* c, int n_pts, const int32_t* fpt, const int32_t* bctype) {
+  if (!c || n_pts < 0 || (n_pts > 0 && (!fpt || !bctype))) return HIFR_ERR_ARG;
+  const size_t tot = (size_t)c->nfp * c->N;
+  for (int q = 0; q < n_pts; q++) {
+    if (fpt[q] < 0 || (size_t)fpt[q] >= tot) return HIFR_ERR_ARG;
+    if (bctype[q] != 11) return HIFR_ERR_UNSUPPORTED;
+  }
+  c->h_bdy_fpt.assign(fpt, fpt + n_pts);
+  c->h_bdy_type.assign(bctype, bctype + n_pts);
+  c->have_faces = false;
+  return HIFR_OK;
+}
+
 hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, const int32_t* right) {
   if (!c || n_pts < 0 || (n_pts > 0 && (!left || !right))) return HIFR_ERR_ARG;
   if (!c->have_metrics) return HIFR_ERR_STATE;
@@ -450,6 +466,16 @@ hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, con
   std::vector<int> nbr(tot, -1);
   std::vector<double> geo((nd + 1) * gtot, 0.0);
   std::vector<char> seen(tot, 0);
+  // physical boundary points: own outward normal and tdA
+  for (size_t q = 0; q < c->h_bdy_fpt.size(); q++) {
+    const size_t B = (size_t)c->h_bdy_fpt[q];
+    if (seen[B]) return HIFR_ERR_ARG;
+    seen[B] = 1;
+    nbr[B] = hifr::BDY_FLAG | c->h_bdy_type[q];
+    const size_t gb = gidx(B);
+    for (size_t m = 0; m < nd; m++) geo[m * gtot + gb] = c->h_norm[B + tot * m];
+    geo[nd * gtot + gb] = c->h_tdA[B];
+  }
   for (int q = 0; q < n_pts; q++) {
     const long L = left[q], R = right[q];
     if (L < 0 || R < 0 || (size_t)L >= tot || (size_t)R >= tot || seen[L] || seen[R] || L == R) return HIFR_ERR_ARG;
@@ -473,7 +499,7 @@ hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, con
   }
   for (size_t i = 0; i < tot; i++)
     if (!seen[i]) {
-      std::fprintf(stderr, "hifr: flux point %zu of element %zu has no interface (boundaries not built)\n", i % nfp,
+      std::fprintf(stderr, "hifr: flux point %zu of element %zu has no interface or boundary\n", i % nfp,
                    i / nfp);
       return HIFR_ERR_UNSUPPORTED;
     }

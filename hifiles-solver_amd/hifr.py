@@ -32,6 +32,8 @@ def lib():
         vp, dp, ip, cp = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_char_p
         L.hifr_solver_from_deck.argtypes = [cp, C.POINTER(vp), cp, C.c_int]
         L.hifr_solver_from_arrays.argtypes = [cp, C.c_int, C.c_int, dp, C.c_int, ip, C.POINTER(vp), cp, C.c_int]
+        L.hifr_solver_from_arrays_bc.argtypes = [cp, C.c_int, C.c_int, dp, C.c_int, ip, C.c_int, ip,
+                                                 C.POINTER(vp), cp, C.c_int]
         L.hifr_solver_box.argtypes = [cp, C.c_int, ip, dp, dp, C.c_int, C.POINTER(vp), cp, C.c_int]
         L.hifr_solver_destroy.argtypes = [vp]
         L.hifr_solver_sizes.argtypes = [vp, ip]
@@ -100,12 +102,21 @@ class Solver:
         return cls(h)
 
     @classmethod
-    def from_arrays(cls, deck, ele_type, xv, c2v):
+    def from_arrays(cls, deck, ele_type, xv, c2v, bfaces=None):
+        """bfaces: optional (n, 3) (element, local face, bctype) physical boundary faces."""
         xv = np.ascontiguousarray(xv, dtype=np.float64)
         c2v = np.ascontiguousarray(c2v, dtype=np.int32)
         h, err = C.c_void_p(), C.create_string_buffer(1024)
+        ip = C.POINTER(C.c_int)
+        if bfaces is not None and len(bfaces):
+            bf = np.ascontiguousarray(np.asarray(bfaces, dtype=np.int32).reshape(-1, 3))
+            rc = lib().hifr_solver_from_arrays_bc(deck.encode(), int(ele_type), xv.shape[0], _dp(xv), c2v.shape[0],
+                                                  c2v.ctypes.data_as(ip), bf.shape[0], bf.ctypes.data_as(ip),
+                                                  C.byref(h), err, 1024)
+            cls._check(rc, err)
+            return cls(h)
         rc = lib().hifr_solver_from_arrays(deck.encode(), int(ele_type), xv.shape[0], _dp(xv), c2v.shape[0],
-                                           c2v.ctypes.data_as(C.POINTER(C.c_int)), C.byref(h), err, 1024)
+                                           c2v.ctypes.data_as(ip), C.byref(h), err, 1024)
         cls._check(rc, err)
         return cls(h)
 

@@ -254,6 +254,8 @@ std::string eles::mv_all_cpu_gpu(const input& in) {
   c.les = in.LES;
   c.sgs_model = in.SGS_model;
   c.filter_ratio = in.filter_ratio;
+  c.R_ref = in.R_ref;
+  c.T_wall = in.T_wall;
   if (ctx) { hifr_destroy(ctx); ctx = nullptr; }
   hifr_status s = hifr_create(&c, &ctx);
   if (s != HIFR_OK) return std::string("hifr_create: ") + hifr_status_string(s);

@@ -20,6 +20,17 @@ const int kQuadPos[4] = {0, 1, 3, 2};
 
 }  // namespace
 
+int bc_number(std::string name) {
+  std::transform(name.begin(), name.end(), name.begin(), ::tolower);
+  static const char* names[] = {"sub_in_simp", "sub_out_simp", "sub_in_char", "sub_out_char", "sup_in",
+                                "sup_out", "slip_wall", "", "cyclic", "", "isotherm_fix", "adiabat_fix",
+                                "isotherm_move", "adiabat_move", "char", "slip_wall_dual"};
+  for (int i = 0; i < 16; i++)
+    if (names[i][0] && name == names[i]) return i + 1;
+  if (name == "ad_wall") return 50;
+  return -1;
+}
+
 std::string read_gambit(const std::string& path, mesh& m) {
   std::ifstream f(path);
   if (!f) return "Unable to open mesh file " + path;
@@ -66,6 +77,35 @@ std::string read_gambit(const std::string& path, mesh& m) {
     }
   }
   if (!f) return "error reading Gambit elements";
+  // element groups, then boundary groups (src/geometry.cpp:1186-1290)
+  // Gambit face number -> internal face (hexes; quads: face - 1)
+  static const int kHexFace[7] = {-1, 0, 3, 5, 1, 4, 2};
+  m.bf_ele.clear();
+  m.bf_face.clear();
+  m.bf_type.clear();
+  while (std::getline(f, line)) {
+    if (line.find("BOUNDARY CONDITIONS") == std::string::npos) continue;
+    std::getline(f, line);
+    std::istringstream hs(line);
+    std::string name;
+    int itype = 0, nent = 0;
+    hs >> name >> itype >> nent;
+    if (!hs) return "error reading a Gambit boundary group header";
+    const int type = bc_number(name);
+    if (type < 0) return "boundary condition '" + name + "' not recognized";
+    for (int q = 0; q < nent; q++) {
+      int ge, gt, gf;
+      f >> ge >> gt >> gf;
+      if (!f || ge < 1 || ge > ne) return "error reading Gambit boundary faces of group " + name;
+      if (type == 9) continue;  // cyclic: matched by translation
+      const int lf = m.ele_type == 4 ? (gf >= 1 && gf <= 6 ? kHexFace[gf] : -1) : gf - 1;
+      if (lf < 0 || lf >= (m.ele_type == 4 ? 6 : 4)) return "invalid Gambit face number in group " + name;
+      m.bf_ele.push_back(ge - 1);
+      m.bf_face.push_back(lf);
+      m.bf_type.push_back(type);
+    }
+    std::getline(f, line);
+  }
   return "";
 }
 
@@ -141,6 +181,19 @@ std::string build_interfaces(const eles& el, const mesh& m, const double* delta,
             nfpf = el.n_fpts_per_inter, ns = m.n_spts;
   const long nrec = (long)N * nfaces;
   std::vector<long> partner(nrec, -1);
+  // boundary faces: partner -2, never matched
+  out.bdy_fpt.clear();
+  out.bdy_type.clear();
+  for (size_t q = 0; q < m.bf_ele.size(); q++) {
+    const int e = m.bf_ele[q], f = m.bf_face[q];
+    if (e < 0 || e >= N || f < 0 || f >= nfaces) return "boundary face out of range";
+    if (partner[(long)e * nfaces + f] == -2) return "boundary face listed twice";
+    partner[(long)e * nfaces + f] = -2;
+    for (int j = 0; j < nfpf; j++) {
+      out.bdy_fpt.push_back(f * nfpf + j + nfp * e);
+      out.bdy_type.push_back(m.bf_type[q]);
+    }
+  }
   {
     std::unordered_map<FaceKey, long, FaceKeyHash> first;
     first.reserve(nrec);
@@ -153,6 +206,7 @@ std::string build_interfaces(const eles& el, const mesh& m, const double* delta,
         for (int q = 0; q < nv; q++) k.v[q] = m.c2v[(size_t)e * ns + fn[q]];
         std::sort(k.v.begin(), k.v.begin() + nv);
         const long id = (long)e * nfaces + f;
+        if (partner[id] == -2) continue;
         auto it = first.find(k);
         if (it == first.end()) {
           first.emplace(k, id);
@@ -166,7 +220,7 @@ std::string build_interfaces(const eles& el, const mesh& m, const double* delta,
   // centroids of the unmatched faces
   std::vector<long> un;
   for (long i = 0; i < nrec; i++)
-    if (partner[i] < 0) un.push_back(i);
+    if (partner[i] == -1) un.push_back(i);
   auto centroid = [&](long id, double* c) {
     const int e = (int)(id / nfaces), f = (int)(id % nfaces);
     int nv;
@@ -227,15 +281,18 @@ std::string build_interfaces(const eles& el, const mesh& m, const double* delta,
       }
     }
   }
-  for (long i = 0; i < nrec; i++)
-    if (partner[i] < 0) {
+  long n_bdy_faces = 0;
+  for (long i = 0; i < nrec; i++) {
+    if (partner[i] == -2) n_bdy_faces++;
+    if (partner[i] == -1) {
       char buf[160];
-      std::snprintf(buf, sizeof buf, "face %ld of element %ld has no neighbour (walls are not built)", i % nfaces,
-                    i / nfaces);
+      std::snprintf(buf, sizeof buf, "face %ld of element %ld has no neighbour and no boundary condition",
+                    i % nfaces, i / nfaces);
       return buf;
     }
+  }
   // pair flux points by position
-  out.n_inters = (int)(nrec / 2);
+  out.n_inters = (int)((nrec - n_bdy_faces) / 2);
   out.left.assign((size_t)out.n_inters * nfpf, 0);
   out.right.assign((size_t)out.n_inters * nfpf, 0);
   const size_t NN = (size_t)N;

@@ -14,23 +14,33 @@ struct mesh {
   int n_dims = 0, ele_type = 0, n_verts = 0, n_eles = 0, n_spts = 0;
   std::vector<double> xv;  // (n_verts, d) row per vertex
   std::vector<int> c2v;    // (n_eles, n_spts) in the internal shape-node order
+  // physical boundary faces: element, local face (internal numbering), bctype
+  std::vector<int> bf_ele, bf_face, bf_type;
 };
 
-// Gambit .neu reader (src/geometry.cpp:1544-1881): one linear element type (quads
-// or 8-node hexes); node order permuted as in read_connectivity_gambit.
+// Boundary-condition name of a Gambit group -> bctype (src/geometry.cpp:121-149);
+// -1 for an unknown name.
+int bc_number(std::string name);
+
+// Gambit .neu reader (src/geometry.cpp:1122-1881): one linear element type (quads
+// or 8-node hexes); node order permuted as in read_connectivity_gambit; boundary
+// groups other than "cyclic" (matched geometrically) become boundary faces.
 std::string read_gambit(const std::string& path, mesh& m);
 
 // Structured box of nx*ny(*nz) elements on [o, o+l], optionally with vertex
 // coordinates rounded to `digits` significant digits (0 = exact).
 void make_box(int ele_type, const int* nxyz, const double* l, const double* o, int digits, mesh& m);
 
-// Interior + periodic interfaces as flux-point pairs (fpt + Nfp*ele) left/right.
+// Interior + periodic interfaces as flux-point pairs (fpt + Nfp*ele) left/right,
+// and the flux points of the boundary faces with their bctype.
 struct int_inters {
   int n_inters = 0;
   std::vector<int> left, right;
+  std::vector<int> bdy_fpt, bdy_type;
 };
 
-// Faces are matched by shared vertices; faces left over are matched by
+// Faces listed in m.bf_* are boundary faces and are not matched.  Other faces are
+// matched by shared vertices; faces left over are matched by
 // translation over the finite entries of delta (dx/dy/dz_cyclic) with the
 // reference's tolerance 1e-6 (src/geometry.cpp:489-563, 3634-3664).  Flux points
 // are paired by position, which is what the rot_tag lookup of

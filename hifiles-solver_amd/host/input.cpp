@@ -7,6 +7,7 @@
 #include <cmath>
 #include <fstream>
 #include <map>
+#include <cstdlib>
 #include <sstream>
 
 namespace hifr_host {
@@ -27,6 +28,14 @@ struct deck {
     std::istringstream ss(it->second);
     T t;
     if (ss >> t) v = t;
+  }
+  // doubles through strtod, which also reads inf / nan
+  void get(const char* k, double& v) const {
+    auto it = kv.find(k);
+    if (it == kv.end()) return;
+    char* end = nullptr;
+    const double t = std::strtod(it->second.c_str(), &end);
+    if (end != it->second.c_str()) v = t;
   }
 };
 

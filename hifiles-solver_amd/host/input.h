@@ -1,6 +1,7 @@
 // Input deck of the solver: the reference's `name value` file format, key names and
 // defaults (src/input.cpp:89-331) and its non-dimensionalisation (:345-515).
 #pragma once
+#include <limits>
 #include <string>
 
 namespace hifr_host {
@@ -21,7 +22,8 @@ struct input {
   // gas
   double gamma = 1.4, prandtl = 0.72, S_gas = 120., T_gas = 291.15, R_gas = 286.9, mu_gas = 1.827E-5;
   // boundary conditions / reference state
-  double dx_cyclic, dy_cyclic, dz_cyclic;
+  double dx_cyclic = std::numeric_limits<double>::infinity(), dy_cyclic = std::numeric_limits<double>::infinity(),
+         dz_cyclic = std::numeric_limits<double>::infinity();
   double Mach_free_stream = 0., nx_free_stream = 0., ny_free_stream = 0., nz_free_stream = 0.;
   double Re_free_stream = 0., L_free_stream = 1., T_free_stream = 300.;
   double rho_bound = 1., v_bound[3] = {0, 0, 0}, p_bound = 1.;

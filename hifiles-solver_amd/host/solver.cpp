@@ -170,9 +170,16 @@ std::string setup_solver(hifr_solver* s) {
   const double delta[3] = {in.dx_cyclic, in.dy_cyclic, in.dz_cyclic};
   err = build_interfaces(e, m, delta, s->ii);
   if (!err.empty()) return err;
+  for (int t : s->ii.bdy_type)
+    if (t != 11) return "boundary condition " + std::to_string(t) + " is not built (isothermal walls, bctype 11)";
   if (in.host_only) return "";
   err = e.mv_all_cpu_gpu(in);
   if (!err.empty()) return err;
+  if (!s->ii.bdy_fpt.empty()) {
+    err = check(hifr_set_boundaries(e.ctx, (int)s->ii.bdy_fpt.size(), s->ii.bdy_fpt.data(), s->ii.bdy_type.data()),
+                "hifr_set_boundaries");
+    if (!err.empty()) return err;
+  }
   err = check(hifr_set_interfaces(e.ctx, (int)s->ii.left.size(), s->ii.left.data(), s->ii.right.data()),
               "hifr_set_interfaces");
   if (!err.empty()) return err;
@@ -229,6 +236,32 @@ int hifr_solver_from_arrays(const char* deck_text, int ele_type, int n_verts, co
   m.n_eles = n_eles;
   m.xv.assign(xv, xv + (size_t)n_verts * m.n_dims);
   m.c2v.assign(c2v, c2v + (size_t)n_eles * m.n_spts);
+  e = setup_solver(s.get());
+  if (!e.empty()) return fail(e, err, errlen);
+  *out = s.release();
+  return 0;
+}
+
+int hifr_solver_from_arrays_bc(const char* deck_text, int ele_type, int n_verts, const double* xv, int n_eles,
+                               const int* c2v, int n_bfaces, const int* bfaces, hifr_solver** out, char* err,
+                               int errlen) {
+  *out = nullptr;
+  std::unique_ptr<hifr_solver> s(new hifr_solver());
+  std::string e = read_text_deck(deck_text, s->in);
+  if (!e.empty()) return fail(e, err, errlen);
+  mesh& m = s->msh;
+  m.ele_type = ele_type;
+  m.n_dims = ele_type == 4 ? 3 : 2;
+  m.n_spts = ele_type == 4 ? 8 : 4;
+  m.n_verts = n_verts;
+  m.n_eles = n_eles;
+  m.xv.assign(xv, xv + (size_t)n_verts * m.n_dims);
+  m.c2v.assign(c2v, c2v + (size_t)n_eles * m.n_spts);
+  for (int q = 0; q < n_bfaces; q++) {
+    m.bf_ele.push_back(bfaces[3 * q]);
+    m.bf_face.push_back(bfaces[3 * q + 1]);
+    m.bf_type.push_back(bfaces[3 * q + 2]);
+  }
   e = setup_solver(s.get());
   if (!e.empty()) return fail(e, err, errlen);
   *out = s.release();

@@ -72,6 +72,9 @@ typedef struct {
    * viscosity (fix_vis 1).  No wall model. */
   int les, sgs_model;
   double filter_ratio;
+  /* isothermal walls (bctype 11): gas constant R_ref and wall temperature T_wall,
+   * non-dimensional (run_input after setup_params, src/input.cpp:476-512) */
+  double R_ref, T_wall;
 } hifr_config;
 
 /* How the context stores the metrics it is given (no counterpart in the reference,
@@ -111,6 +114,13 @@ hifr_status hifr_set_metrics(hifr_ctx* ctx, const double* detjac_upts, const dou
 
 /* 1 if the context keeps one metric set per element (see HIFR_METRICS_AUTO), else 0. */
 hifr_status hifr_metrics_per_element(const hifr_ctx* ctx, int* per_element);
+
+/* Physical boundary flux points (bdy_inters::set_boundary pointer tables,
+ * src/bdy_inters.cpp:114-148): the element flux-point index (fpt + Nfp*ele) and the
+ * bctype of each point; bctype 11 (isothermal no-slip wall, static mesh) is built.
+ * Call before hifr_set_interfaces, which then covers the remaining points; a
+ * new call replaces the list (n_pts 0 clears it). */
+hifr_status hifr_set_boundaries(hifr_ctx* ctx, int n_pts, const int32_t* fpt, const int32_t* bctype);
 
 /* Interior and periodic interfaces: for each of n_pts interface points, the
  * element flux-point index (fpt + Nfp*ele) on the left and on the right.  This is

@@ -29,6 +29,14 @@ int hifr_solver_from_deck(const char* deck_path, hifr_solver** out, char* err, i
 int hifr_solver_from_arrays(const char* deck_text, int ele_type, int n_verts, const double* xv, int n_eles,
                             const int* c2v, hifr_solver** out, char* err, int errlen);
 
+/* As hifr_solver_from_arrays, with n_bfaces physical boundary faces given as
+ * (element, local face in the reference's numbering, bctype) triples (Gambit
+ * boundary groups, src/geometry.cpp:1122-1290); bctype 11 (isothermal no-slip
+ * wall) is built. */
+int hifr_solver_from_arrays_bc(const char* deck_text, int ele_type, int n_verts, const double* xv, int n_eles,
+                               const int* c2v, int n_bfaces, const int* bfaces, hifr_solver** out, char* err,
+                               int errlen);
+
 /* Deck given as text; structured box mesh of n[0] x n[1] (x n[2]) elements on
  * [o, o+l], vertex coordinates rounded to `digits` significant digits (0: exact). */
 int hifr_solver_box(const char* deck_text, int ele_type, const int* n, const double* l, const double* o,

@@ -169,6 +169,8 @@ struct or_solver {
       *norm_tconf_fpts, *delta_disu_fpts, *grad_disu_upts, *grad_disu_fpts, *src_upts;
   double *sgsf_upts, *sgsf_fpts; /* LES: transformed SGS flux (Nu,N,nf,d), (Nfp,N,nf,d) */
   int n_inters;
+  int n_bdy;                 /* boundary flux points */
+  int *bdy_fpt, *bdy_type;   /* (n_bdy) flux-point index fpt + Nfp*ele, bctype */
   int *face_l, *face_r; /* (n_inters*nfpf) flux-point indices fpt + Nfp*ele */
 };
 
@@ -547,13 +549,17 @@ static void pair_points(or_solver* s, int eL, int fL, int eR, int fR, const doub
   }
 }
 
-static int set_connectivity(or_solver* s, const int* c2v, const double* xv) {
+/* is_bdy: (N*nfaces) 1 for faces on a physical boundary (not matched) */
+static int set_connectivity(or_solver* s, const int* c2v, const double* xv, const char* is_bdy) {
   int N = s->n_eles, nfaces = s->nfaces, nv = s->ele_type == 4 ? 4 : 2;
-  long nrec = (long)N * nfaces;
+  long nrec = 0;
+  for (long i = 0; i < (long)N * nfaces; i++) nrec += !is_bdy[i];
   face_rec* r = xcalloc(nrec, sizeof(face_rec));
+  long ir = 0;
   for (int e = 0; e < N; e++)
     for (int f = 0; f < nfaces; f++) {
-      face_rec* q = r + (long)e * nfaces + f;
+      if (is_bdy[(long)e * nfaces + f]) continue;
+      face_rec* q = r + ir++;
       q->e = e; q->f = f;
       for (int k = 0; k < 4; k++) q->key[k] = -1;
       for (int k = 0; k < nv; k++) {
@@ -563,9 +569,10 @@ static int set_connectivity(or_solver* s, const int* c2v, const double* xv) {
       sort4(q->key, nv);
     }
   qsort(r, nrec, sizeof(face_rec), cmp_face);
-  /* partner[e*nfaces+f] = other (e,f) or -1 */
-  long* partner = xcalloc(nrec, sizeof(long));
-  for (long i = 0; i < nrec; i++) partner[i] = -1;
+  /* partner[e*nfaces+f] = other (e,f) or -1; boundary faces -2 */
+  long nall = (long)N * nfaces;
+  long* partner = xcalloc(nall, sizeof(long));
+  for (long i = 0; i < nall; i++) partner[i] = is_bdy[i] ? -2 : -1;
   for (long i = 0; i + 1 < nrec; i++) {
     if (memcmp(r[i].key, r[i + 1].key, sizeof(r[i].key)) == 0) {
       long a = (long)r[i].e * nfaces + r[i].f, b = (long)r[i + 1].e * nfaces + r[i + 1].f;
@@ -577,8 +584,8 @@ static int set_connectivity(or_solver* s, const int* c2v, const double* xv) {
   /* periodic matching of the remaining faces */
   double delta[3] = {s->deck.dx_cyclic, s->deck.dy_cyclic, s->deck.dz_cyclic};
   long nun = 0;
-  long* un = xcalloc(nrec, sizeof(long));
-  for (long i = 0; i < nrec; i++) if (partner[i] < 0) un[nun++] = i;
+  long* un = xcalloc(nall, sizeof(long));
+  for (long i = 0; i < nall; i++) if (partner[i] == -1) un[nun++] = i;
   double* cen = xcalloc((size_t)nun * 3, sizeof(double));
   for (long i = 0; i < nun; i++) face_centroid(s, c2v, xv, (int)(un[i] / nfaces), (int)(un[i] % nfaces), cen + 3 * i);
   const double tol = 1.e-6;
@@ -597,9 +604,9 @@ static int set_connectivity(or_solver* s, const int* c2v, const double* xv) {
       if (ok) { partner[un[i]] = un[j]; partner[un[j]] = un[i]; break; }
     }
   }
-  for (long i = 0; i < nrec; i++)
-    if (partner[i] < 0) {
-      fprintf(stderr, "fr_oracle: unmatched face (ele %ld, face %ld); walls are not restated\n",
+  for (long i = 0; i < nall; i++)
+    if (partner[i] == -1) {
+      fprintf(stderr, "fr_oracle: unmatched face (ele %ld, face %ld) without boundary condition\n",
               i / nfaces, i % nfaces);
       free(partner); free(un); free(cen);
       return -1;
@@ -609,7 +616,7 @@ static int set_connectivity(or_solver* s, const int* c2v, const double* xv) {
   s->face_l = xcalloc((size_t)s->n_inters * s->nfpf, sizeof(int));
   s->face_r = xcalloc((size_t)s->n_inters * s->nfpf, sizeof(int));
   int it = 0;
-  for (long a = 0; a < nrec; a++) {
+  for (long a = 0; a < nall; a++) {
     long b = partner[a];
     if (b < a) continue;
     int eL = (int)(a / nfaces), fL = (int)(a % nfaces), eR = (int)(b / nfaces), fR = (int)(b % nfaces);
@@ -631,6 +638,15 @@ static int set_connectivity(or_solver* s, const int* c2v, const double* xv) {
 
 or_solver* or_create(const or_deck* deck, int ele_type, int n_verts, const double* xv, int n_eles,
                      const int* c2v) {
+  return or_create_bc(deck, ele_type, n_verts, xv, n_eles, c2v, 0, NULL, NULL, NULL);
+}
+
+/* Boundary faces (element, local face in the reference's numbering, bctype) are
+ * taken out of the face matching and become boundary interfaces
+ * (bdy_inters::set_boundary, src/bdy_inters.cpp:114-148); bctype 11 only. */
+or_solver* or_create_bc(const or_deck* deck, int ele_type, int n_verts, const double* xv, int n_eles,
+                        const int* c2v, int n_bfaces, const int* bf_ele, const int* bf_face,
+                        const int* bf_type) {
   if (ele_type != 1 && ele_type != 4) { fprintf(stderr, "fr_oracle: ele_type %d\n", ele_type); return NULL; }
   if (deck->order < 0 || deck->order > 6 || deck->upts_type != 0 || deck->equation != 0 ||
       (deck->LES && (!deck->viscous || deck->SGS_model < 0 || deck->SGS_model > 1))) {
@@ -661,7 +677,26 @@ or_solver* or_create(const or_deck* deck, int ele_type, int n_verts, const doubl
       for (int m = 0; m < s->nd; m++)
         s->shape[m + s->nd * (j + s->nsp * e)] = xv[m + s->nd * c2v[j + s->nsp * e]];
   set_transforms(s);
-  if (set_connectivity(s, c2v, xv) != 0) { or_destroy(s); return NULL; }
+  char* is_bdy = xcalloc((size_t)n_eles * s->nfaces, 1);
+  for (int q = 0; q < n_bfaces; q++) {
+    if (bf_ele[q] < 0 || bf_ele[q] >= n_eles || bf_face[q] < 0 || bf_face[q] >= s->nfaces || bf_type[q] != 11 ||
+        !deck->viscous) {
+      fprintf(stderr, "fr_oracle: boundary face %d not supported (isothermal walls, viscous only)\n", q);
+      free(is_bdy); or_destroy(s); return NULL;
+    }
+    is_bdy[(long)bf_ele[q] * s->nfaces + bf_face[q]] = 1;
+  }
+  s->n_bdy = n_bfaces * s->nfpf;
+  s->bdy_fpt = xcalloc((size_t)s->n_bdy, sizeof(int));
+  s->bdy_type = xcalloc((size_t)s->n_bdy, sizeof(int));
+  for (int q = 0; q < n_bfaces; q++)
+    for (int j = 0; j < s->nfpf; j++) {
+      s->bdy_fpt[q * s->nfpf + j] = bf_face[q] * s->nfpf + j + s->nfp * bf_ele[q];
+      s->bdy_type[q * s->nfpf + j] = bf_type[q];
+    }
+  int cerr = set_connectivity(s, c2v, xv, is_bdy);
+  free(is_bdy);
+  if (cerr != 0) { or_destroy(s); return NULL; }
   long nuN = (long)s->nu * n_eles, nfpN = (long)s->nfp * n_eles;
   int nf = s->nf, nd = s->nd;
   s->disu_upts[0] = xcalloc(nuN * nf, sizeof(double));
@@ -694,7 +729,7 @@ void or_destroy(or_solver* s) {
   free(s->disu_fpts); free(s->tdisf_upts); free(s->norm_tdisf_fpts); free(s->norm_tconf_fpts);
   free(s->delta_disu_fpts); free(s->grad_disu_upts); free(s->grad_disu_fpts);
   free(s->sgsf_upts); free(s->sgsf_fpts);
-  free(s->face_l); free(s->face_r);
+  free(s->face_l); free(s->face_r); free(s->bdy_fpt); free(s->bdy_type);
   free(s);
 }
 
@@ -913,6 +948,31 @@ static void ldg_flux(int nd, const double* ul, const double* ur, const double* f
   }
 }
 
+/* Ghost state of bdy_inters::set_inv_boundary_conditions (src/bdy_inters.cpp:391-
+ * 420, 668-697, 884-889) for bctype 11, isothermal no-slip wall on a static mesh
+ * (grid velocity zero): pressure extrapolated, T = T_wall, rho = p/(R_ref T). */
+static void set_inv_bc(int nd, int type, const or_phys* ph, const double* ul, double* ur) {
+  double gamma = ph->gamma, v_l[3], v_r[3] = {0., 0., 0.};
+  double rho_l = ul[0];
+  for (int i = 0; i < nd; i++) v_l[i] = ul[i + 1] / ul[0];
+  double e_l = ul[nd + 1];
+  double v_sq = 0.;
+  for (int i = 0; i < nd; i++) v_sq += (v_l[i] * v_l[i]);
+  double p_l = (gamma - 1.0) * (e_l - 0.5 * rho_l * v_sq);
+  double p_r = p_l, rho_r = 0., e_r = 0.;
+  if (type == 11) {
+    double T_r = ph->T_wall;
+    rho_r = p_r / (ph->R_ref * T_r);
+    for (int i = 0; i < nd; i++) v_r[i] = 0.0;
+    v_sq = 0.;
+    for (int i = 0; i < nd; i++) v_sq += (v_r[i] * v_r[i]);
+    e_r = (p_r / (gamma - 1.0)) + 0.5 * rho_r * v_sq;
+  }
+  ur[0] = rho_r;
+  for (int i = 0; i < nd; i++) ur[i + 1] = rho_r * v_r[i];
+  ur[nd + 1] = e_r;
+}
+
 /* ------------------------------------------------------------------------- */
 /* CalcResidual steps */
 
@@ -976,6 +1036,35 @@ static void common_invFlux(or_solver* s) {
         s->delta_disu_fpts[b + nfpN * k] = (uc - ur[k]);
       }
     }
+  }
+}
+
+/* bdy_inters::evaluate_boundaryConditions_invFlux, src/bdy_inters.cpp:245-389:
+ * central flux (convective_flux_boundary, src/inters.cpp:368-387) between the
+ * trace and the ghost state; LDG Dirichlet common solution u_c = (u_l + u_r)/2. */
+static void bdy_invFlux(or_solver* s) {
+  int nf = s->nf, nd = s->nd;
+  long nfpN = (long)s->nfp * s->n_eles;
+#pragma omp parallel for schedule(static)
+  for (int q = 0; q < s->n_bdy; q++) {
+    int a = s->bdy_fpt[q];
+    double ul[5], ur[5], fl[15], fr[15], n[3] = {0, 0, 0}, fn[5], fnl[5], fnr[5];
+    for (int m = 0; m < nd; m++) n[m] = s->norm_fpts[a + nfpN * m];
+    for (int k = 0; k < nf; k++) ul[k] = s->disu_fpts[a + nfpN * k];
+    set_inv_bc(nd, s->bdy_type[q], &s->ph, ul, ur);
+    calc_invf(nd, s->ph.gamma, ul, fl);
+    calc_invf(nd, s->ph.gamma, ur, fr);
+    for (int k = 0; k < nf; k++) {
+      fnl[k] = 0.; fnr[k] = 0.;
+      for (int l = 0; l < nd; l++) { fnl[k] += fl[k + nf * l] * n[l]; fnr[k] += fr[k + nf * l] * n[l]; }
+    }
+    for (int k = 0; k < nf; k++) fn[k] = 0.5 * (fnl[k] + fnr[k]);
+    for (int k = 0; k < nf; k++) s->norm_tconf_fpts[a + nfpN * k] = fn[k] * s->tdA_fpts[a];
+    if (s->deck.viscous)
+      for (int k = 0; k < nf; k++) {
+        double uc = 0.5 * (ur[k] + ul[k]);
+        s->delta_disu_fpts[a + nfpN * k] = (uc - ul[k]);
+      }
   }
 }
 
@@ -1184,6 +1273,36 @@ static void common_viscFlux(or_solver* s) {
   }
 }
 
+/* bdy_inters::evaluate_boundaryConditions_viscFlux, src/bdy_inters.cpp:903-1046,
+ * flux_spec 1 (Dirichlet, bctype 11): f_c = f_l + tau n (u_l - u_r), with the SGS
+ * traces added to f_l under LES; fn = f_c.n (src/inters.cpp:550-641). */
+static void bdy_viscFlux(or_solver* s) {
+  int nf = s->nf, nd = s->nd;
+  long nfpN = (long)s->nfp * s->n_eles;
+  double tau = s->ph.tau;
+#pragma omp parallel for schedule(static)
+  for (int q = 0; q < s->n_bdy; q++) {
+    int a = s->bdy_fpt[q];
+    double ul[5], ur[5], gl[15], fl[15], fc[15] = {0}, n[3] = {0, 0, 0}, fn[5];
+    for (int k = 0; k < nf; k++) ul[k] = s->disu_fpts[a + nfpN * k];
+    for (int m = 0; m < nd; m++) n[m] = s->norm_fpts[a + nfpN * m];
+    set_inv_bc(nd, s->bdy_type[q], &s->ph, ul, ur);
+    for (int m = 0; m < nd; m++)
+      for (int l = 0; l < nf; l++) gl[l + nf * m] = s->grad_disu_fpts[a + nfpN * (l + (long)nf * m)];
+    calc_visf(nd, &s->ph, ul, gl, fl);
+    if (s->deck.LES)
+      for (int m = 0; m < nd; m++)
+        for (int l = 0; l < nf; l++) fl[l + nf * m] += s->sgsf_fpts[a + nfpN * (l + (long)nf * m)];
+    for (int k = 0; k < nf; k++)
+      for (int m = 0; m < nd; m++) fc[k + nf * m] = fl[k + nf * m] + tau * n[m] * (ul[k] - ur[k]);
+    for (int k = 0; k < nf; k++) {
+      fn[k] = fc[k + nf * 0] * n[0];
+      for (int l = 1; l < nd; l++) fn[k] += fc[k + nf * l] * n[l];
+    }
+    for (int k = 0; k < nf; k++) s->norm_tconf_fpts[a + nfpN * k] += fn[k] * s->tdA_fpts[a];
+  }
+}
+
 /* src/eles.cpp:1594-1661 */
 static void calculate_corrected_divergence(or_solver* s) {
   long n = (long)s->n_eles * s->nf * s->nfp;
@@ -1200,6 +1319,7 @@ void or_residual(or_solver* s) {
   if (s->deck.viscous) calculate_gradient(s);
   evaluate_invFlux(s);
   common_invFlux(s);
+  bdy_invFlux(s);
   if (s->deck.viscous) {
     correct_gradient(s);
     extrapolate_corrected_gradient(s);
@@ -1208,7 +1328,10 @@ void or_residual(or_solver* s) {
   if (s->deck.LES) evaluate_sgsFlux(s);
   extrapolate_totalFlux(s);
   calculate_divergence(s);
-  if (s->deck.viscous) common_viscFlux(s);
+  if (s->deck.viscous) {
+    common_viscFlux(s);
+    bdy_viscFlux(s);
+  }
   calculate_corrected_divergence(s);
 }
 

@@ -70,6 +70,11 @@ void or_setup_params(const or_deck* d, or_phys* out);
  * Periodic faces are matched by translation over dx/dy/dz_cyclic of the deck. */
 or_solver* or_create(const or_deck* deck, int ele_type, int n_verts, const double* xv,
                      int n_eles, const int* c2v);
+/* As or_create, with boundary faces (element, local face, bctype 11 isothermal
+ * no-slip wall) taken out of the face matching. */
+or_solver* or_create_bc(const or_deck* deck, int ele_type, int n_verts, const double* xv, int n_eles,
+                        const int* c2v, int n_bfaces, const int* bf_ele, const int* bf_face,
+                        const int* bf_type);
 void or_destroy(or_solver* s);
 
 int or_n_upts(const or_solver* s);

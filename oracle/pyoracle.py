@@ -52,6 +52,8 @@ def lib():
         L.or_setup_params.argtypes = [C.POINTER(Deck), C.POINTER(Phys)]
         L.or_create.argtypes = [C.POINTER(Deck), C.c_int, C.c_int, dp, C.c_int, ip]
         L.or_create.restype = vp
+        L.or_create_bc.argtypes = [C.POINTER(Deck), C.c_int, C.c_int, dp, C.c_int, ip, C.c_int, ip, ip, ip]
+        L.or_create_bc.restype = vp
         L.or_destroy.argtypes = [vp]
         for f in ("or_n_upts", "or_n_fpts", "or_n_fields", "or_n_dims", "or_n_eles", "or_n_inters"):
             getattr(L, f).argtypes = [vp]
@@ -102,15 +104,20 @@ class Oracle:
             "opp_0": 12, "opp_3": 13, "opp_4": 14, "loc_upts": 15, "tloc_fpts": 16,
             "pos_fpts": 17, "sgsf_upts": 18, "sgsf_fpts": 19}
 
-    def __init__(self, deck, ele_type, xv, c2v, threads=None):
+    def __init__(self, deck, ele_type, xv, c2v, threads=None, bfaces=None):
+        """bfaces: optional (n, 3) int array of (element, local face, bctype) boundary faces."""
         L = lib()
         if threads is not None:
             L.or_set_threads(int(threads))
         self.deck = deck
         xv = np.ascontiguousarray(xv, dtype=np.float64)
         c2v = np.ascontiguousarray(c2v, dtype=np.int32)
-        self._h = L.or_create(C.byref(deck), int(ele_type), xv.shape[0], _dp(xv), c2v.shape[0],
-                              c2v.ctypes.data_as(C.POINTER(C.c_int)))
+        bf = np.zeros((0, 3), dtype=np.int32) if bfaces is None else np.asarray(bfaces, dtype=np.int32).reshape(-1, 3)
+        be, bfc, bt = (np.ascontiguousarray(bf[:, i]) for i in range(3))
+        ip = C.POINTER(C.c_int)
+        self._h = L.or_create_bc(C.byref(deck), int(ele_type), xv.shape[0], _dp(xv), c2v.shape[0],
+                                 c2v.ctypes.data_as(ip), bf.shape[0], be.ctypes.data_as(ip),
+                                 bfc.ctypes.data_as(ip), bt.ctypes.data_as(ip))
         if not self._h:
             raise RuntimeError("or_create failed")
         self.nu = L.or_n_upts(self._h)

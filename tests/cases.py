@@ -33,6 +33,33 @@ def mesh_of(g):
     return ele_type, g["xv"], frmesh.gambit_to_internal(ele_type, g["conn_gambit"])
 
 
+def bfaces_of(g):
+    """(n, 3) boundary faces (element, local face, bctype) of a fixture, or None."""
+    bf = g.get("bfaces")
+    return bf if bf is not None and len(bf) else None
+
+
+def phys_of(g):
+    """Physical deck parameters of a fixture (Mach, Re, T_wall may differ)."""
+    ph = dict(PHYS)
+    for key, name in (("mach", "Mach_free_stream"), ("Re", "Re_free_stream"), ("T_wall", "T_wall")):
+        if key in g:
+            ph[name] = float(g[key])
+    return ph
+
+
+def solver_of(g, **over):
+    import hifr
+    ele_type, xv, c2v = mesh_of(g)
+    return hifr.Solver.from_arrays(product_deck(g, **over), ele_type, xv, c2v, bfaces=bfaces_of(g))
+
+
+def oracle_of(g, threads=None, **over):
+    from oracle import pyoracle as po
+    ele_type, xv, c2v = mesh_of(g)
+    return po.Oracle(oracle_deck(g, **over), ele_type, xv, c2v, threads=threads, bfaces=bfaces_of(g))
+
+
 def product_deck(g, host_only=False, **over):
     import hifr
     ele_type = int(g["ele_type"])
@@ -41,7 +68,7 @@ def product_deck(g, host_only=False, **over):
               pen_fact=float(g["pen_fact"]), dt=float(g["dt"]),
               adv_type=3 if str(g["kind"]) == "rk45" else 0,
               dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2], fix_vis=1)
-    kw.update(PHYS)
+    kw.update(phys_of(g))
     if ele_type == 4:
         kw["vcjh_scheme_hexa"] = int(g["vcjh"])
     else:
@@ -63,7 +90,7 @@ def oracle_deck(g, **over):
               pen_fact=float(g["pen_fact"]), vcjh_scheme=int(g["vcjh"]), eta=0.0,
               dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2], LES=int(g.get("les", 0)),
               SGS_model=int(g.get("sgs_model", 0)), filter_ratio=float(g.get("filter_ratio", 1.0)))
-    kw.update(PHYS)
+    kw.update(phys_of(g))
     kw.update(over)
     return po.default_deck(**kw)
 

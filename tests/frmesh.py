@@ -142,6 +142,34 @@ def boundary_faces(ele_type, c2v_internal):
     return sorted(v[0] for v in seen.values() if len(v) == 1)
 
 
+def channel_state(pos, mach, u_bulk=1.0):
+    """Plane-channel field between walls at y = 0 and y = 2: parabolic
+    u = 1.5 U_b y (2 - y), rho = 1, p = 1/(gamma Mach^2) (non-dimensional units of
+    the reference with the bulk velocity as the free-stream velocity)."""
+    gamma = 1.4
+    y = pos[..., 1]
+    u = 1.5 * u_bulk * y * (2.0 - y)
+    nd = pos.shape[-1]
+    out = np.zeros(pos.shape[:-1] + (nd + 2,))
+    p = 1.0 / (gamma * mach ** 2)
+    out[..., 0] = 1.0
+    out[..., 1] = u
+    out[..., nd + 1] = p / (gamma - 1.0) + 0.5 * u * u
+    return out
+
+
+def wall_faces(ele_type, c2v_internal, xv, axis, values, tol=1e-9):
+    """(element, internal face) of the boundary faces whose vertices all lie on the
+    plane x_axis = v for one of `values`."""
+    fn = HEX_FACE_NODES if ele_type == 4 else QUAD_FACE_NODES
+    out = []
+    for e, f in boundary_faces(ele_type, c2v_internal):
+        coords = xv[[int(c2v_internal[e][i]) for i in fn[f]], axis]
+        if any(np.all(np.abs(coords - v) < tol) for v in values):
+            out.append((e, f))
+    return out
+
+
 def read_gambit(path):
     """Vertices and element connectivity (Gambit order, 0-based) of a neutral file."""
     with open(path) as fh:

@@ -48,8 +48,8 @@ def deck_text(c, restart_iter=0):
         "tau %.17g" % c.get("tau", 0.0), "pen_fact %.17g" % c.get("pen_fact", 0.5),
         "restart_flag 1", "restart_iter %d" % restart_iter, "n_restart_files 1",
         "mesh_file mesh.neu",
-        "dx_cyclic %.17g" % c["cyc"][0], "dy_cyclic %.17g" % c["cyc"][1],
-        "dz_cyclic %.17g" % c["cyc"][2],
+        "dx_cyclic %.17g" % min(c["cyc"][0], 1e30), "dy_cyclic %.17g" % min(c["cyc"][1], 1e30),
+        "dz_cyclic %.17g" % min(c["cyc"][2], 1e30),
         "p_res 2", "write_type 0", "inters_cub_order 3", "volume_cub_order 3",
         "monitor_res_freq 1", "plot_freq 1000000", "data_file_name golden",
         "restart_dump_freq %d" % c["n_steps"], "res_norm_type 1", "error_norm_type 1", "res_norm_field 0",
@@ -58,7 +58,8 @@ def deck_text(c, restart_iter=0):
         "gamma 1.4", "prandtl 0.72", "S_gas 120.", "T_gas 291.15", "R_gas 286.9", "mu_gas 1.827E-05",
         "fix_vis 1", "Mach_free_stream %.17g" % c.get("mach", 0.1), "nx_free_stream 1.",
         "ny_free_stream 0.", "nz_free_stream 0.", "Re_free_stream %.17g" % c.get("Re", 1600.0),
-        "L_free_stream 1.", "T_free_stream 300", "T_wall 300",
+        "L_free_stream 1.", "T_free_stream 300", "T_wall %.17g" % c.get("T_wall", 300.0),
+        "Mach_wall 0.", "nx_wall 0.", "ny_wall 0.", "nz_wall 0.",
         "rho_bound 1.", "u_bound 1.0", "v_bound 0.0", "w_bound 0.0", "p_bound 71.428571428571431",
     ]
     if c.get("les"):
@@ -72,7 +73,7 @@ def oracle_deck(c):
         equation=0, viscous=c["viscous"], order=c["order"], adv_type=c["adv_type"], dt=c["dt"],
         tau=c.get("tau", 0.0), pen_fact=c.get("pen_fact", 0.5), ic_form=c.get("ic_form", 1),
         gamma=1.4, prandtl=0.72, Mach_free_stream=c.get("mach", 0.1), nx_free_stream=1.0,
-        Re_free_stream=c.get("Re", 1600.0), L_free_stream=1.0, T_free_stream=300.0, T_wall=300.0,
+        Re_free_stream=c.get("Re", 1600.0), L_free_stream=1.0, T_free_stream=300.0, T_wall=c.get("T_wall", 300.0),
         vcjh_scheme=c["vcjh"], eta=0.0, dx_cyclic=c["cyc"][0], dy_cyclic=c["cyc"][1],
         dz_cyclic=c["cyc"][2], LES=int(c.get("les", 0)), SGS_model=c.get("sgs_model", 0),
         filter_ratio=c.get("filter_ratio", 1.0))
@@ -111,10 +112,14 @@ def run_reference(c, xv, conn_g, u0, loc1d):
     wd = tempfile.mkdtemp(prefix="golden_")
     try:
         c2v = frmesh.gambit_to_internal(c["ele_type"], conn_g)
-        bfaces = [(e, frmesh.internal_face_to_gambit(c["ele_type"], f))
-                  for e, f in frmesh.boundary_faces(c["ele_type"], c2v)]
-        frmesh.write_gambit(os.path.join(wd, "mesh.neu"), xv, conn_g, c["ele_type"],
-                            bc_groups=[("Cyclic", bfaces)])
+        walls = set(map(tuple, c.get("walls", [])))
+        cyc = [(e, frmesh.internal_face_to_gambit(c["ele_type"], f))
+               for e, f in frmesh.boundary_faces(c["ele_type"], c2v) if (e, f) not in walls]
+        groups = [("Cyclic", cyc)]
+        if walls:
+            groups.append(("isotherm_fix", [(e, frmesh.internal_face_to_gambit(c["ele_type"], f))
+                                            for e, f in sorted(walls)]))
+        frmesh.write_gambit(os.path.join(wd, "mesh.neu"), xv, conn_g, c["ele_type"], bc_groups=groups)
         with open(os.path.join(wd, "input"), "w") as fh:
             fh.write(deck_text(c))
         write_restart(os.path.join(wd, "Rest_000000000_p0000.dat"), c["ele_type"], loc1d, c["order"], u0)
@@ -144,6 +149,19 @@ CASES = [
          pen_fact=0.5, state="vortex", noise=1e-3, seed=3, perturb=0.1, kind="residual"),
     dict(name="hex_p3_ns_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=2, viscous=1, tau=1.0,
          pen_fact=0.5, state="tgv", noise=1e-3, seed=1234, perturb=0.0, kind="rk45", steps=3, dt=1e-3),
+    # isothermal no-slip walls at y = 0 and y = 2 (channel), periodic in x and z
+    dict(name="hex_p3_channel_walls_3x3x2", ele_type=4, n=(3, 3, 2), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=31, perturb=0.1, kind="residual", channel=True,
+         mach=0.2, Re=2800.0, T_wall=310.0),
+    dict(name="hex_p2_channel_walls_les_3x2x2", ele_type=4, n=(3, 2, 2), order=2, vcjh=2, viscous=1, tau=0.5,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=32, perturb=0.0, kind="residual", channel=True,
+         mach=0.2, Re=2800.0, les=1, sgs_model=1, filter_ratio=2.0),
+    dict(name="quad_p3_channel_walls_4x3", ele_type=1, n=(4, 3), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=33, perturb=0.1, kind="residual", channel=True,
+         mach=0.2, Re=2800.0, T_wall=290.0),
+    dict(name="hex_p3_channel_walls_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=34, perturb=0.0, kind="rk45", steps=3, dt=1e-3,
+         channel=True, mach=0.2, Re=2800.0),
     # LES: WALE / Smagorinsky eddy viscosity (reference CPU path)
     dict(name="hex_p3_les_wale_perturbed_3x3x3", ele_type=4, n=(3, 3, 3), order=3, vcjh=1, viscous=1, tau=1.0,
          pen_fact=0.5, state="tgv", noise=1e-2, seed=21, perturb=0.1, kind="residual", les=1, sgs_model=1,
@@ -161,6 +179,22 @@ CASES = [
 
 
 def build_case(c):
+    if c.get("channel"):
+        # [0, 4pi] x [0, 2] (x [0, 2pi]), walls at y = 0, 2
+        if c["ele_type"] == 4:
+            nx, ny, nz = c["n"]
+            xv, conn = frmesh.box_hex(nx, ny, nz, 2 * TWO_PI, 2.0, TWO_PI)
+            c["cyc"] = (2 * TWO_PI, float("inf"), TWO_PI)
+        else:
+            nx, ny = c["n"]
+            xv, conn = frmesh.box_quad(nx, ny, 2 * TWO_PI, 2.0)
+            c["cyc"] = (2 * TWO_PI, float("inf"), float("inf"))
+        h = 2.0 / c["n"][1]
+        if c["perturb"] > 0:
+            xv = frmesh.perturb_interior(xv, c["perturb"] * h, seed=c["seed"] + 100)
+        c2v = frmesh.gambit_to_internal(c["ele_type"], conn)
+        c["walls"] = frmesh.wall_faces(c["ele_type"], c2v, xv, 1, (0.0, 2.0))
+        return xv, conn
     if c["ele_type"] == 4:
         nx, ny, nz = c["n"]
         xv, conn = frmesh.box_hex(nx, ny, nz, TWO_PI, TWO_PI, TWO_PI)
@@ -189,10 +223,13 @@ def main():
             c.update(adv_type=0, dt=DT_BIG, n_steps=1)
         else:
             c.update(adv_type=3, n_steps=c["steps"])
-        o = po.Oracle(oracle_deck(c), c["ele_type"], xv, c2v)
+        bfaces = [(e, f, 11) for e, f in c.get("walls", [])]
+        o = po.Oracle(oracle_deck(c), c["ele_type"], xv, c2v, bfaces=bfaces or None)
         pos = o.get("pos_upts")
         if c["state"] == "tgv":
             u0 = frmesh.tgv_state(pos, p0=1.0 / (1.4 * 0.1 ** 2))
+        elif c["state"] == "channel":
+            u0 = frmesh.channel_state(pos, c["mach"])
         else:
             u0 = frmesh.shu_vortex_state(pos)
         if c["noise"] > 0:
@@ -204,7 +241,9 @@ def main():
                    vcjh=c["vcjh"], viscous=c["viscous"], tau=c.get("tau", 0.0),
                    pen_fact=c.get("pen_fact", 0.5), cyc=np.array(c["cyc"]), dt=c["dt"],
                    u0=u0, kind=c["kind"], les=int(c.get("les", 0)), sgs_model=int(c.get("sgs_model", 0)),
-                   filter_ratio=float(c.get("filter_ratio", 1.0)))
+                   filter_ratio=float(c.get("filter_ratio", 1.0)),
+                   bfaces=np.array(bfaces, dtype=np.int32).reshape(-1, 3), mach=c.get("mach", 0.1),
+                   Re=c.get("Re", 1600.0), T_wall=c.get("T_wall", 300.0))
         if c["kind"] == "residual":
             R = (u0 - u1) / c["dt"]
             rec["R"] = R

@@ -60,12 +60,11 @@ def _kinetic_energy(u, detj, w3):
 
 
 @pytest.mark.parametrize("name", ["hex_p3_ns_tgv_2x2x2", "hex_p3_ns_perturbed_3x3x3", "hex_p4_ns_2x2x2",
-                                  "hex_p3_les_wale_perturbed_3x3x3"])
+                                  "hex_p3_les_wale_perturbed_3x3x3", "hex_p3_channel_walls_3x3x2"])
 def test_mixed_residual_close_to_oracle(require_gpu, name):
     g = cases.load_golden(name)
-    ele_type, xv, c2v = cases.mesh_of(g)
-    s = hifr.Solver.from_arrays(cases.product_deck(g, hifr_precision=1), ele_type, xv, c2v)
-    o = po.Oracle(cases.oracle_deck(g), ele_type, xv, c2v)
+    s = cases.solver_of(g, hifr_precision=1)
+    o = cases.oracle_of(g)
     s.set_state(g["u0"])
     o.set_state(g["u0"])
     o.residual()

@@ -22,10 +22,7 @@ TOL = 1e-12
 
 
 def _pair(g, **over):
-    ele_type, xv, c2v = cases.mesh_of(g)
-    s = hifr.Solver.from_arrays(cases.product_deck(g, **over), ele_type, xv, c2v)
-    o = po.Oracle(cases.oracle_deck(g), ele_type, xv, c2v)
-    return s, o
+    return cases.solver_of(g, **over), cases.oracle_of(g)
 
 
 @pytest.mark.parametrize("metrics", [0, 1], ids=["auto", "per_point"])

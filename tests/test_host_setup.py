@@ -18,9 +18,8 @@ ROOT = os.path.dirname(os.path.dirname(cases.GOLDEN))
 @pytest.mark.parametrize("name", cases.golden_names())
 def test_host_setup_matches_oracle(name):
     g = cases.load_golden(name)
-    ele_type, xv, c2v = cases.mesh_of(g)
-    s = hifr.Solver.from_arrays(cases.product_deck(g, host_only=True), ele_type, xv, c2v)
-    o = po.Oracle(cases.oracle_deck(g), ele_type, xv, c2v)
+    s = cases.solver_of(g, host_only=True)
+    o = cases.oracle_of(g)
     for what in ("opp_0", "opp_3", "opp_4", "detjac_upts", "JGinv_upts", "tdA_fpts", "norm_fpts", "pos_upts"):
         a, b = s.get(what), o.get(what)
         assert a.shape == b.shape, what

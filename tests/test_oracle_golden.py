@@ -22,8 +22,7 @@ from oracle import pyoracle as po
 
 
 def _oracle(g, **over):
-    ele_type, xv, c2v = cases.mesh_of(g)
-    return po.Oracle(cases.oracle_deck(g, **over), ele_type, xv, c2v)
+    return cases.oracle_of(g, **over)
 
 
 @pytest.mark.parametrize("name", cases.golden_names("residual"))

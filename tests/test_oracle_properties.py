@@ -246,3 +246,62 @@ def test_viscous_terms_manufactured(ele_type, case):
     e2, r2 = _manufactured_viscous(ele_type, 8, case)
     assert e2 < 0.08 and e1 / e2 > 2.5, (e1, e2)
     assert r2 == 0.0
+
+
+def _channel_oracle(ele_type, order, T_wall, perturb=0.1):
+    """Channel [0, 4pi] x [0, 2] (x [0, 2pi]) with isothermal walls at y = 0, 2."""
+    if ele_type == 4:
+        xv, conn = frmesh.box_hex(3, 2, 2, 2 * TWO_PI, 2.0, TWO_PI)
+        cyc = dict(dx_cyclic=2 * TWO_PI, dy_cyclic=float("inf"), dz_cyclic=TWO_PI)
+    else:
+        xv, conn = frmesh.box_quad(4, 3, 2 * TWO_PI, 2.0)
+        cyc = dict(dx_cyclic=2 * TWO_PI, dy_cyclic=float("inf"), dz_cyclic=float("inf"))
+    xv = frmesh.perturb_interior(xv, perturb * 0.5, seed=5)
+    c2v = frmesh.gambit_to_internal(ele_type, conn)
+    walls = frmesh.wall_faces(ele_type, c2v, xv, 1, (0.0, 2.0))
+    d = dict(equation=0, viscous=1, order=order, adv_type=0, dt=1e-3, tau=1.0, pen_fact=0.5, vcjh_scheme=1,
+             eta=0.0, **cyc)
+    d.update(cases.PHYS)
+    d.update(Mach_free_stream=0.2, T_wall=T_wall)
+    return po.Oracle(po.default_deck(**d), ele_type, xv, c2v, bfaces=[(e, f, 11) for e, f in walls])
+
+
+def _quiescent(o, T):
+    """Gas at rest, rho = 1, temperature T (non-dimensional): p = rho R_ref T."""
+    ph = o.phys()
+    u = np.zeros((o.nu, o.n_eles, o.nf))
+    u[..., 0] = 1.0
+    u[..., o.nf - 1] = ph.R_ref * T / (ph.gamma - 1.0)
+    return u
+
+
+@pytest.mark.parametrize("ele_type", [1, 4])
+def test_wall_keeps_isothermal_gas_at_rest(ele_type):
+    # gas at rest at the wall temperature: ghost state = trace, no heat flux,
+    # so the residual vanishes to round-off (on a perturbed mesh)
+    o = _channel_oracle(ele_type, 3, T_wall=300.0)
+    u = _quiescent(o, o.phys().T_wall)
+    o.set_state(u)
+    o.residual()
+    R = o.get("div_tconf_upts")
+    assert np.abs(R).max() < 1e-10 * np.abs(u).max(), np.abs(R).max()
+
+
+@pytest.mark.parametrize("ele_type", [1, 4])
+def test_hot_wall_heats_the_gas(ele_type):
+    # walls 10% hotter than the gas at rest (T_wall = 1.1): heat flows in
+    # (dE/dt = -div/J > 0 next to the walls), the momentum stays at rest (equal
+    # pressure on both sides of the wall), and the LDG penalty tau (u_l - u_r) on
+    # the density -- the ghost density p/(R_ref T_wall) is lower -- draws mass out
+    # of the wall cells, as the reference's Dirichlet LDG flux does for every field
+    o = _channel_oracle(ele_type, 3, T_wall=330.0)
+    u = _quiescent(o, 1.0)
+    o.set_state(u)
+    o.residual()
+    dudt = -o.get("div_tconf_upts") / o.get("detjac_upts")[:, :, None]
+    y = o.get("pos_upts")[..., 1]
+    near = (y < 0.3) | (y > 1.7)
+    e = dudt[..., o.nf - 1]
+    assert e[near].mean() > 0 and e[near].mean() > 10 * abs(e[~near].mean())
+    assert np.abs(dudt[..., 1:o.nf - 1]).max() < 1e-10 * np.abs(e).max()
+    assert dudt[..., 0][near].mean() < 0
