@@ -24,6 +24,7 @@
  *       own regression test (testcases/regression_tests.py:300-307).
  */
 #include "fr_oracle.h"
+#include "tet_points.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -169,6 +170,7 @@ struct or_solver {
       *norm_tconf_fpts, *delta_disu_fpts, *grad_disu_upts, *grad_disu_fpts, *src_upts;
   double *sgsf_upts, *sgsf_fpts; /* LES: transformed SGS flux (Nu,N,nf,d), (Nfp,N,nf,d) */
   int n_inters;
+  double* inv_vdm;           /* tets: inverse Vandermonde of the Dubiner basis (Nu,Nu) */
   int n_bdy;                 /* boundary flux points */
   int *bdy_fpt, *bdy_type;   /* (n_bdy) flux-point index fpt + Nfp*ele, bctype */
   int *face_l, *face_r; /* (n_inters*nfpf) flux-point indices fpt + Nfp*ele */
@@ -238,13 +240,239 @@ void or_setup_params(const or_deck* d, or_phys* o) {
 }
 
 /* ------------------------------------------------------------------------- */
+/* Tetrahedra: orthonormal (Dubiner) polynomial bases on the reference simplex
+ * (r,s,t >= -1, r+s+t <= -1), src/funcs.cpp:1015-1480 */
+
+/* (n-1)! for n >= 1, src/funcs.cpp:1096-1114 */
+static double gamma_int(int n) {
+  double g = 1.0;
+  for (int i = 2; i < n; i++) g *= i;
+  return g;
+}
+
+/* Jacobi polynomial P_n^(a,b)(x), normalised to unit norm on [-1,1] with weight
+ * (1-x)^a (1+x)^b: the reference's three-term recursion with its operation order,
+ * src/funcs.cpp:1117-1170 */
+static double jacobi_n(double x, int a, int b, int n) {
+  if (n == 0) {
+    double t0 = pow(2.0, (-a - b - 1)), t1 = gamma_int(a + b + 2), t2 = gamma_int(a + 1) * gamma_int(b + 1);
+    return sqrt(t0 * (t1 / t2));
+  }
+  if (n == 1) {
+    double t0 = pow(2.0, (-a - b - 1)), t1 = gamma_int(a + b + 2), t2 = gamma_int(a + 1) * gamma_int(b + 1);
+    double t3 = a + b + 3, t4 = (a + 1) * (b + 1), t5 = (x * (a + b + 2) + (a - b));
+    return 0.5 * sqrt(t0 * (t1 / t2)) * sqrt(t3 / t4) * t5;
+  }
+  double t0 = n * (n + a + b) * (n + a) * (n + b);
+  double t1 = ((2 * n) + a + b - 1) * ((2 * n) + a + b + 1);
+  double t3 = (2 * n) + a + b;
+  double t4 = (n - 1) * ((n - 1) + a + b) * ((n - 1) + a) * ((n - 1) + b);
+  double t5 = ((2 * (n - 1)) + a + b - 1) * ((2 * (n - 1)) + a + b + 1);
+  double t6 = (2 * (n - 1)) + a + b;
+  double t7 = -((a * a) - (b * b));
+  double t8 = ((2 * (n - 1)) + a + b) * ((2 * (n - 1)) + a + b + 2);
+  double t9 = (2.0 / t3) * sqrt(t0 / t1);
+  double t10 = (2.0 / t6) * sqrt(t4 / t5);
+  double t11 = t7 / t8;
+  double t12 = x * jacobi_n(x, a, b, n - 1);
+  double t13 = t10 * jacobi_n(x, a, b, n - 2);
+  double t14 = t11 * jacobi_n(x, a, b, n - 1);
+  return (1.0 / t9) * (t12 - t13 - t14);
+}
+
+/* src/funcs.cpp:1174-1188 */
+static double grad_jacobi_n(double x, int a, int b, int n) {
+  return n == 0 ? 0.0 : sqrt(1.0 * n * (n + a + b + 1)) * jacobi_n(x, a + 1, b + 1, n - 1);
+}
+
+/* collapsed coordinates, src/funcs.cpp:1015-1031, 1067-1093 */
+static void rs_to_ab(double r, double s, double* ab) {
+  ab[0] = s == 1.0 ? -1.0 : 2.0 * ((1.0 + r) / (1.0 - s)) - 1.0;
+  ab[1] = s;
+}
+static void rst_to_abc(double r, double s, double t, double* abc) {
+  abc[0] = (s + t == 0.0) ? -1.0 : -2.0 * (1.0 + r) / (s + t) - 1.0;
+  abc[1] = (t == 1.0) ? -1.0 : 2.0 * (1.0 + s) / (1.0 - t) - 1.0;
+  abc[2] = t;
+}
+
+/* mode -> (i,j) of the triangle basis in the reference's order (k = i+j outer,
+ * j inner); (i,j,k) of the tetrahedron basis (m = i+j+k, n = j+k, k inner) */
+static void tri_mode(int mode, int* pi, int* pj) {
+  int q = 0;
+  for (int k = 0;; k++)
+    for (int j = 0; j <= k; j++, q++)
+      if (q == mode) { *pi = k - j; *pj = j; return; }
+}
+static void tet_mode(int mode, int* pi, int* pj, int* pk) {
+  int q = 0;
+  for (int m = 0;; m++)
+    for (int n = 0; n <= m; n++)
+      for (int k = 0; k <= n; k++, q++)
+        if (q == mode) { *pk = k; *pj = n - k; *pi = m - n; return; }
+}
+
+/* orthonormal triangle basis, src/funcs.cpp:1190-1226 */
+static double dubiner_2d(double r, double s, int mode) {
+  int i, j;
+  double ab[2];
+  tri_mode(mode, &i, &j);
+  rs_to_ab(r, s, ab);
+  return sqrt(2.0) * jacobi_n(ab[0], 0, 0, i) * jacobi_n(ab[1], 2 * i + 1, 0, j) * pow(1.0 - ab[1], i);
+}
+
+/* orthonormal tetrahedron basis, src/funcs.cpp:1333-1378 */
+static double dubiner_3d(double r, double s, double t, int mode) {
+  int i, j, k;
+  double c[3];
+  tet_mode(mode, &i, &j, &k);
+  rst_to_abc(r, s, t, c);
+  return 2.0 * sqrt(2.0) * jacobi_n(c[0], 0, 0, i) * jacobi_n(c[1], 2 * i + 1, 0, j) *
+         jacobi_n(c[2], 2 * i + 2 * j + 2, 0, k) * pow(1.0 - c[1], i) * pow(1.0 - c[2], i + j);
+}
+
+/* gradient of the tetrahedron basis by the chain rule through the collapsed
+ * coordinates, src/funcs.cpp:1381-1480 */
+static double grad_dubiner_3d(double r, double s, double t, int mode, int comp) {
+  int i, j, k;
+  double c[3];
+  tet_mode(mode, &i, &j, &k);
+  rst_to_abc(r, s, t, c);
+  double fa = jacobi_n(c[0], 0, 0, i), gb = jacobi_n(c[1], 2 * i + 1, 0, j), hc = jacobi_n(c[2], 2 * (i + j) + 2, 0, k);
+  double dfa = grad_jacobi_n(c[0], 0, 0, i), dgb = grad_jacobi_n(c[1], 2 * i + 1, 0, j),
+         dhc = grad_jacobi_n(c[2], 2 * (i + j) + 2, 0, k);
+  double hb = 0.5 * (1.0 - c[1]), hcc = 0.5 * (1.0 - c[2]);
+  double dr = dfa * gb * hc;
+  if (i > 0) dr *= pow(hb, i - 1);
+  if (i + j > 0) dr *= pow(hcc, i + j - 1);
+  double scale = pow(2.0, 2 * i + j + 1.5);
+  if (comp == 0) return dr * scale;
+  double ds = 0.5 * (1.0 + c[0]) * dr;
+  double tmp = dgb * pow(hb, i);
+  if (i > 0) tmp += (-0.5 * i) * gb * pow(hb, i - 1);
+  if (i + j > 0) tmp *= pow(hcc, i + j - 1);
+  tmp = fa * tmp * hc;
+  ds += tmp;
+  if (comp == 1) return ds * scale;
+  double dt = 0.5 * (1.0 + c[0]) * dr + 0.5 * (1.0 + c[1]) * tmp;
+  tmp = dhc * pow(hcc, i + j);
+  if (i + j > 0) tmp -= 0.5 * (i + j) * (hc * pow(hcc, i + j - 1));
+  tmp = fa * (gb * tmp) * pow(hb, i);
+  dt += tmp;
+  return dt * scale;
+}
+
+/* In-place inverse of an n x n column-major matrix: Gaussian elimination with full
+ * pivoting applied to the identity, back substitution and the undoing of the
+ * column swaps, in the order of inv_array (src/funcs.cpp:2719-2845). */
+static void invert(double* A, int n) {
+#define IN(i, j) A[(i) + n * (j)]
+#define ID(i, j) I[(i) + n * (j)]
+#define INV(i, j) X[(i) + n * (j)]
+  double* I = xcalloc((size_t)n * n, sizeof(double));
+  double* X = xcalloc((size_t)n * n, sizeof(double));
+  int* sw0 = xcalloc((size_t)n, sizeof(int));
+  int* sw1 = xcalloc((size_t)n, sizeof(int));
+  for (int i = 0; i < n; i++) { sw0[i] = i; sw1[i] = i; ID(i, i) = 1.0; }
+  for (int k = 0; k < n - 1; k++) {
+    double mx = 0;
+    int pi = k, pj = k;
+    for (int i = k; i < n; i++)
+      for (int j = k; j < n; j++) {
+        double mag = IN(i, j) * IN(i, j);
+        if (mag > mx) { pi = i; pj = j; mx = mag; }
+      }
+    int t = sw0[k]; sw0[k] = sw0[pi]; sw0[pi] = t;
+    t = sw1[k]; sw1[k] = sw1[pj]; sw1[pj] = t;
+    for (int i = 0; i < n; i++) { double a = IN(i, pj); IN(i, pj) = IN(i, k); IN(i, k) = a; }
+    for (int j = 0; j < n; j++) {
+      double a = IN(pi, j); IN(pi, j) = IN(k, j); IN(k, j) = a;
+      a = ID(pi, j); ID(pi, j) = ID(k, j); ID(k, j) = a;
+    }
+    for (int i = k + 1; i < n; i++) {
+      double first = IN(i, k);
+      for (int j = 0; j < n; j++) {
+        if (j >= k) IN(i, j) = IN(i, j) - ((first / IN(k, k)) * IN(k, j));
+        ID(i, j) = ID(i, j) - ((first / IN(k, k)) * ID(k, j));
+      }
+    }
+    for (int j = 0; j < k + 1; j++)
+      for (int i = j + 1; i < n; i++) IN(i, j) = 0.0;
+  }
+  for (int i = n - 1; i >= 0; i = i - 1)
+    for (int j = 0; j < n; j++) {
+      double d = 0.0;
+      for (int k = i + 1; k < n; k++) d = d + (IN(i, k) * INV(k, j));
+      INV(i, j) = (ID(i, j) - d) / IN(i, i);
+    }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) A[sw1[i] + n * j] = INV(i, j);
+#undef IN
+#undef ID
+#undef INV
+  free(I); free(X); free(sw0); free(sw1);
+}
+
+/* DG correction field of a tetrahedron flux point (eval_div_dg_tet,
+ * src/eles_tets.cpp:1018-1131): the orthonormal projection onto P_P(tet) of the
+ * face Lagrange polynomial of that flux point, integrated over its face with the
+ * reference's 36-point triangle rule (face Jacobian sqrt(3) on the slanted face). */
+static void tet_dg_lift(or_solver* s, double* opp3 /* (nu,nfp) */) {
+  int nu = s->nu, nfpf = s->nfpf, nfp = s->nfp;
+  double* Mf = xcalloc((size_t)nfpf * nfpf, sizeof(double));
+  double* coef = xcalloc((size_t)nu, sizeof(double));
+  for (int fpt = 0; fpt < nfp; fpt++) {
+    int face = fpt / nfpf, ffp = fpt - face * nfpf;
+    /* face coordinates of the face's flux points and the face Lagrange coefficients */
+    for (int i = 0; i < nfpf; i++) {
+      const double* t = s->tloc_fpts + 3 * (face * nfpf + i);
+      double rf, sf;
+      if (face == 0) { rf = t[0]; sf = t[2]; }
+      else if (face == 1) { rf = t[2]; sf = t[1]; }
+      else if (face == 2) { rf = t[0]; sf = t[2]; }
+      else { rf = t[1]; sf = t[0]; }
+      for (int j = 0; j < nfpf; j++) Mf[i + nfpf * j] = dubiner_2d(rf, sf, j);
+    }
+    invert(Mf, nfpf);  /* column ffp of the inverse: coefficients of the Lagrange polynomial */
+    double face_jac = face == 0 ? sqrt(3.) : 1.0;
+    for (int i = 0; i < nu; i++) {
+      double integral = 0.;
+      for (int q = 0; q < 36; q++) {
+        double rf = tri_cub12[q][0], sf = tri_cub12[q][1];
+        double r, ss, t;
+        if (face == 0) { r = rf; t = sf; ss = -1. - t - r; }
+        else if (face == 1) { r = -1.0; ss = sf; t = rf; }
+        else if (face == 2) { r = rf; ss = -1.0; t = sf; }
+        else { r = sf; ss = rf; t = -1.0; }
+        double g = 0.;
+        for (int k = 0; k < nfpf; k++) g += Mf[k + nfpf * ffp] * dubiner_2d(rf, sf, k);
+        integral += tri_cub12[q][2] * dubiner_3d(r, ss, t, i) * g;
+      }
+      coef[i] = integral * face_jac;
+    }
+    for (int j = 0; j < nu; j++) {
+      const double* x = s->loc_upts + 3 * j;
+      double v = 0.;
+      for (int i = 0; i < nu; i++) v += coef[i] * dubiner_3d(x[0], x[1], x[2], i);
+      opp3[j + nu * fpt] = v;
+    }
+  }
+  free(Mf); free(coef);
+}
+
+/* ------------------------------------------------------------------------- */
 /* Element-type specific point sets and bases */
 
 static int idx3(int n, int k, int j, int i) { return k + n * j + n * n * i; }
 
-/* src/eles_hexas.cpp:917-928 / src/eles_quads.cpp:855-866 */
+/* src/eles_hexas.cpp:917-928 / src/eles_quads.cpp:855-866 / src/eles_tets.cpp:805-822 */
 static double nodal_basis(const or_solver* s, int idx, const double* loc) {
   int n = s->order + 1;
+  if (s->ele_type == 2) {
+    double v = 0.0;
+    for (int i = 0; i < s->nu; i++) v += s->inv_vdm[i + s->nu * idx] * dubiner_3d(loc[0], loc[1], loc[2], i);
+    return v;
+  }
   if (s->ele_type == 4) {
     int i = idx / (n * n), j = (idx - n * n * i) / n, k = idx - n * j - n * n * i;
     return eval_lagrange(loc[0], k, s->loc1d, n) * eval_lagrange(loc[1], j, s->loc1d, n) *
@@ -259,6 +487,11 @@ static double nodal_basis(const or_solver* s, int idx, const double* loc) {
 static double d_nodal_basis(const or_solver* s, int idx, int c, const double* loc) {
   int n = s->order + 1;
   const double* p = s->loc1d;
+  if (s->ele_type == 2) {  /* src/eles_tets.cpp:843-860 */
+    double v = 0.0;
+    for (int i = 0; i < s->nu; i++) v += s->inv_vdm[i + s->nu * idx] * grad_dubiner_3d(loc[0], loc[1], loc[2], i, c);
+    return v;
+  }
   if (s->ele_type == 4) {
     int i = idx / (n * n), j = (idx - n * n * i) / n, k = idx - n * j - n * n * i;
     if (c == 0) return eval_d_lagrange(loc[0], k, p, n) * eval_lagrange(loc[1], j, p, n) * eval_lagrange(loc[2], i, p, n);
@@ -304,7 +537,44 @@ static void set_points(or_solver* s) {
   s->loc_upts = xcalloc((size_t)d * s->nu, sizeof(double));
   s->tloc_fpts = xcalloc((size_t)d * s->nfp, sizeof(double));
   s->tnorm_fpts = xcalloc((size_t)d * s->nfp, sizeof(double));
-  if (s->ele_type == 4) {
+  if (s->ele_type == 2) {
+    /* src/eles_tets.cpp:233-331 (interior points), 589-620 (normals) */
+    const double* up = tet_pts_tab[P];
+    const double* tp = tri_pts_tab[P];
+    for (int u = 0; u < s->nu; u++)
+      for (int k = 0; k < 3; k++) s->loc_upts[k + 3 * u] = up[3 * u + k];
+    int nf1 = s->nfpf;
+    for (int j = 0; j < P + 1; j++)
+      for (int i = 0; i < P + 1 - j; i++) {
+        int itri = j * (P + 1) - (j - 1) * j / 2 + i;
+        int ialpha = j * (P + 1) - (j - 1) * j / 2 + (P - j - i);
+        double* t0 = s->tloc_fpts + 3 * itri;
+        t0[0] = tp[2 * ialpha + 0];
+        ialpha = itri;
+        t0[1] = tp[2 * ialpha + 0];
+        t0[2] = tp[2 * ialpha + 1];
+        double* t1 = s->tloc_fpts + 3 * (nf1 + itri);
+        t1[0] = -1; t1[1] = tp[2 * ialpha + 1]; t1[2] = tp[2 * ialpha + 0];
+        double* t2 = s->tloc_fpts + 3 * (2 * nf1 + itri);
+        t2[0] = tp[2 * ialpha + 0]; t2[1] = -1; t2[2] = tp[2 * ialpha + 1];
+        double* t3 = s->tloc_fpts + 3 * (3 * nf1 + itri);
+        t3[0] = tp[2 * ialpha + 1]; t3[1] = tp[2 * ialpha + 0]; t3[2] = -1;
+      }
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < nf1; j++) {
+        double* tn = s->tnorm_fpts + 3 * (nf1 * i + j);
+        tn[0] = tn[1] = tn[2] = 0.0;
+        if (i == 0) tn[0] = tn[1] = tn[2] = 1. / sqrt(3.);
+        else tn[i - 1] = -1.0;
+      }
+    /* Vandermonde of the orthonormal basis and its inverse (set_vandermonde,
+       src/eles_tets.cpp:722-733) */
+    s->inv_vdm = xcalloc((size_t)s->nu * s->nu, sizeof(double));
+    for (int i = 0; i < s->nu; i++)
+      for (int j = 0; j < s->nu; j++)
+        s->inv_vdm[i + s->nu * j] = dubiner_3d(s->loc_upts[3 * i], s->loc_upts[3 * i + 1], s->loc_upts[3 * i + 2], j);
+    invert(s->inv_vdm, s->nu);
+  } else if (s->ele_type == 4) {
     for (int i = 0; i < n; i++)
       for (int j = 0; j < n; j++)
         for (int k = 0; k < n; k++) {
@@ -372,10 +642,15 @@ static void set_operators(or_solver* s) {
         s->opp_4[dim][k + nu * j] = v;
       }
   }
-  /* fill_opp_3: src/eles_hexas.cpp:1148-1162 */
-  for (int i = 0; i < nfp; i++)
-    for (int j = 0; j < nu; j++)
-      s->opp_3[j + nu * i] = div_vcjh_basis(s, i, s->loc_upts + d * j);
+  /* fill_opp_3: src/eles_hexas.cpp:1148-1162; tets: Filt * opp_3_dg with the DG
+     filter Filt = I (c_tet = 0), src/eles_tets.cpp:972-1016 */
+  if (s->ele_type == 2) {
+    tet_dg_lift(s, s->opp_3);
+  } else {
+    for (int i = 0; i < nfp; i++)
+      for (int j = 0; j < nu; j++)
+        s->opp_3[j + nu * i] = div_vcjh_basis(s, i, s->loc_upts + d * j);
+  }
   for (int dim = 0; dim < d; dim++)
     for (int j = 0; j < nfp; j++)
       for (int k = 0; k < nu; k++)
@@ -388,6 +663,19 @@ static void set_operators(or_solver* s) {
 static void shape_basis(const or_solver* s, const double* loc, double* N, double* dN /* (nsp,d) */) {
   double sp[2];
   for (int i = 0; i < 2; i++) sp[i] = -1.0 + ((2.0 * i) / (1.0 * (2 - 1)));
+  if (s->ele_type == 2) {  /* linear tet, src/eles_tets.cpp:862-880, 909-935 */
+    if (N) {
+      N[0] = -0.5 * (loc[0] + loc[1] + loc[2] + 1.);
+      N[1] = 0.5 * (loc[0] + 1.);
+      N[2] = 0.5 * (loc[1] + 1.);
+      N[3] = 0.5 * (loc[2] + 1.);
+    }
+    if (dN) {
+      for (int m = 0; m < 12; m++) dN[m] = 0.0;
+      for (int k = 0; k < 3; k++) { dN[0 + 4 * k] = -0.5; dN[k + 1 + 4 * k] = 0.5; }
+    }
+    return;
+  }
   if (s->ele_type == 4) {
     for (int m = 0; m < 8; m++) {
       int i = m / 4, j = (m - 4 * i) / 2, k = m - 2 * j - 4 * i;
@@ -498,6 +786,15 @@ static void set_transforms(or_solver* s) {
 static const int hex_face_nodes[6][4] = {{0, 1, 2, 3}, {0, 1, 4, 5}, {1, 3, 5, 7},
                                          {2, 3, 6, 7}, {0, 2, 4, 6}, {4, 5, 6, 7}};
 static const int quad_face_nodes[4][2] = {{0, 1}, {1, 3}, {2, 3}, {0, 2}};
+/* src/geometry.cpp:3156-3185 (linear tet) */
+static const int tet_face_nodes[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+
+static int face_node(const or_solver* s, int f, int k) {
+  if (s->ele_type == 4) return hex_face_nodes[f][k];
+  if (s->ele_type == 2) return tet_face_nodes[f][k];
+  return quad_face_nodes[f][k];
+}
+static int face_nv(const or_solver* s) { return s->ele_type == 4 ? 4 : (s->ele_type == 2 ? 3 : 2); }
 
 typedef struct { int key[4]; int e, f; } face_rec;
 
@@ -516,11 +813,11 @@ static void sort4(int* k, int n) {
 }
 
 static void face_centroid(const or_solver* s, const int* c2v, const double* xv, int e, int f, double* c) {
-  int nv = s->ele_type == 4 ? 4 : 2;
+  int nv = face_nv(s);
   int nsp = s->nsp;
   for (int m = 0; m < s->nd; m++) c[m] = 0.0;
   for (int k = 0; k < nv; k++) {
-    int node = s->ele_type == 4 ? hex_face_nodes[f][k] : quad_face_nodes[f][k];
+    int node = face_node(s, f, k);
     int v = c2v[node + nsp * e];
     for (int m = 0; m < s->nd; m++) c[m] += xv[m + s->nd * v] / nv;
   }
@@ -551,7 +848,7 @@ static void pair_points(or_solver* s, int eL, int fL, int eR, int fR, const doub
 
 /* is_bdy: (N*nfaces) 1 for faces on a physical boundary (not matched) */
 static int set_connectivity(or_solver* s, const int* c2v, const double* xv, const char* is_bdy) {
-  int N = s->n_eles, nfaces = s->nfaces, nv = s->ele_type == 4 ? 4 : 2;
+  int N = s->n_eles, nfaces = s->nfaces, nv = face_nv(s);
   long nrec = 0;
   for (long i = 0; i < (long)N * nfaces; i++) nrec += !is_bdy[i];
   face_rec* r = xcalloc(nrec, sizeof(face_rec));
@@ -563,7 +860,7 @@ static int set_connectivity(or_solver* s, const int* c2v, const double* xv, cons
       q->e = e; q->f = f;
       for (int k = 0; k < 4; k++) q->key[k] = -1;
       for (int k = 0; k < nv; k++) {
-        int node = s->ele_type == 4 ? hex_face_nodes[f][k] : quad_face_nodes[f][k];
+        int node = face_node(s, f, k);
         q->key[k] = c2v[node + s->nsp * e];
       }
       sort4(q->key, nv);
@@ -647,9 +944,12 @@ or_solver* or_create(const or_deck* deck, int ele_type, int n_verts, const doubl
 or_solver* or_create_bc(const or_deck* deck, int ele_type, int n_verts, const double* xv, int n_eles,
                         const int* c2v, int n_bfaces, const int* bf_ele, const int* bf_face,
                         const int* bf_type) {
-  if (ele_type != 1 && ele_type != 4) { fprintf(stderr, "fr_oracle: ele_type %d\n", ele_type); return NULL; }
+  if (ele_type != 1 && ele_type != 2 && ele_type != 4) { fprintf(stderr, "fr_oracle: ele_type %d\n", ele_type); return NULL; }
   if (deck->order < 0 || deck->order > 6 || deck->upts_type != 0 || deck->equation != 0 ||
-      (deck->LES && (!deck->viscous || deck->SGS_model < 0 || deck->SGS_model > 1))) {
+      (deck->LES && (!deck->viscous || deck->SGS_model < 0 || deck->SGS_model > 1)) ||
+      (ele_type == 2 && (deck->order < 1 || deck->order > 5 || !(deck->vcjh_scheme == 1 ||
+                                                                 (deck->vcjh_scheme == 0 && deck->eta == 0.0))))) {
+    /* tets: orders 1-5 of the tabulated point sets, DG correction (c_tet = 0) */
     fprintf(stderr, "fr_oracle: unsupported deck\n");
     return NULL;
   }
@@ -659,13 +959,20 @@ or_solver* or_create_bc(const or_deck* deck, int ele_type, int n_verts, const do
   s->ele_type = ele_type;
   s->order = deck->order;
   int n = s->order + 1;
-  s->nd = ele_type == 4 ? 3 : 2;
+  s->nd = ele_type == 1 ? 2 : 3;
   s->nf = s->nd + 2;
-  s->nu = ele_type == 4 ? n * n * n : n * n;
-  s->nfaces = ele_type == 4 ? 6 : 4;
-  s->nfpf = ele_type == 4 ? n * n : n;
+  if (ele_type == 2) {  /* src/eles_tets.cpp:68-141 */
+    s->nu = n * (n + 1) * (n + 2) / 6;
+    s->nfaces = 4;
+    s->nfpf = n * (n + 1) / 2;
+    s->nsp = 4;
+  } else {
+    s->nu = ele_type == 4 ? n * n * n : n * n;
+    s->nfaces = ele_type == 4 ? 6 : 4;
+    s->nfpf = ele_type == 4 ? n * n : n;
+    s->nsp = ele_type == 4 ? 8 : 4;
+  }
   s->nfp = s->nfaces * s->nfpf;
-  s->nsp = ele_type == 4 ? 8 : 4;
   s->n_eles = n_eles;
   s->n_verts = n_verts;
   for (int i = 0; i < n; i++) s->loc1d[i] = gauss_tab[s->order][i];
@@ -729,7 +1036,7 @@ void or_destroy(or_solver* s) {
   free(s->disu_fpts); free(s->tdisf_upts); free(s->norm_tdisf_fpts); free(s->norm_tconf_fpts);
   free(s->delta_disu_fpts); free(s->grad_disu_upts); free(s->grad_disu_fpts);
   free(s->sgsf_upts); free(s->sgsf_fpts);
-  free(s->face_l); free(s->face_r); free(s->bdy_fpt); free(s->bdy_type);
+  free(s->face_l); free(s->face_r); free(s->bdy_fpt); free(s->bdy_type); free(s->inv_vdm);
   free(s);
 }
 
@@ -1128,8 +1435,8 @@ static void calc_sgsf(const or_solver* s, const double* u, const double* g, doub
   }
   double inte = u[nf - 1] / rho - ke;
   for (int k = 0; k < nf * nd; k++) sgsf[k] = 0.0;
-  /* calc_ele_vol: detjac * reference volume (quads 4, hexes 8) */
-  double vol = detjac * (nd == 3 ? 8. : 4.);
+  /* calc_ele_vol: detjac * reference volume (quads 4, hexes 8, tets 8/6) */
+  double vol = s->ele_type == 2 ? detjac * 8. / 6. : detjac * (nd == 3 ? 8. : 4.);
   double delta = s->deck.filter_ratio * pow(vol, 1. / nd) / (s->order + 1.);
   for (int i = 0; i < nd; i++) {
     drho[i] = g[0 + nf * i];

@@ -17,6 +17,8 @@ QUAD_GAMBIT_TO_INTERNAL = [0, 1, 3, 2]
 
 def gambit_to_internal(ele_type, conn):
     conn = np.asarray(conn)
+    if ele_type == 2:  # linear tets: Gambit order is the internal order
+        return conn.copy()
     pos = HEX_GAMBIT_TO_INTERNAL if ele_type == 4 else QUAD_GAMBIT_TO_INTERNAL
     out = np.empty_like(conn)
     for i, p in enumerate(pos):
@@ -85,7 +87,7 @@ def write_gambit(path, xv, conn_gambit, ele_type, bc_groups=()):
     bc_groups: iterable of (name, [(ele_index0, gambit_face_number), ...])."""
     nd = xv.shape[1]
     ne = conn_gambit.shape[0]
-    gtype = 4 if ele_type == 4 else 2
+    gtype = {4: 4, 2: 6, 1: 2}[ele_type]
     lines = ["        CONTROL INFO 2.3.16", "** GAMBIT NEUTRAL FILE", "graft", "PROGRAM: graft  VERSION: 1",
              "date", "     NUMNP     NELEM     NGRPS    NBSETS     NDFCD     NDFVL",
              "%10d%10d%10d%10d%10d%10d" % (xv.shape[0], ne, 1, len(bc_groups), nd, nd),
@@ -121,19 +123,49 @@ def write_gambit(path, xv, conn_gambit, ele_type, bc_groups=()):
 # quad 0:s=-1 1:r=+1 2:s=+1 3:r=-1 (src/eles_quads.cpp:259-300).
 HEX_FACE_NODES = [(0, 1, 2, 3), (0, 1, 4, 5), (1, 3, 5, 7), (2, 3, 6, 7), (0, 2, 4, 6), (4, 5, 6, 7)]
 QUAD_FACE_NODES = [(0, 1), (1, 3), (2, 3), (0, 2)]
+# tet 0: opposite vertex 0 (slanted face), 1: r=-1, 2: s=-1, 3: t=-1 (src/geometry.cpp:3156-3185)
+TET_FACE_NODES = [(1, 2, 3), (0, 3, 2), (0, 1, 3), (0, 2, 1)]
 # Gambit face number k -> internal face (src/geometry.cpp read_boundary_gambit)
 HEX_GAMBIT_FACE_TO_INTERNAL = {1: 0, 2: 3, 3: 5, 4: 1, 5: 4, 6: 2}
+TET_GAMBIT_FACE_TO_INTERNAL = {1: 3, 2: 2, 3: 0, 4: 1}
 
 
 def internal_face_to_gambit(ele_type, f):
     if ele_type == 4:
         return {v: k for k, v in HEX_GAMBIT_FACE_TO_INTERNAL.items()}[f]
+    if ele_type == 2:
+        return {v: k for k, v in TET_GAMBIT_FACE_TO_INTERNAL.items()}[f]
     return f + 1
+
+
+def hex_to_tets(xv, conn_hex_gambit):
+    """Split every hex of a Gambit brick connectivity into 6 tets around its
+    (0,0,0)-(1,1,1) diagonal, the same split in every hex so that the face
+    diagonals match between neighbours; each tet is ordered so that its Jacobian
+    is positive.  Returns (n_hex*6, 4) vertex indices."""
+    conn = np.asarray(conn_hex_gambit)
+    # Gambit brick order: v(i,j,k) at index i + 2j + 4k
+    v = lambda c, i, j, k: c[i + 2 * j + 4 * k]
+    out = []
+    for c in conn:
+        d0, d1 = v(c, 0, 0, 0), v(c, 1, 1, 1)
+        ring = [v(c, 1, 0, 0), v(c, 1, 1, 0), v(c, 0, 1, 0), v(c, 0, 1, 1), v(c, 0, 0, 1), v(c, 1, 0, 1)]
+        for q in range(6):
+            t = [d0, ring[q], ring[(q + 1) % 6], d1]
+            a, b, cc, d = (xv[i] for i in t)
+            if np.linalg.det(np.array([b - a, cc - a, d - a])) < 0:
+                t[1], t[2] = t[2], t[1]
+            out.append(t)
+    return np.array(out, dtype=np.int64)
+
+
+def face_nodes_of(ele_type):
+    return {4: HEX_FACE_NODES, 2: TET_FACE_NODES, 1: QUAD_FACE_NODES}[ele_type]
 
 
 def boundary_faces(ele_type, c2v_internal):
     """(element, internal face) of every face that no other element shares."""
-    fn = HEX_FACE_NODES if ele_type == 4 else QUAD_FACE_NODES
+    fn = face_nodes_of(ele_type)
     seen = {}
     for e, nodes in enumerate(c2v_internal):
         for f, loc in enumerate(fn):
@@ -161,7 +193,7 @@ def channel_state(pos, mach, u_bulk=1.0):
 def wall_faces(ele_type, c2v_internal, xv, axis, values, tol=1e-9):
     """(element, internal face) of the boundary faces whose vertices all lie on the
     plane x_axis = v for one of `values`."""
-    fn = HEX_FACE_NODES if ele_type == 4 else QUAD_FACE_NODES
+    fn = face_nodes_of(ele_type)
     out = []
     for e, f in boundary_faces(ele_type, c2v_internal):
         coords = xv[[int(c2v_internal[e][i]) for i in fn[f]], axis]

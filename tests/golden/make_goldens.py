@@ -55,6 +55,7 @@ def deck_text(c, restart_iter=0):
         "restart_dump_freq %d" % c["n_steps"], "res_norm_type 1", "error_norm_type 1", "res_norm_field 0",
         "upts_type_quad 0", "vcjh_scheme_quad %d" % c["vcjh"], "eta_quad 0.0", "sparse_quad 0",
         "upts_type_hexa 0", "vcjh_scheme_hexa %d" % c["vcjh"], "eta_hexa 0.0", "sparse_hexa 0", "sparse_pri 0",
+        "upts_type_tet 0", "fpts_type_tet 0", "vcjh_scheme_tet %d" % c["vcjh"], "c_tet 0.0", "sparse_tet 0",
         "gamma 1.4", "prandtl 0.72", "S_gas 120.", "T_gas 291.15", "R_gas 286.9", "mu_gas 1.827E-05",
         "fix_vis 1", "Mach_free_stream %.17g" % c.get("mach", 0.1), "nx_free_stream 1.",
         "ny_free_stream 0.", "nz_free_stream 0.", "Re_free_stream %.17g" % c.get("Re", 1600.0),
@@ -79,14 +80,20 @@ def oracle_deck(c):
         filter_ratio=c.get("filter_ratio", 1.0))
 
 
-def write_restart(path, ele_type, loc1d, order, u0):
-    """u0: (Nu, N, nf) in the reference's point order."""
+def write_restart(path, ele_type, loc1d, order, u0, loc_upts=None):
+    """u0: (Nu, N, nf) in the reference's point order; loc_upts (3, Nu) for tets."""
     nu, n, nf = u0.shape
-    name = "HEXAS" if ele_type == 4 else "QUADS"
-    what = "hexahedral" if ele_type == 4 else "quadrilateral"
-    out = ["0", name, "Order", "%d" % order, "Number of solution points per %s element" % what,
-           "%d" % nu, "Location of solution points in 1D", " ".join("%.17g" % v for v in loc1d) + " ",
-           "n_eles", "%d" % n, "ele2global_ele array", " ".join("%d" % i for i in range(n)) + " ", "data"]
+    if ele_type == 2:  # src/eles_tets.cpp:749-800
+        out = ["0", "TETS", "Order", "%d" % order, "Number of solution points per element", "%d" % nu,
+               "Location of solution points in tetrahedral elements"]
+        out += [" ".join("%.17g" % loc_upts[k, j] for k in range(3)) + " " for j in range(nu)]
+        out += ["n_eles", "%d" % n, "ele2global_ele array", " ".join("%d" % i for i in range(n)) + " ", "data"]
+    else:
+        name = "HEXAS" if ele_type == 4 else "QUADS"
+        what = "hexahedral" if ele_type == 4 else "quadrilateral"
+        out = ["0", name, "Order", "%d" % order, "Number of solution points per %s element" % what,
+               "%d" % nu, "Location of solution points in 1D", " ".join("%.17g" % v for v in loc1d) + " ",
+               "n_eles", "%d" % n, "ele2global_ele array", " ".join("%d" % i for i in range(n)) + " ", "data"]
     for e in range(n):
         out.append("%d" % e)
         for j in range(nu):
@@ -108,7 +115,7 @@ def read_restart(path, nu, n, nf):
     return u
 
 
-def run_reference(c, xv, conn_g, u0, loc1d):
+def run_reference(c, xv, conn_g, u0, loc1d, loc_upts=None):
     wd = tempfile.mkdtemp(prefix="golden_")
     try:
         c2v = frmesh.gambit_to_internal(c["ele_type"], conn_g)
@@ -122,7 +129,7 @@ def run_reference(c, xv, conn_g, u0, loc1d):
         frmesh.write_gambit(os.path.join(wd, "mesh.neu"), xv, conn_g, c["ele_type"], bc_groups=groups)
         with open(os.path.join(wd, "input"), "w") as fh:
             fh.write(deck_text(c))
-        write_restart(os.path.join(wd, "Rest_000000000_p0000.dat"), c["ele_type"], loc1d, c["order"], u0)
+        write_restart(os.path.join(wd, "Rest_000000000_p0000.dat"), c["ele_type"], loc1d, c["order"], u0, loc_upts)
         env = dict(os.environ, HIFILES_HOME=REF_HOME)
         r = subprocess.run([REF_BIN, "input"], cwd=wd, env=env, capture_output=True, text=True, timeout=3600)
         if r.returncode != 0:
@@ -162,6 +169,19 @@ CASES = [
     dict(name="hex_p3_channel_walls_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=1, viscous=1, tau=1.0,
          pen_fact=0.5, state="channel", noise=5e-2, seed=34, perturb=0.0, kind="rk45", steps=3, dt=1e-3,
          channel=True, mach=0.2, Re=2800.0),
+    # tetrahedra (hex boxes split into 6 tets), DG correction
+    dict(name="tet_p3_ns_perturbed_2x2x2", ele_type=2, n=(2, 2, 2), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=41, perturb=0.1, kind="residual"),
+    dict(name="tet_p2_euler_2x2x2", ele_type=2, n=(2, 2, 2), order=2, vcjh=1, viscous=0,
+         state="tgv", noise=1e-2, seed=42, perturb=0.0, kind="residual"),
+    dict(name="tet_p1_ns_2x2x1", ele_type=2, n=(2, 2, 1), order=1, vcjh=1, viscous=1, tau=0.5,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=43, perturb=0.0, kind="residual"),
+    dict(name="tet_p3_channel_wale_walls_2x2x2", ele_type=2, n=(2, 2, 2), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=44, perturb=0.1, kind="residual", channel=True,
+         mach=0.2, Re=2800.0, les=1, sgs_model=1, filter_ratio=2.0),
+    dict(name="tet_p3_channel_rk45_2x1x1", ele_type=2, n=(2, 1, 1), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=45, perturb=0.0, kind="rk45", steps=3, dt=1e-3,
+         channel=True, mach=0.2, Re=2800.0, les=1, sgs_model=1, filter_ratio=2.0),
     # LES: WALE / Smagorinsky eddy viscosity (reference CPU path)
     dict(name="hex_p3_les_wale_perturbed_3x3x3", ele_type=4, n=(3, 3, 3), order=3, vcjh=1, viscous=1, tau=1.0,
          pen_fact=0.5, state="tgv", noise=1e-2, seed=21, perturb=0.1, kind="residual", les=1, sgs_model=1,
@@ -179,6 +199,23 @@ CASES = [
 
 
 def build_case(c):
+    if c["ele_type"] == 2:
+        # hex box split into 6 tets per hex; channel walls or periodic
+        nx, ny, nz = c["n"]
+        if c.get("channel"):
+            xv, hexes = frmesh.box_hex(nx, ny, nz, 2 * TWO_PI, 2.0, TWO_PI)
+            c["cyc"] = (2 * TWO_PI, float("inf"), TWO_PI)
+            h = 2.0 / ny
+        else:
+            xv, hexes = frmesh.box_hex(nx, ny, nz, TWO_PI, TWO_PI, TWO_PI)
+            c["cyc"] = (TWO_PI, TWO_PI, TWO_PI)
+            h = TWO_PI / max(c["n"])
+        if c["perturb"] > 0:
+            xv = frmesh.perturb_interior(xv, c["perturb"] * h, seed=c["seed"] + 100)
+        conn = frmesh.hex_to_tets(xv, hexes)
+        if c.get("channel"):
+            c["walls"] = frmesh.wall_faces(2, conn, xv, 1, (0.0, 2.0))
+        return xv, conn
     if c.get("channel"):
         # [0, 4pi] x [0, 2] (x [0, 2pi]), walls at y = 0, 2
         if c["ele_type"] == 4:
@@ -236,7 +273,7 @@ def main():
             u0 = u0 * (1.0 + frmesh.seeded_perturbation(u0.shape, c["noise"], c["seed"]))
         u0 = np.array([float("%.17g" % v) for v in u0.ravel()]).reshape(u0.shape)
         loc1d = o.get("loc_upts")[0, : c["order"] + 1]
-        u1, log = run_reference(c, xv, conn_g, u0, loc1d)
+        u1, log = run_reference(c, xv, conn_g, u0, loc1d, o.get("loc_upts"))
         rec = dict(xv=xv, conn_gambit=conn_g, ele_type=c["ele_type"], order=c["order"],
                    vcjh=c["vcjh"], viscous=c["viscous"], tau=c.get("tau", 0.0),
                    pen_fact=c.get("pen_fact", 0.5), cyc=np.array(c["cyc"]), dt=c["dt"],

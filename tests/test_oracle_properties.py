@@ -305,3 +305,136 @@ def test_hot_wall_heats_the_gas(ele_type):
     assert e[near].mean() > 0 and e[near].mean() > 10 * abs(e[~near].mean())
     assert np.abs(dudt[..., 1:o.nf - 1]).max() < 1e-10 * np.abs(e).max()
     assert dudt[..., 0][near].mean() < 0
+
+
+# --------------------------------------------------------------------------
+# tetrahedra: operators pinned by exact polynomial calculus on the reference
+# simplex (r, s, t >= -1, r + s + t <= -1), independent of the Dubiner formulas
+
+def _tet_oracle(order, perturb=0.0, viscous=1):
+    xv, hexes = frmesh.box_hex(2, 2, 2, TWO_PI, TWO_PI, TWO_PI)
+    if perturb:
+        xv = frmesh.perturb_interior(xv, perturb * TWO_PI / 2, seed=9)
+    tets = frmesh.hex_to_tets(xv, hexes)
+    d = dict(equation=0, viscous=viscous, order=order, adv_type=0, dt=1e-3, tau=1.0, pen_fact=0.5,
+             vcjh_scheme=1, eta=0.0, dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI)
+    d.update(cases.PHYS)
+    return po.Oracle(po.default_deck(**d), 2, xv, tets)
+
+
+def _monomials(P):
+    return [(a, b, c) for a in range(P + 1) for b in range(P + 1 - a) for c in range(P + 1 - a - b)]
+
+
+def _mono(x, e):
+    return x[0] ** e[0] * x[1] ** e[1] * x[2] ** e[2]
+
+
+def _tet_rule(m=8):
+    """Collapsed Gauss rule on the reference tet, exact to degree 2m-3."""
+    g, w = np.polynomial.legendre.leggauss(m)
+    pts, wts = [], []
+    for i in range(m):
+        for j in range(m):
+            for k in range(m):
+                a, b, c = g[i], g[j], g[k]
+                r = 0.25 * (1 + a) * (1 - b) * (1 - c) - 1
+                s = 0.5 * (1 + b) * (1 - c) - 1
+                t = c
+                pts.append((r, s, t))
+                wts.append(w[i] * w[j] * w[k] * 0.125 * (1 - b) * (1 - c) ** 2)
+    return np.array(pts), np.array(wts)
+
+
+def _tri_rule(m=8):
+    g, w = np.polynomial.legendre.leggauss(m)
+    pts, wts = [], []
+    for i in range(m):
+        for j in range(m):
+            a, b = g[i], g[j]
+            pts.append((0.5 * (1 + a) * (1 - b) - 1, b))
+            wts.append(w[i] * w[j] * 0.5 * (1 - b))
+    return np.array(pts), np.array(wts)
+
+
+def _nodal_basis_eval(upts, P, x):
+    """Values at x (n,3) of the Lagrange basis on the solution points, built from
+    monomials (an independent construction of the reference's nodal basis)."""
+    E = _monomials(P)
+    V = np.array([[_mono(p, e) for e in E] for p in upts])
+    M = np.array([[_mono(p, e) for e in E] for p in x])
+    return M @ np.linalg.inv(V)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_tet_interpolation_and_derivative_exact(order):
+    o = _tet_oracle(order)
+    up = o.get("loc_upts").T
+    fp = o.get("tloc_fpts").T
+    I, D = o.get("opp_0"), o.get("opp_4")
+    for e in _monomials(order):
+        v = np.array([_mono(p, e) for p in up])
+        assert np.allclose(I @ v, [_mono(p, e) for p in fp], atol=1e-12)
+        for c in range(3):
+            de = list(e)
+            if e[c] == 0:
+                exact = np.zeros(len(up))
+            else:
+                de[c] -= 1
+                exact = e[c] * np.array([_mono(p, de) for p in up])
+            assert np.allclose(D[..., c] @ v, exact, atol=1e-11)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_tet_dg_correction_is_the_face_lift(order):
+    # DG (c_tet = 0): the correction field L_f of flux point f is the L2
+    # projection onto P_P(tet) of its face Lagrange polynomial, i.e. for every v in
+    # P_P: int_tet v L_f = int_face v l_f dA.  The reference's face rule has
+    # 15-digit weights (sum 2 - 2e-13), so the identity holds to ~1e-12.
+    o = _tet_oracle(order)
+    up = o.get("loc_upts").T
+    fp = o.get("tloc_fpts").T
+    L = o.get("opp_3")  # (nu, nfp)
+    nfpf = (order + 1) * (order + 2) // 2
+    q3, w3 = _tet_rule()
+    B = _nodal_basis_eval(up, order, q3)  # (nq, nu)
+    q2, w2 = _tri_rule()
+    faces = [(lambda a, b: (a, -1 - a - b, b), np.sqrt(3.0)), (lambda a, b: (-1.0, b, a), 1.0),
+             (lambda a, b: (a, -1.0, b), 1.0), (lambda a, b: (b, a, -1.0), 1.0)]
+    face_coords = [lambda p: (p[0], p[2]), lambda p: (p[2], p[1]), lambda p: (p[0], p[2]), lambda p: (p[1], p[0])]
+    E2 = [(a, b) for a in range(order + 1) for b in range(order + 1 - a)]
+    worst = 0.0
+    for f in range(4):
+        fx = np.array([face_coords[f](p) for p in fp[f * nfpf:(f + 1) * nfpf]])
+        V2 = np.array([[x[0] ** a * x[1] ** b for a, b in E2] for x in fx])
+        C2 = np.linalg.inv(V2)  # face Lagrange coefficients in monomials
+        M2 = np.array([[x[0] ** a * x[1] ** b for a, b in E2] for x in q2]) @ C2  # (nq2, nfpf)
+        x3 = np.array([faces[f][0](a, b) for a, b in q2])
+        for e in _monomials(order):
+            vq = np.array([_mono(p, e) for p in q3])
+            lhs = (w3 * vq) @ (B @ L[:, f * nfpf:(f + 1) * nfpf])
+            rhs = faces[f][1] * ((w2 * np.array([_mono(p, e) for p in x3])) @ M2)
+            worst = max(worst, np.abs(lhs - rhs).max())
+    assert worst < 5e-12, worst
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_tet_free_stream_and_conservation(order):
+    o = _tet_oracle(order, perturb=0.08)
+    u = np.zeros((o.nu, o.n_eles, 5))
+    u[..., 0], u[..., 1], u[..., 2], u[..., 3] = 1.0, 0.3, -0.2, 0.1
+    u[..., 4] = 71.4 / 0.4 + 0.5 * 0.14
+    o.set_state(u)
+    o.residual()
+    assert np.abs(o.get("div_tconf_upts")).max() < 1e-9
+    u = frmesh.tgv_state(o.get("pos_upts"), p0=1 / (1.4 * 0.01))
+    o.set_state(u)
+    o.residual()
+    R = o.get("div_tconf_upts")
+    # integral of the reference-space divergence over each element with the
+    # weights W_j = int l_j (exact for the degree-P interpolant) sums to zero
+    q3, w3 = _tet_rule()
+    W = w3 @ _nodal_basis_eval(o.get("loc_upts").T, order, q3)
+    total = np.einsum("j,jek->k", W, R)
+    scale = np.einsum("j,jek->k", W, np.abs(R))
+    assert np.all(np.abs(total) < 1e-12 * scale), (total, scale)
