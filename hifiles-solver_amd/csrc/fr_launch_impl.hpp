@@ -82,5 +82,57 @@ hipError_t launch(int nd, int P, int mode, int vis, bool aff, int les, const Arg
   }
 }
 
+// ---- tetrahedra ----
+template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES>
+hipError_t tet_les(const Args<MIX>& a, hipStream_t s) {
+  using T = TetP<P>;
+  const int grid = (a.N + T::EPB - 1) / T::EPB;
+  hipLaunchKernelGGL((fr_dense_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int P, int VIS, int MODE, bool AFF, bool MIX>
+hipError_t tet_aff(const Args<MIX>& a, hipStream_t s, int les) {
+  if constexpr (VIS == 1 && MODE != MODE_TRACE_U) {
+    if (les == 1) return tet_les<P, VIS, MODE, AFF, MIX, 1>(a, s);
+    if (les == 2) return tet_les<P, VIS, MODE, AFF, MIX, 2>(a, s);
+  }
+  if (les != 0) return hipErrorInvalidValue;
+  return tet_les<P, VIS, MODE, AFF, MIX, 0>(a, s);
+}
+
+template <int P, int VIS, int MODE, bool MIX>
+hipError_t tet_one(const Args<MIX>& a, hipStream_t s, bool aff, int les) {
+  if constexpr (MODE == MODE_TRACE_U) return tet_aff<P, VIS, MODE, false, MIX>(a, s, 0);
+  return aff ? tet_aff<P, VIS, MODE, true, MIX>(a, s, les) : tet_aff<P, VIS, MODE, false, MIX>(a, s, les);
+}
+
+template <int P, bool MIX>
+hipError_t tet_mode(int mode, int vis, const Args<MIX>& a, hipStream_t s, bool f, int les) {
+  switch (mode) {
+    case MODE_TRACE_U: return tet_one<P, 0, MODE_TRACE_U, MIX>(a, s, f, les);
+    case MODE_TRACE_GRAD:
+      return vis == 2 ? tet_one<P, 2, MODE_TRACE_GRAD, MIX>(a, s, f, les)
+                      : tet_one<P, 1, MODE_TRACE_GRAD, MIX>(a, s, f, les);
+    case MODE_RESID:
+      return vis == 2 ? tet_one<P, 2, MODE_RESID, MIX>(a, s, f, les)
+           : vis == 1 ? tet_one<P, 1, MODE_RESID, MIX>(a, s, f, les)
+                      : tet_one<P, 0, MODE_RESID, MIX>(a, s, f, les);
+    default:
+      return vis == 2 ? tet_one<P, 2, MODE_ADVANCE, MIX>(a, s, f, les)
+           : vis == 1 ? tet_one<P, 1, MODE_ADVANCE, MIX>(a, s, f, les)
+                      : tet_one<P, 0, MODE_ADVANCE, MIX>(a, s, f, les);
+  }
+}
+
+template <bool MIX>
+hipError_t launch_tet(int P, int mode, int vis, bool aff, int les, const Args<MIX>& a, hipStream_t s) {
+  switch (P) {
+    case 1: return tet_mode<1, MIX>(mode, vis, a, s, aff, les);
+    case 2: return tet_mode<2, MIX>(mode, vis, a, s, aff, les);
+    default: return tet_mode<3, MIX>(mode, vis, a, s, aff, les);
+  }
+}
+
 }  // namespace dispatch
 }  // namespace hifr

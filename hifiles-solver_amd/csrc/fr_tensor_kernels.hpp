@@ -79,6 +79,10 @@ struct StageArgsT {
   Phys ph;
   RefState<double> ref;  // mixed path: reference state of the deviations
   double filter_ratio;   // LES filter width factor
+  // dense element operators (tets): I [Nfp][Nu], D [d][Nu][Nu], L [Nu][Nfp]
+  const R* opI;
+  const R* opD;
+  const R* opL;
 };
 using StageArgs = StageArgsT<double>;
 

@@ -16,7 +16,7 @@ using hifr::StageArgsT;
 
 struct hifr_ctx {
   hifr_config cfg;
-  int nd, nf, n1, nu, nfp, nfpf, N;
+  int nd, nf, n1, nu, nfp, nfpf, nfaces, N;
   hipStream_t stream;
   Ops1D ops;
   hifr::Ops1DT<float> opsf;
@@ -28,6 +28,7 @@ struct hifr_ctx {
   // device buffers
   double *u, *res, *div, *detj, *fgeo;
   void *uf[2], *fv, *gf, *jg;  // element type double, or float when mixed
+  void *opI, *opD, *opL;       // tets: dense operators (double, or float when mixed)
   int* nbr;
   double* red;  // reduction scratch
   int cur;      // which uf buffer holds the traces of the current state
@@ -60,6 +61,7 @@ hifr_status dev_err(hipError_t e, const char* what) {
 bool order_supported(int ele_type, int P) {
   if (ele_type == HIFR_HEX) return P >= 1 && P <= 5;
   if (ele_type == HIFR_QUAD) return P >= 1 && P <= 6;
+  if (ele_type == HIFR_TET) return P >= 1 && P <= 3;
   return false;
 }
 
@@ -94,6 +96,9 @@ StageArgsT<R> base_args(hifr_ctx* c) {
   a.ph.T_wall = c->cfg.T_wall;
   a.ref = c->ref;
   a.filter_ratio = c->cfg.filter_ratio;
+  a.opI = (const R*)c->opI;
+  a.opD = (const R*)c->opD;
+  a.opL = (const R*)c->opL;
   return a;
 }
 
@@ -118,6 +123,12 @@ hipError_t launch_io(hifr_ctx* c, int mode, const StageIO& io) {
   a.rk_a = io.rk_a;
   a.rk_b = io.rk_b;
   a.dt = io.dt;
+  if (c->cfg.ele_type == HIFR_TET) {
+    if constexpr (std::is_same_v<R, float>)
+      return hifr::launch_tet_mixed(c->cfg.order, mode, vis_kind(c), c->affine, les_kind(c), a, c->stream);
+    else
+      return hifr::launch_tet_fp64(c->cfg.order, mode, vis_kind(c), c->affine, les_kind(c), a, c->stream);
+  }
   if constexpr (std::is_same_v<R, float>)
     return hifr::launch_mixed(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, les_kind(c), a, c->stream);
   else
@@ -212,7 +223,8 @@ const char* hifr_status_string(hifr_status s) {
 hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   if (!cfg || !out) return HIFR_ERR_ARG;
   *out = nullptr;
-  if (cfg->ele_type != HIFR_HEX && cfg->ele_type != HIFR_QUAD) return HIFR_ERR_UNSUPPORTED;
+  if (cfg->ele_type != HIFR_HEX && cfg->ele_type != HIFR_QUAD && cfg->ele_type != HIFR_TET)
+    return HIFR_ERR_UNSUPPORTED;
   if (!order_supported(cfg->ele_type, cfg->order)) return HIFR_ERR_UNSUPPORTED;
   if (cfg->n_eles <= 0) return HIFR_ERR_ARG;
   if (cfg->riemann_solve_type != 0 || cfg->vis_riemann_solve_type != 0) return HIFR_ERR_UNSUPPORTED;
@@ -223,12 +235,19 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HIFR_ERR_DEVICE;
   hifr_ctx* c = new hifr_ctx();
   c->cfg = *cfg;
-  c->nd = cfg->ele_type == HIFR_HEX ? 3 : 2;
+  c->nd = cfg->ele_type == HIFR_QUAD ? 2 : 3;
   c->nf = c->nd + 2;
   c->n1 = cfg->order + 1;
-  c->nu = c->nd == 3 ? c->n1 * c->n1 * c->n1 : c->n1 * c->n1;
-  c->nfpf = c->nd == 3 ? c->n1 * c->n1 : c->n1;
-  c->nfp = 2 * c->nd * c->nfpf;
+  if (cfg->ele_type == HIFR_TET) {
+    c->nu = c->n1 * (c->n1 + 1) * (c->n1 + 2) / 6;
+    c->nfpf = c->n1 * (c->n1 + 1) / 2;
+    c->nfaces = 4;
+  } else {
+    c->nu = c->nd == 3 ? c->n1 * c->n1 * c->n1 : c->n1 * c->n1;
+    c->nfpf = c->nd == 3 ? c->n1 * c->n1 : c->n1;
+    c->nfaces = 2 * c->nd;
+  }
+  c->nfp = c->nfaces * c->nfpf;
   c->N = cfg->n_eles;
   c->stream = nullptr;
   c->cur = 0;
@@ -252,6 +271,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   for (auto& al : allocs) *al.p = nullptr;
   c->detj = c->fgeo = nullptr;  // sized by hifr_set_metrics / hifr_set_interfaces
   c->jg = nullptr;
+  c->opI = c->opD = c->opL = nullptr;
   c->affine = false;
   for (auto& al : allocs) {
     if (hipMalloc(al.p, al.bytes) != hipSuccess) {
@@ -269,7 +289,8 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
 
 hifr_status hifr_destroy(hifr_ctx* c) {
   if (!c) return HIFR_OK;
-  void* ps[] = {c->u, c->res, c->div, c->uf[0], c->uf[1], c->fv, c->gf, c->detj, c->jg, c->fgeo, c->nbr, c->red};
+  void* ps[] = {c->u, c->res, c->div, c->uf[0], c->uf[1], c->fv, c->gf, c->detj, c->jg, c->fgeo, c->nbr, c->red,
+                c->opI, c->opD, c->opL};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -292,8 +313,48 @@ hifr_status hifr_sizes(const hifr_ctx* c, int* nu, int* nfp, int* nf, int* nd) {
   return HIFR_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Tets: the dense operators in the kernel layouts I [Nfp][Nu], D [d][Nu][Nu],
+// L [Nu][Nfp], in the context's arithmetic type.
+template <class R>
+hifr_status upload_dense(hifr_ctx* c, const double* opp_0, const double* opp_3, const double* opp_4) {
+  const size_t nu = c->nu, nfp = c->nfp, nd = c->nd;
+  std::vector<R> I(nfp * nu), D(nd * nu * nu), L(nu * nfp);
+  for (size_t f = 0; f < nfp; f++)
+    for (size_t j = 0; j < nu; j++) I[f * nu + j] = R(opp_0[f + nfp * j]);
+  for (size_t d = 0; d < nd; d++)
+    for (size_t p = 0; p < nu; p++)
+      for (size_t m = 0; m < nu; m++) D[(d * nu + p) * nu + m] = R(opp_4[p + nu * (m + nu * d)]);
+  for (size_t p = 0; p < nu; p++)
+    for (size_t f = 0; f < nfp; f++) L[p * nfp + f] = R(opp_3[p + nu * f]);
+  for (double v : I) if (!std::isfinite(v)) return HIFR_ERR_ARG;
+  for (double v : D) if (!std::isfinite(v)) return HIFR_ERR_ARG;
+  for (double v : L) if (!std::isfinite(v)) return HIFR_ERR_ARG;
+  struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->opI, &I}, {&c->opD, &D}, {&c->opL, &L}};
+  for (auto& b : bufs) {
+    if (*b.p) (void)hipFree(*b.p);
+    *b.p = nullptr;
+    HIP_TRY(hipMalloc(b.p, sizeof(R) * b.v->size()));
+    HIP_TRY(hipMemcpy(*b.p, b.v->data(), sizeof(R) * b.v->size(), hipMemcpyHostToDevice));
+  }
+  return HIFR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 hifr_status hifr_set_operators(hifr_ctx* c, const double* opp_0, const double* opp_3, const double* opp_4) {
   if (!c || !opp_0 || !opp_3 || !opp_4) return HIFR_ERR_ARG;
+  if (c->cfg.ele_type == HIFR_TET) {
+    hifr_status st = c->mixed ? upload_dense<float>(c, opp_0, opp_3, opp_4) : upload_dense<double>(c, opp_0, opp_3, opp_4);
+    if (st != HIFR_OK) return st;
+    c->have_ops = true;
+    return HIFR_OK;
+  }
   const int n = c->n1, P = c->cfg.order, nu = c->nu, nfp = c->nfp, nd = c->nd;
   Ops1D o;
   std::memset(&o, 0, sizeof(o));
@@ -378,7 +439,7 @@ bool all_affine(const hifr_ctx* c, const double* detjac, const double* JGinv, co
     const double* dj = detjac + nu * e;
     for (size_t p = 0; p < nu; p++)
       if (!(std::fabs(dj[p] - dj[0]) <= rel * std::fabs(dj[0]))) return false;
-    for (size_t f = 0; f < 2 * nd; f++) {
+    for (size_t f = 0; f < (size_t)c->nfaces; f++) {
       const size_t f0 = f * nfpf + nfp * e;
       for (size_t q = 0; q < nfpf; q++) {
         const size_t fe = f0 + q;
@@ -461,7 +522,7 @@ hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, con
   if (!c->have_metrics) return HIFR_ERR_STATE;
   const size_t N = c->N, nfp = c->nfp, nd = c->nd, tot = nfp * N, nfpf = c->nfpf;
   // face geometry per flux point, or per face when the metrics are per element
-  const size_t gtot = c->affine ? 2 * nd * N : tot;
+  const size_t gtot = c->affine ? (size_t)c->nfaces * N : tot;
   auto gidx = [&](size_t fe) { return c->affine ? fe / nfpf : fe; };
   std::vector<int> nbr(tot, -1);
   std::vector<double> geo((nd + 1) * gtot, 0.0);

@@ -4,6 +4,9 @@
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
+#include <utility>
+
+#include "tet_tables.h"
 
 namespace hifr_host {
 
@@ -101,6 +104,29 @@ eles::~eles() {
 std::string eles::setup(const input& in, int n) {
   n_eles = n;
   order = in.order;
+  const int n1 = order + 1;
+  if (ele_type == 2) {
+    // src/eles_tets.cpp:68-141
+    if (in.upts_type_tet != 0 || in.fpts_type_tet != 0)
+      return "tets are built with the tabulated interior points only (upts_type_tet 0, fpts_type_tet 0)";
+    if (!(in.vcjh_scheme_tet == 1 || (in.vcjh_scheme_tet == 0 && in.c_tet == 0.0)))
+      return "tets are built with the DG correction only (vcjh_scheme_tet 1, or 0 with c_tet 0)";
+    if (order < 1 || order > 3) return "tets are built for orders 1-3";
+    n_dims = 3;
+    n_inters_per_ele = 4;
+    n_upts_per_ele = n1 * (n1 + 1) * (n1 + 2) / 6;
+    n_fpts_per_inter = n1 * (n1 + 1) / 2;
+    n_spts_per_ele = 4;
+    n_fields = 5;
+    n_fpts_per_ele = n_inters_per_ele * n_fpts_per_inter;
+    eta = 0.0;
+    set_points();
+    set_opp_0();
+    set_opp_3();
+    set_opp_4();
+    shape.assign((size_t)n_dims * n_spts_per_ele * n_eles, 0.0);
+    return "";
+  }
   const int scheme = ele_type == 4 ? in.vcjh_scheme_hexa : in.vcjh_scheme_quad;
   const int upts_type = ele_type == 4 ? in.upts_type_hexa : in.upts_type_quad;
   if (upts_type != 0) return "only Gauss solution points (upts_type 0) are supported";
@@ -110,7 +136,6 @@ std::string eles::setup(const input& in, int n) {
   } catch (const std::exception& e) {
     return e.what();
   }
-  const int n1 = order + 1;
   if (ele_type == 4) {
     n_dims = 3; n_inters_per_ele = 6; n_upts_per_ele = n1 * n1 * n1; n_fpts_per_inter = n1 * n1; n_spts_per_ele = 8;
   } else {
@@ -410,6 +435,295 @@ void eles_hexas::eval_shape(const double* loc, double* N, double* dN) const {
 const int* eles_hexas::face_nodes(int face, int& n) const {
   static const int fn[6][4] = {{0, 1, 2, 3}, {0, 1, 4, 5}, {1, 3, 5, 7}, {2, 3, 6, 7}, {0, 2, 4, 6}, {4, 5, 6, 7}};
   n = 4;
+  return fn[face];
+}
+
+// ----------------------------------------------------------------------------
+// tets (src/eles_tets.cpp)
+
+namespace {
+
+// Orthonormal Jacobi polynomial P_n^(a,b)(x) (unit norm on [-1,1] with weight
+// (1-x)^a (1+x)^b), by the normalised three-term recurrence (Hesthaven & Warburton,
+// Nodal Discontinuous Galerkin Methods, 2008, JacobiP).
+double jacobi(double x, double a, double b, int n) {
+  const double g0 = std::pow(2.0, a + b + 1.0) / (a + b + 1.0) * std::tgamma(a + 1.0) * std::tgamma(b + 1.0) /
+                    std::tgamma(a + b + 1.0);
+  double pm = 1.0 / std::sqrt(g0);
+  if (n == 0) return pm;
+  const double g1 = (a + 1.0) * (b + 1.0) / (a + b + 3.0) * g0;
+  double p = ((a + b + 2.0) * x / 2.0 + (a - b) / 2.0) / std::sqrt(g1);
+  double aold = 2.0 / (2.0 + a + b) * std::sqrt((a + 1.0) * (b + 1.0) / (a + b + 3.0));
+  for (int i = 1; i < n; i++) {
+    const double h1 = 2.0 * i + a + b;
+    const double anew =
+        2.0 / (h1 + 2.0) * std::sqrt((i + 1.0) * (i + 1.0 + a + b) * (i + 1.0 + a) * (i + 1.0 + b) / (h1 + 1.0) / (h1 + 3.0));
+    const double bnew = -(a * a - b * b) / h1 / (h1 + 2.0);
+    const double pn = (-aold * pm + (x - bnew) * p) / anew;
+    pm = p;
+    p = pn;
+    aold = anew;
+  }
+  return p;
+}
+
+double d_jacobi(double x, double a, double b, int n) {
+  return n == 0 ? 0.0 : std::sqrt(n * (n + a + b + 1.0)) * jacobi(x, a + 1.0, b + 1.0, n - 1);
+}
+
+// Mode numbering of the orthonormal bases as the reference orders them
+// (src/funcs.cpp:1190-1378): triangles by total degree, then j; tetrahedra by total
+// degree m, then n = j + k, then k.
+void tri_ij(int mode, int& i, int& j) {
+  int q = 0;
+  for (int deg = 0;; deg++)
+    for (int jj = 0; jj <= deg; jj++, q++)
+      if (q == mode) { i = deg - jj; j = jj; return; }
+}
+void tet_ijk(int mode, int& i, int& j, int& k) {
+  int q = 0;
+  for (int m = 0;; m++)
+    for (int n = 0; n <= m; n++)
+      for (int kk = 0; kk <= n; kk++, q++)
+        if (q == mode) { i = m - n; j = n - kk; k = kk; return; }
+}
+
+// Orthonormal triangle basis on {r, s >= -1, r + s <= 0} through the collapsed
+// coordinates a = 2(1+r)/(1-s) - 1, b = s.
+double tri_mode(double r, double s, int mode) {
+  int i, j;
+  tri_ij(mode, i, j);
+  const double a = s == 1.0 ? -1.0 : 2.0 * (1.0 + r) / (1.0 - s) - 1.0;
+  return std::sqrt(2.0) * jacobi(a, 0, 0, i) * jacobi(s, 2 * i + 1, 0, j) * std::pow(1.0 - s, i);
+}
+
+void collapse3(double r, double s, double t, double& a, double& b, double& c) {
+  a = (s + t == 0.0) ? -1.0 : -2.0 * (1.0 + r) / (s + t) - 1.0;
+  b = t == 1.0 ? -1.0 : 2.0 * (1.0 + s) / (1.0 - t) - 1.0;
+  c = t;
+}
+
+// Orthonormal tetrahedron basis (Hesthaven & Warburton, Simplex3DP).
+double tet_mode(double r, double s, double t, int mode) {
+  int i, j, k;
+  tet_ijk(mode, i, j, k);
+  double a, b, c;
+  collapse3(r, s, t, a, b, c);
+  return 2.0 * std::sqrt(2.0) * jacobi(a, 0, 0, i) * jacobi(b, 2 * i + 1, 0, j) * std::pow(1.0 - b, i) *
+         jacobi(c, 2 * (i + j) + 2, 0, k) * std::pow(1.0 - c, i + j);
+}
+
+// Gradient of the tetrahedron basis (Hesthaven & Warburton, GradSimplex3DP).
+void tet_mode_grad(double r, double s, double t, int mode, double* g) {
+  int i, j, k;
+  tet_ijk(mode, i, j, k);
+  double a, b, c;
+  collapse3(r, s, t, a, b, c);
+  const double fa = jacobi(a, 0, 0, i), dfa = d_jacobi(a, 0, 0, i);
+  const double gb = jacobi(b, 2 * i + 1, 0, j), dgb = d_jacobi(b, 2 * i + 1, 0, j);
+  const double hc = jacobi(c, 2 * (i + j) + 2, 0, k), dhc = d_jacobi(c, 2 * (i + j) + 2, 0, k);
+  const double hb = 0.5 * (1.0 - b), hh = 0.5 * (1.0 - c);
+  double vr = dfa * gb * hc;
+  if (i > 0) vr *= std::pow(hb, i - 1);
+  if (i + j > 0) vr *= std::pow(hh, i + j - 1);
+  double tmp = dgb * std::pow(hb, i);
+  if (i > 0) tmp += -0.5 * i * gb * std::pow(hb, i - 1);
+  if (i + j > 0) tmp *= std::pow(hh, i + j - 1);
+  tmp = fa * tmp * hc;
+  const double vs = 0.5 * (1.0 + a) * vr + tmp;
+  double vt = 0.5 * (1.0 + a) * vr + 0.5 * (1.0 + b) * tmp;
+  double tc = dhc * std::pow(hh, i + j);
+  if (i + j > 0) tc -= 0.5 * (i + j) * hc * std::pow(hh, i + j - 1);
+  vt += fa * gb * tc * std::pow(hb, i);
+  const double sc = std::pow(2.0, 2 * i + j + 1.5);
+  g[0] = vr * sc;
+  g[1] = vs * sc;
+  g[2] = vt * sc;
+}
+
+// Inverse of an n x n column-major matrix by Gauss-Jordan elimination with partial
+// pivoting.
+std::vector<double> inverse(std::vector<double> A, int n) {
+  std::vector<double> X((size_t)n * n, 0.0);
+  for (int i = 0; i < n; i++) X[i + (size_t)n * i] = 1.0;
+  for (int k = 0; k < n; k++) {
+    int piv = k;
+    for (int i = k + 1; i < n; i++)
+      if (std::fabs(A[i + (size_t)n * k]) > std::fabs(A[piv + (size_t)n * k])) piv = i;
+    if (A[piv + (size_t)n * k] == 0.0) throw std::runtime_error("singular Vandermonde matrix");
+    if (piv != k)
+      for (int j = 0; j < n; j++) {
+        std::swap(A[k + (size_t)n * j], A[piv + (size_t)n * j]);
+        std::swap(X[k + (size_t)n * j], X[piv + (size_t)n * j]);
+      }
+    const double d = A[k + (size_t)n * k];
+    for (int j = 0; j < n; j++) {
+      A[k + (size_t)n * j] /= d;
+      X[k + (size_t)n * j] /= d;
+    }
+    for (int i = 0; i < n; i++) {
+      if (i == k) continue;
+      const double f = A[i + (size_t)n * k];
+      if (f == 0.0) continue;
+      for (int j = 0; j < n; j++) {
+        A[i + (size_t)n * j] -= f * A[k + (size_t)n * j];
+        X[i + (size_t)n * j] -= f * X[k + (size_t)n * j];
+      }
+    }
+  }
+  return X;
+}
+
+// Face coordinates (r_f, s_f) of a point on face f of the reference tet, and the
+// inverse map; the slanted face 0 is parametrised by (r, t) (src/eles_tets.cpp:1018-1131).
+void tet_face_coords(int f, const double* x, double& rf, double& sf) {
+  switch (f) {
+    case 0: rf = x[0]; sf = x[2]; break;
+    case 1: rf = x[2]; sf = x[1]; break;
+    case 2: rf = x[0]; sf = x[2]; break;
+    default: rf = x[1]; sf = x[0]; break;
+  }
+}
+void tet_face_point(int f, double rf, double sf, double* x) {
+  switch (f) {
+    case 0: x[0] = rf; x[2] = sf; x[1] = -1.0 - sf - rf; break;
+    case 1: x[0] = -1.0; x[1] = sf; x[2] = rf; break;
+    case 2: x[0] = rf; x[1] = -1.0; x[2] = sf; break;
+    default: x[0] = sf; x[1] = rf; x[2] = -1.0; break;
+  }
+}
+
+}  // namespace
+
+// Solution points: the tabulated interior set (src/eles_tets.cpp:233-270).  Flux
+// points: the triangle set mapped onto each face (set_tloc_fpts, :273-329), with the
+// first coordinate of face 0 taken from the mirrored triangle point.  Normals: face 0
+// (1,1,1)/sqrt(3), faces 1-3 -e_r, -e_s, -e_t (:589-620).
+void eles_tets::set_points() {
+  const int P = order, nu = n_upts_per_ele, nf1 = n_fpts_per_inter;
+  const double* up = tet_tables::upts(P);
+  const double* tp = tet_tables::tri(P);
+  loc_upts.assign(up, up + 3 * nu);
+  tloc_fpts.assign(3 * n_fpts_per_ele, 0.0);
+  tnorm_fpts.assign(3 * n_fpts_per_ele, 0.0);
+  for (int j = 0; j <= P; j++)
+    for (int i = 0; i <= P - j; i++) {
+      const int it = j * (P + 1) - (j - 1) * j / 2 + i;
+      const int im = j * (P + 1) - (j - 1) * j / 2 + (P - j - i);
+      const double r = tp[2 * it], s = tp[2 * it + 1];
+      double* t0 = &tloc_fpts[3 * it];
+      t0[0] = tp[2 * im]; t0[1] = r; t0[2] = s;
+      double* t1 = &tloc_fpts[3 * (nf1 + it)];
+      t1[0] = -1.0; t1[1] = s; t1[2] = r;
+      double* t2 = &tloc_fpts[3 * (2 * nf1 + it)];
+      t2[0] = r; t2[1] = -1.0; t2[2] = s;
+      double* t3 = &tloc_fpts[3 * (3 * nf1 + it)];
+      t3[0] = s; t3[1] = r; t3[2] = -1.0;
+    }
+  for (int f = 0; f < 4; f++)
+    for (int q = 0; q < nf1; q++) {
+      double* tn = &tnorm_fpts[3 * (f * nf1 + q)];
+      if (f == 0) tn[0] = tn[1] = tn[2] = 1.0 / std::sqrt(3.0);
+      else tn[f - 1] = -1.0;
+    }
+  // Vandermonde V(i, m) = psi_m(x_i) and its inverse (set_vandermonde, :722-733)
+  std::vector<double> V((size_t)nu * nu);
+  for (int i = 0; i < nu; i++)
+    for (int m = 0; m < nu; m++) V[i + (size_t)nu * m] = tet_mode(loc_upts[3 * i], loc_upts[3 * i + 1], loc_upts[3 * i + 2], m);
+  inv_vdm = inverse(V, nu);
+}
+
+// l_idx(x) = sum_m Vinv(m, idx) psi_m(x)   (src/eles_tets.cpp:805-822)
+double eles_tets::eval_nodal_basis(int idx, const double* loc) const {
+  const int nu = n_upts_per_ele;
+  double v = 0.0;
+  for (int m = 0; m < nu; m++) v += inv_vdm[m + (size_t)nu * idx] * tet_mode(loc[0], loc[1], loc[2], m);
+  return v;
+}
+
+// (src/eles_tets.cpp:843-860)
+double eles_tets::eval_d_nodal_basis(int idx, int c, const double* loc) const {
+  const int nu = n_upts_per_ele;
+  double v = 0.0, g[3];
+  for (int m = 0; m < nu; m++) {
+    tet_mode_grad(loc[0], loc[1], loc[2], m, g);
+    v += inv_vdm[m + (size_t)nu * idx] * g[c];
+  }
+  return v;
+}
+
+double eles_tets::eval_div_vcjh_basis(int, const double*) const {
+  throw std::logic_error("tets build their correction operator in set_opp_3");
+}
+
+// DG correction (c_tet = 0, so the filter of src/eles_tets.cpp:972-1016 is the
+// identity): the correction field of flux point q is the L2 projection onto P_P(tet)
+// of its face Lagrange polynomial, lifted from the face (eval_div_dg_tet,
+// :1018-1131).  With the orthonormal basis the projection's coefficients are the
+// face integrals int_face psi_m l_q dA, taken with the 36-point triangle rule and the
+// face Jacobian (sqrt(3) on the slanted face).
+void eles_tets::set_opp_3() {
+  const int nu = n_upts_per_ele, nf1 = n_fpts_per_inter, nfp = n_fpts_per_ele;
+  opp_3.assign((size_t)nu * nfp, 0.0);
+  std::vector<double> coef(nu);
+  for (int f = 0; f < 4; f++) {
+    // face Vandermonde W(q, m) = phi_m(face point q); column q of W^-1 gives the
+    // orthonormal coefficients of the face Lagrange polynomial l_q
+    std::vector<double> W((size_t)nf1 * nf1);
+    for (int q = 0; q < nf1; q++) {
+      double rf, sf;
+      tet_face_coords(f, &tloc_fpts[3 * (f * nf1 + q)], rf, sf);
+      for (int m = 0; m < nf1; m++) W[q + (size_t)nf1 * m] = tri_mode(rf, sf, m);
+    }
+    const std::vector<double> Wi = inverse(W, nf1);
+    const double jac = f == 0 ? std::sqrt(3.0) : 1.0;
+    // quadrature values of psi_m on the face and of phi_k in face coordinates
+    std::vector<double> psi((size_t)36 * nu), phi((size_t)36 * nf1);
+    for (int g = 0; g < 36; g++) {
+      const double rf = tet_tables::tri_rule12[g][0], sf = tet_tables::tri_rule12[g][1];
+      double x[3];
+      tet_face_point(f, rf, sf, x);
+      for (int m = 0; m < nu; m++) psi[g + 36 * (size_t)m] = tet_mode(x[0], x[1], x[2], m);
+      for (int k = 0; k < nf1; k++) phi[g + 36 * (size_t)k] = tri_mode(rf, sf, k);
+    }
+    for (int q = 0; q < nf1; q++) {
+      for (int m = 0; m < nu; m++) {
+        double acc = 0.0;
+        for (int g = 0; g < 36; g++) {
+          double l = 0.0;
+          for (int k = 0; k < nf1; k++) l += Wi[k + (size_t)nf1 * q] * phi[g + 36 * (size_t)k];
+          acc += tet_tables::tri_rule12[g][2] * psi[g + 36 * (size_t)m] * l;
+        }
+        coef[m] = acc * jac;
+      }
+      const int fp = f * nf1 + q;
+      for (int j = 0; j < nu; j++) {
+        double v = 0.0;
+        for (int m = 0; m < nu; m++) v += coef[m] * tet_mode(loc_upts[3 * j], loc_upts[3 * j + 1], loc_upts[3 * j + 2], m);
+        opp_3[j + (size_t)nu * fp] = v;
+      }
+    }
+  }
+}
+
+// Linear tet shape functions (src/eles_tets.cpp:862-880, 909-935)
+void eles_tets::eval_shape(const double* loc, double* N, double* dN) const {
+  N[0] = -0.5 * (loc[0] + loc[1] + loc[2] + 1.0);
+  N[1] = 0.5 * (loc[0] + 1.0);
+  N[2] = 0.5 * (loc[1] + 1.0);
+  N[3] = 0.5 * (loc[2] + 1.0);
+  for (int m = 0; m < 12; m++) dN[m] = 0.0;
+  for (int k = 0; k < 3; k++) {
+    dN[0 + 4 * k] = -0.5;
+    dN[k + 1 + 4 * k] = 0.5;
+  }
+}
+
+// face f is the face opposite vertex f for f = 0 (the slanted face) and the
+// coordinate planes r, s, t = -1 for f = 1, 2, 3 (src/geometry.cpp:3156-3185)
+const int* eles_tets::face_nodes(int face, int& n) const {
+  static const int fn[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+  n = 3;
   return fn[face];
 }
 

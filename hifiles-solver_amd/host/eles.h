@@ -1,5 +1,6 @@
-// Element classes of the host solver: the reference's eles / eles_quads / eles_hexas
-// surface (include/eles.h, src/eles.cpp, src/eles_quads.cpp, src/eles_hexas.cpp)
+// Element classes of the host solver: the reference's eles / eles_quads / eles_hexas /
+// eles_tets surface (include/eles.h, src/eles.cpp, src/eles_quads.cpp,
+// src/eles_hexas.cpp, src/eles_tets.cpp)
 // reduced to what the residual needs.  They build the solution/flux point sets,
 // the dense element operators opp_0, opp_3, opp_4 in the reference layouts and the
 // static-mesh metrics, hand them to the device context (include/hifr.h) and then
@@ -49,7 +50,7 @@ class eles {
   virtual double eval_div_vcjh_basis(int idx, const double* loc) const = 0;
   virtual void eval_shape(const double* loc, double* N, double* dN) const = 0;
   void set_opp_0();
-  void set_opp_3();
+  virtual void set_opp_3();
   void set_opp_4();
 };
 
@@ -75,6 +76,23 @@ class eles_hexas : public eles {
   double eval_d_nodal_basis(int idx, int cpnt, const double* loc) const override;
   double eval_div_vcjh_basis(int idx, const double* loc) const override;
   void eval_shape(const double* loc, double* N, double* dN) const override;
+};
+
+// Tetrahedra with the DG correction (vcjh_scheme_tet 1, or 0 with c_tet 0), orders
+// 1-3, linear (4-node) shape.  The nodal basis is expressed through the orthonormal
+// (Dubiner) basis and the inverse of its Vandermonde matrix at the solution points.
+class eles_tets : public eles {
+ public:
+  const int* face_nodes(int face, int& n) const override;
+
+ protected:
+  void set_points() override;
+  double eval_nodal_basis(int idx, const double* loc) const override;
+  double eval_d_nodal_basis(int idx, int cpnt, const double* loc) const override;
+  double eval_div_vcjh_basis(int idx, const double* loc) const override;
+  void eval_shape(const double* loc, double* N, double* dN) const override;
+  void set_opp_3() override;
+  std::vector<double> inv_vdm;  // (Nu, Nu): nodal coefficients of each orthonormal mode
 };
 
 // 1-D bases, src/funcs.cpp:57-250, 1503-1546

@@ -17,6 +17,7 @@ namespace {
 // (src/geometry.cpp:1800-1855)
 const int kHexPos[8] = {0, 2, 4, 6, 1, 3, 5, 7};
 const int kQuadPos[4] = {0, 1, 3, 2};
+const int kTetPos[4] = {0, 1, 2, 3};
 
 }  // namespace
 
@@ -59,9 +60,10 @@ std::string read_gambit(const std::string& path, mesh& m) {
   for (int e = 0; e < ne; e++) {
     int id, gtype, nv;
     f >> id >> gtype >> nv;
-    int et = gtype == 4 ? 4 : (gtype == 2 ? 1 : -1);
-    if (et < 0 || (et == 4 && nv != 8) || (et == 1 && nv != 4))
-      return "only linear quads and 8-node hexes are supported";
+    // Gambit element types: 2 quadrilateral, 4 brick, 6 tetrahedron
+    int et = gtype == 4 ? 4 : (gtype == 2 ? 1 : (gtype == 6 ? 2 : -1));
+    if (et < 0 || (et == 4 && nv != 8) || (et != 4 && nv != 4))
+      return "only linear quads, linear tets and 8-node hexes are supported";
     if (m.ele_type == 0) {
       m.ele_type = et;
       m.n_spts = nv;
@@ -69,7 +71,7 @@ std::string read_gambit(const std::string& path, mesh& m) {
     } else if (m.ele_type != et) {
       return "mixed element types are not supported";
     }
-    const int* pos = et == 4 ? kHexPos : kQuadPos;
+    const int* pos = et == 4 ? kHexPos : (et == 1 ? kQuadPos : kTetPos);
     for (int k = 0; k < nv; k++) {
       int v;
       f >> v;
@@ -78,8 +80,9 @@ std::string read_gambit(const std::string& path, mesh& m) {
   }
   if (!f) return "error reading Gambit elements";
   // element groups, then boundary groups (src/geometry.cpp:1186-1290)
-  // Gambit face number -> internal face (hexes; quads: face - 1)
+  // Gambit face number -> internal face (hexes, tets; quads: face - 1)
   static const int kHexFace[7] = {-1, 0, 3, 5, 1, 4, 2};
+  static const int kTetFace[5] = {-1, 3, 2, 0, 1};
   m.bf_ele.clear();
   m.bf_face.clear();
   m.bf_type.clear();
@@ -98,7 +101,9 @@ std::string read_gambit(const std::string& path, mesh& m) {
       f >> ge >> gt >> gf;
       if (!f || ge < 1 || ge > ne) return "error reading Gambit boundary faces of group " + name;
       if (type == 9) continue;  // cyclic: matched by translation
-      const int lf = m.ele_type == 4 ? (gf >= 1 && gf <= 6 ? kHexFace[gf] : -1) : gf - 1;
+      int lf = gf - 1;
+      if (m.ele_type == 4) lf = gf >= 1 && gf <= 6 ? kHexFace[gf] : -1;
+      if (m.ele_type == 2) lf = gf >= 1 && gf <= 4 ? kTetFace[gf] : -1;
       if (lf < 0 || lf >= (m.ele_type == 4 ? 6 : 4)) return "invalid Gambit face number in group " + name;
       m.bf_ele.push_back(ge - 1);
       m.bf_face.push_back(lf);

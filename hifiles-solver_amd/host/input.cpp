@@ -139,6 +139,12 @@ std::string input::read_input_file(const std::string& path) {
   d.get("upts_type_hexa", upts_type_hexa);
   d.get("vcjh_scheme_hexa", vcjh_scheme_hexa);
   d.get("eta_hexa", eta_hexa);
+  d.get("upts_type_tet", upts_type_tet);
+  d.get("fpts_type_tet", fpts_type_tet);
+  d.get("vcjh_scheme_tet", vcjh_scheme_tet);
+  d.get("c_tet", c_tet);
+  d.get("eta_tet", eta_tet);
+  d.get("sparse_tet", sparse_tet);
   d.get("const_src", const_src);
   d.get("hifr_host_only", host_only);
   d.get("hifr_metrics", metric_storage);

@@ -34,6 +34,8 @@ struct input {
   // elements
   int upts_type_quad = 0, vcjh_scheme_quad = 0, upts_type_hexa = 0, vcjh_scheme_hexa = 0;
   double eta_quad = 0., eta_hexa = 0.;
+  int upts_type_tet = 0, fpts_type_tet = 0, vcjh_scheme_tet = 0, sparse_tet = 0;
+  double c_tet = 0., eta_tet = 0.;
   double const_src = 0.;
   // Not a reference key: hifr_host_only 1 builds the host-side element data
   // (operators, metrics, interfaces) without creating the device context.

@@ -33,6 +33,17 @@ int fail(const std::string& msg, char* err, int errlen) {
   return 1;
 }
 
+// dimension and linear shape-node count of quads (1), tets (2) and hexes (4)
+bool mesh_shape(int ele_type, int& nd, int& nspts) {
+  if (ele_type == 1) { nd = 2; nspts = 4; return true; }
+  if (ele_type == 2) { nd = 3; nspts = 4; return true; }
+  if (ele_type == 4) { nd = 3; nspts = 8; return true; }
+  return false;
+}
+
+// section tag of an element type in restart files (src/eles_*.cpp write_restart_info)
+const char* ele_tag(int ele_type) { return ele_type == 4 ? "HEXAS" : ele_type == 2 ? "TETS" : "QUADS"; }
+
 std::string check(hifr_status s, const char* what) {
   if (s == HIFR_OK) return "";
   return std::string(what) + ": " + hifr_status_string(s);
@@ -106,7 +117,7 @@ std::string read_restart(hifr_solver* s, std::vector<double>& u) {
   const int nu = e.n_upts_per_ele, N = e.n_eles, nf = e.n_fields;
   f >> s->time;
   std::string line;
-  const char* tag = e.ele_type == 4 ? "HEXAS" : "QUADS";
+  const char* tag = ele_tag(e.ele_type);
   while (std::getline(f, line) && line != tag) {}
   int order_rest = -1, nu_rest = -1;
   std::getline(f, line);
@@ -116,13 +127,16 @@ std::string read_restart(hifr_solver* s, std::vector<double>& u) {
   f >> nu_rest;
   std::getline(f, line);
   std::getline(f, line);
-  for (int i = 0; i <= order_rest; i++) {
+  if (order_rest != e.order || nu_rest != nu)
+    return "restart file has other solution points (interpolating restarts are not supported)";
+  // quads/hexes list the 1-D points, tets every solution point (src/eles_tets.cpp:749-780)
+  const std::vector<double>& pts = e.ele_type == 2 ? e.loc_upts : e.loc_1d_upts;
+  for (size_t i = 0; i < pts.size(); i++) {
     double x;
     f >> x;
-    if (order_rest != e.order || std::fabs(x - e.loc_1d_upts[i]) > 1e-14)
+    if (std::fabs(x - pts[i]) > 1e-14)
       return "restart file has other solution points (interpolating restarts are not supported)";
   }
-  if (nu_rest != nu) return "restart point count mismatch";
   while (std::getline(f, line) && line != "n_eles") {}
   int nread = 0;
   f >> nread;
@@ -159,6 +173,7 @@ std::string setup_solver(hifr_solver* s) {
   const mesh& m = s->msh;
   if (m.ele_type == 4) s->el.reset(new eles_hexas());
   else if (m.ele_type == 1) s->el.reset(new eles_quads());
+  else if (m.ele_type == 2) s->el.reset(new eles_tets());
   else return "mesh element type not supported";
   eles& e = *s->el;
   e.ele_type = m.ele_type;
@@ -230,8 +245,7 @@ int hifr_solver_from_arrays(const char* deck_text, int ele_type, int n_verts, co
   if (!e.empty()) return fail(e, err, errlen);
   mesh& m = s->msh;
   m.ele_type = ele_type;
-  m.n_dims = ele_type == 4 ? 3 : 2;
-  m.n_spts = ele_type == 4 ? 8 : 4;
+  if (!mesh_shape(ele_type, m.n_dims, m.n_spts)) return fail("element type not supported", err, errlen);
   m.n_verts = n_verts;
   m.n_eles = n_eles;
   m.xv.assign(xv, xv + (size_t)n_verts * m.n_dims);
@@ -251,8 +265,7 @@ int hifr_solver_from_arrays_bc(const char* deck_text, int ele_type, int n_verts,
   if (!e.empty()) return fail(e, err, errlen);
   mesh& m = s->msh;
   m.ele_type = ele_type;
-  m.n_dims = ele_type == 4 ? 3 : 2;
-  m.n_spts = ele_type == 4 ? 8 : 4;
+  if (!mesh_shape(ele_type, m.n_dims, m.n_spts)) return fail("element type not supported", err, errlen);
   m.n_verts = n_verts;
   m.n_eles = n_eles;
   m.xv.assign(xv, xv + (size_t)n_verts * m.n_dims);
@@ -274,6 +287,7 @@ int hifr_solver_box(const char* deck_text, int ele_type, const int* n, const dou
   std::unique_ptr<hifr_solver> s(new hifr_solver());
   std::string e = read_text_deck(deck_text, s->in);
   if (!e.empty()) return fail(e, err, errlen);
+  if (ele_type != 1 && ele_type != 4) return fail("the box generator builds quads and hexes", err, errlen);
   make_box(ele_type, n, l, o, digits, s->msh);
   e = setup_solver(s.get());
   if (!e.empty()) return fail(e, err, errlen);
@@ -375,10 +389,19 @@ int hifr_solver_write_restart(hifr_solver* s, const char* path) {
   if (!f) return 1;
   f.precision(15);
   f << s->time << "\n";
-  f << (e.ele_type == 4 ? "HEXAS" : "QUADS") << "\nOrder\n" << e.order << "\n";
-  f << "Number of solution points per " << (e.ele_type == 4 ? "hexahedral" : "quadrilateral") << " element\n"
-    << nu << "\nLocation of solution points in 1D\n";
-  for (double x : e.loc_1d_upts) f << x << " ";
+  f << ele_tag(e.ele_type) << "\nOrder\n" << e.order << "\n";
+  if (e.ele_type == 2) {
+    // src/eles_tets.cpp:782-800
+    f << "Number of solution points per element\n" << nu << "\nLocation of solution points in tetrahedral elements\n";
+    for (int j = 0; j < nu; j++) {
+      for (int k = 0; k < 3; k++) f << e.loc_upts[3 * j + k] << " ";
+      f << "\n";
+    }
+  } else {
+    f << "Number of solution points per " << (e.ele_type == 4 ? "hexahedral" : "quadrilateral") << " element\n"
+      << nu << "\nLocation of solution points in 1D\n";
+    for (double x : e.loc_1d_upts) f << x << " ";
+  }
   f << "\nn_eles\n" << N << "\nele2global_ele array\n";
   for (int i = 0; i < N; i++) f << i << " ";
   f << "\ndata\n";

@@ -32,8 +32,10 @@ typedef enum {
   HIFR_ERR_NAN = 6           /* NaN in the residual (src/eles.cpp:1634-1638) */
 } hifr_status;
 
-/* Element types use the reference's numbering (src/eles_*.cpp setup_ele_type_specific). */
-enum { HIFR_QUAD = 1, HIFR_HEX = 4 };
+/* Element types use the reference's numbering (src/eles_*.cpp setup_ele_type_specific).
+ * Quads and hexes use sum-factorised operators (orders 1-6 / 1-5); tets use the
+ * dense operators (orders 1-3, DG correction). */
+enum { HIFR_QUAD = 1, HIFR_TET = 2, HIFR_HEX = 4 };
 
 /* Arithmetic of the residual kernels.  HIFR_MIXED (no counterpart in the
  * reference): state, RK register, detJ, face normals / tdA and the RK update in
@@ -55,7 +57,7 @@ enum {
 /* Physical and numerical parameters after non-dimensionalisation
  * (run_input after input::setup_params, src/input.cpp:345-515). */
 typedef struct {
-  int ele_type;              /* HIFR_QUAD or HIFR_HEX */
+  int ele_type;              /* HIFR_QUAD, HIFR_TET or HIFR_HEX */
   int order;                 /* polynomial order P */
   int n_eles;
   int viscous;               /* 0 Euler, 1 Navier-Stokes (LDG) */

@@ -60,7 +60,8 @@ def _kinetic_energy(u, detj, w3):
 
 
 @pytest.mark.parametrize("name", ["hex_p3_ns_tgv_2x2x2", "hex_p3_ns_perturbed_3x3x3", "hex_p4_ns_2x2x2",
-                                  "hex_p3_les_wale_perturbed_3x3x3", "hex_p3_channel_walls_3x3x2"])
+                                  "hex_p3_les_wale_perturbed_3x3x3", "hex_p3_channel_walls_3x3x2",
+                                  "tet_p3_ns_perturbed_2x2x2", "tet_p3_channel_wale_walls_2x2x2"])
 def test_mixed_residual_close_to_oracle(require_gpu, name):
     g = cases.load_golden(name)
     s = cases.solver_of(g, hifr_precision=1)

@@ -31,12 +31,15 @@ def test_residual_matches_reference_and_oracle(require_gpu, name, metrics):
     g = cases.load_golden(name)
     s, o = _pair(g, hifr_metrics=metrics)
     print("case", name, "metrics", metrics, flush=True)
-    # box meshes are affine: auto keeps one metric set per element; meshes with
-    # perturbed interior vertices have non-affine elements and keep per-point metrics
+    # box meshes are affine: auto keeps one metric set per element; hex and quad
+    # meshes with perturbed interior vertices have non-affine elements and keep
+    # per-point metrics (linear tets are affine whatever their vertices)
     J = s.get("JGinv_upts")
     affine = np.abs(J - J[:, :, :1, :]).max() < 1e-13 * np.abs(J).max()
-    if "perturbed" in name:
+    if "perturbed" in name and int(g["ele_type"]) != 2:
         assert not affine
+    if int(g["ele_type"]) == 2:
+        assert affine
     assert s.metrics_per_element() == (metrics == 0 and affine)
     s.set_state(g["u0"])
     div = s.residual()

@@ -17,13 +17,23 @@ ROOT = os.path.dirname(os.path.dirname(cases.GOLDEN))
 
 @pytest.mark.parametrize("name", cases.golden_names())
 def test_host_setup_matches_oracle(name):
+    """Quads and hexes: bit-identical (both sides evaluate the same Lagrange products
+    in the same order).  Tets: the host builds its operators independently of the
+    oracle (iterative orthonormal Jacobi recurrence, Gauss-Jordan inverse with
+    partial pivoting, against the oracle's recursive Jacobi and full-pivoting
+    inverse), so they agree to rounding: 1e-14 of each operator's max magnitude."""
     g = cases.load_golden(name)
     s = cases.solver_of(g, host_only=True)
     o = cases.oracle_of(g)
+    tet = int(g["ele_type"]) == 2
     for what in ("opp_0", "opp_3", "opp_4", "detjac_upts", "JGinv_upts", "tdA_fpts", "norm_fpts", "pos_upts"):
         a, b = s.get(what), o.get(what)
         assert a.shape == b.shape, what
-        assert np.array_equal(a, b), what
+        if tet and what.startswith("opp"):
+            err = np.abs(a - b).max() / np.abs(b).max()
+            assert err < 1e-14, (what, err)
+        else:
+            assert np.array_equal(a, b), what
     l, r = s.interfaces()
     lo, ro = o.face_pairs()
     assert set(map(tuple, map(sorted, zip(l, r)))) == set(map(tuple, map(sorted, zip(lo, ro))))
