@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of the FR residual + RK45 update on one MI355X, FP64 and mixed precision.
 
-Workload (the headline configuration): 3-D compressible Navier-Stokes,
+Default workload (the headline configuration): 3-D compressible Navier-Stokes,
 Taylor-Green initial field (Mach 0.1, Re 1600, Pr 0.71) plus a seeded 1e-3
 perturbation, periodic cube [0,2pi]^3 with 32^3 hexes at P=3 (64 solution points
 per element, 10,485,760 DOF), Rusanov + LDG, RK45, dt = 1e-3, FP64.  A step is one
@@ -9,6 +9,13 @@ RK45 step (5 stages, i.e. 5 residual evaluations + updates); the metric counts
 DOF x stages per second.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+                  [--workload tgv32 | tgv48-p4 | tet-channel]
+
+Other workloads (same JSON line, for the record of each configuration):
+  tgv48-p4     the same flow on 48^3 hexes at P=4 (69.1 M DOF), dt = 5e-4;
+  tet-channel  24 x 12 x 12 hexes split into 6 tets each (20,736 tets, P=3,
+               2.07 M DOF), 10% vertex perturbation, isothermal no-slip walls,
+               WALE, Mach 0.2, Re_bulk 2800, dt = 1e-3.
 
 Prints one JSON line.  `value` is the FP64 path; `mixed` holds the same
 measurement of the mixed-precision path (FP32 arithmetic and traces, FP64 state
@@ -40,9 +47,9 @@ HBM_PEAK_GBS = 8000.0
 TWO_PI = 6.2831853071795862
 
 
-def deck(order, n_steps, metrics=0, precision=0):
+def deck(order, n_steps, metrics=0, precision=0, dt=1e-3):
     return hifr.deck_text(hifr_metrics=metrics, hifr_precision=precision,
-                          viscous=1, order=order, vcjh_scheme_hexa=1, dt=1e-3, adv_type=3, tau=1.0, pen_fact=0.5,
+                          viscous=1, order=order, vcjh_scheme_hexa=1, dt=dt, adv_type=3, tau=1.0, pen_fact=0.5,
                           gamma=1.4, prandtl=0.71, Mach_free_stream=0.1, nx_free_stream=1.0,
                           Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, fix_vis=1,
                           dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI, n_steps=n_steps)
@@ -54,12 +61,44 @@ def tgv_state(pos, seed=1234):
     return u * (1.0 + frmesh.seeded_perturbation(u.shape, 1e-3, seed))
 
 
-def algorithmic_bytes(order, nd=3, per_element=False, mixed=False):
+WORKLOADS = {
+    "tgv32": dict(kind="tgv", n=32, order=3, dt=1e-3),
+    "tgv48-p4": dict(kind="tgv", n=48, order=4, dt=5e-4),
+    "tet-channel": dict(kind="tet", order=3, dt=1e-3),
+}
+
+
+def describe(w, n, order):
+    if w["kind"] == "tet":
+        return ("3-D Navier-Stokes + WALE plane channel (Mach 0.2, Re_bulk 2800), 24x12x12 hexes split into "
+                "20,736 tets with 10%% vertex perturbation, P=%d, isothermal no-slip walls, Rusanov+LDG, "
+                "RK45 dt=%g, FP64" % (order, w["dt"]))
+    return ("3-D Navier-Stokes Taylor-Green (Mach 0.1, Re 1600), %d^3 periodic hexes, P=%d, Rusanov+LDG, "
+            "RK45 dt=%g, FP64" % (n, order, w["dt"]))
+
+
+def make_solver(w, n, order, steps, per_point, precision):
+    """Solver with the workload's initial state uploaded; the oracle's inputs."""
+    prec = int(precision == "mixed")
+    if w["kind"] == "tet":
+        import cases
+        g = cases.channel_tet_case(order=order, dt=w["dt"])
+        s = cases.solver_of(g, hifr_metrics=int(per_point), hifr_precision=prec)
+        s.set_state(cases.channel_tet_state(g, s.get("pos_upts")))
+        return s, g
+    s = hifr.Solver.box(deck(order, steps, int(per_point), prec, w["dt"]), hifr.HEX, (n, n, n), (TWO_PI,) * 3,
+                        (0.0,) * 3)
+    s.set_state(tgv_state(s.get("pos_upts")))
+    return s, None
+
+
+def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
     """HBM bytes per element that the two kernels of a Navier-Stokes stage must
     move, counting every array once:
       gradient-trace kernel: read u, neighbour u_f, JGinv, detJ, face normals,
                              neighbour index; write the viscous normal flux
-                             traces F_v.n (nf per flux point);
+                             traces F_v.n (nf per flux point; with LES the SGS
+                             normal flux is folded into the same traces);
       flux/RK kernel:        read u, RK register, neighbour u_f, own and neighbour
                              F_v.n, JGinv, detJ, face normal + tdA, neighbour
                              index; write u, RK register, next u_f.
@@ -67,46 +106,59 @@ def algorithmic_bytes(order, nd=3, per_element=False, mixed=False):
     FP32 on the mixed path; the neighbour index is int32.  Metrics are per
     solution point / flux point, or with per_element (every element affine) one
     JGinv + detJ per element and one normal + tdA per face."""
-    n = order + 1
-    nu, nfp, nf = n ** nd, 2 * nd * n ** (nd - 1), nd + 2
+    nf = nd + 2
     t = 4 if mixed else 8
     nv = 1 if per_element else nu
     vol = t * nd * nd * nv + 8 * nv
-    fpts = 2 * nd if per_element else nfp
+    fpts = nfaces if per_element else nfp
     grad = 8 * nu * nf + t * nfp * nf + vol + 8 * fpts * nd + t * nfp * nf + 4 * nfp
     rk = 8 * 4 * nu * nf + t * nfp * nf + t * 2 * nfp * nf + vol + 8 * fpts * (nd + 1) + t * nfp * nf + 4 * nfp
     return grad, rk
 
 
-# HBM traffic per element of each stage kernel at 32^3, P=3, from rocprofv3 --pmc
-# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) per dispatch, the factors read off a
-# 1 GiB copy kernel (profiles/fr_stage_tgv32_p3_fp64.md); keyed by (order,
-# per-element metrics).
-MEASURED_TRAFFIC = {(3, False): {"grad": (227408.0 * 2.0 + 122880.0) * 1024 / 32768,
-                                 "rk": (388554.0 * 2.0 + 297642.7) * 1024 / 32768}}
+# HBM traffic per element of each stage kernel from rocprofv3 --pmc FETCH_SIZE (x2.0)
+# and WRITE_SIZE (x1.0), KiB per dispatch, the factors read off a 1 GiB copy kernel;
+# keyed by (workload, per-element metrics, precision).  Sources: the summaries
+# under profiles/ named for each kernel and workload.
+MEASURED_KIB = {("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (227408.0, 122880.0),
+                                           "rk": (388554.0, 297642.7)}}
 
 
-def survey_model_bytes(order, nd=3):
+def measured_traffic(workload, per_element, precision):
+    m = MEASURED_KIB.get((workload, per_element, precision))
+    if not m:
+        return None
+    return {k: (m[k][0] * 2.0 + m[k][1]) * 1024 / m["n_eles"] for k in ("grad", "rk")}
+
+
+def survey_model_bytes(nu, nfp, nd=3, les=False):
     """Bytes per element of one Navier-Stokes stage in the three-phase model
     (u -> u_f; u + u_f -> grad u_f; u + u_f + grad u_f -> flux, div, RK) that
     counts every operand once per phase; this schedule moves fewer bytes."""
-    n = order + 1
-    nu, nfp, nf = n ** nd, 2 * nd * n ** (nd - 1), nd + 2
-    return 8 * (3 * nu * nf + nu * nf + 2 * nu * nf + 3 * nfp * nf + 2 * nfp * nf * nd + 2 * nu * (1 + nd * nd)
-                + 2 * nfp * (nd + 1))
+    nf = nd + 2
+    b = 8 * (3 * nu * nf + nu * nf + 2 * nu * nf + 3 * nfp * nf + 2 * nfp * nf * nd + 2 * nu * (1 + nd * nd)
+             + 2 * nfp * (nd + 1))
+    return b + (8 * 2 * nfp * nf * nd if les else 0)
 
 
-def cpu_baseline(s, order):
+def cpu_baseline(s, w, order, g):
     from oracle import pyoracle as po
-    xv = s.get("xv").reshape(-1, 3)
-    c2v = s.get("c2v").reshape(-1, 8).astype(np.int32)
     threads = min(16, os.cpu_count() or 1)
-    od = po.default_deck(equation=0, viscous=1, order=order, adv_type=3, dt=1e-3, tau=1.0, pen_fact=0.5,
-                         gamma=1.4, prandtl=0.71, Mach_free_stream=0.1, nx_free_stream=1.0,
-                         Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, vcjh_scheme=1,
-                         dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI)
-    o = po.Oracle(od, 4, xv, c2v, threads=threads)
-    o.set_state(tgv_state(o.get("pos_upts")))
+    if w["kind"] == "tet":
+        import cases
+        o = cases.oracle_of(g, threads=threads)
+        o.set_state(cases.channel_tet_state(g, o.get("pos_upts")))
+        what = "the same 20,736-tet P=%d channel workload" % order
+    else:
+        xv = s.get("xv").reshape(-1, 3)
+        c2v = s.get("c2v").reshape(-1, 8).astype(np.int32)
+        od = po.default_deck(equation=0, viscous=1, order=order, adv_type=3, dt=w["dt"], tau=1.0, pen_fact=0.5,
+                             gamma=1.4, prandtl=0.71, Mach_free_stream=0.1, nx_free_stream=1.0,
+                             Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, vcjh_scheme=1,
+                             dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI)
+        o = po.Oracle(od, 4, xv, c2v, threads=threads)
+        o.set_state(tgv_state(o.get("pos_upts")))
+        what = "the same %d^3 P=%d Navier-Stokes workload" % (round(s.n_eles ** (1 / 3)), order)
     o.zero_rk_register()
     o.run_steps(1)  # warm-up
     t0 = time.perf_counter()
@@ -114,8 +166,8 @@ def cpu_baseline(s, order):
     dt = time.perf_counter() - t0
     dof = o.nu * o.n_eles * o.nf
     return {"value": round(dof * 5 / dt / 1e9, 4), "unit": "GDOF-stage/s", "cores": threads, "kind": "port",
-            "sample": "1 RK45 step (5 residual stages) of the same 32^3 P=3 Navier-Stokes workload, dense "
-                      "reference-order operators, %d OpenMP threads, %.2f s" % (threads, dt)}
+            "sample": "1 RK45 step (5 residual stages) of %s, dense reference-order operators, %d OpenMP "
+                      "threads, %.2f s" % (what, threads, dt)}
 
 
 def main():
@@ -123,8 +175,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=32)
-    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="tgv32")
+    ap.add_argument("--n", type=int, default=0, help="hexes per direction (TGV workloads)")
+    ap.add_argument("--order", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--per-point-metrics", action="store_true",
                     help="keep per-point metrics on the device even though the mesh is affine")
@@ -147,27 +200,30 @@ def main():
         print(json.dumps({"copy_bytes_read": nd * 8, "copy_bytes_written": nd * 8, "ms_per_copy": ms.value,
                           "GBs": 2 * nd * 8 / (ms.value * 1e-3) / 1e9}), flush=True)
         return
-    n, order = args.n, args.order
-    fp64 = measure(args, "fp64")
-    mixed = measure(args, "mixed")
+    w = WORKLOADS[args.workload]
+    n = args.n if args.n else w.get("n", 0)
+    order = args.order if args.order else w["order"]
+    fp64 = measure(args, w, n, order, "fp64")
+    mixed = measure(args, w, n, order, "mixed")
     s, r = fp64["solver"], fp64
     l2, mx = rel_errors(mixed["solver"].get("disu_upts"), s.get("disu_upts"))
-    traffic = MEASURED_TRAFFIC.get((order, r["per_element"])) if n == 32 else None
+    traffic = measured_traffic(args.workload, r["per_element"], "fp64") if (n, order) == (w.get("n", 0), w["order"]) \
+        else None
+    kname = "fr_dense_kernel" if w["kind"] == "tet" else "fr_tensor_kernel"
     rec = {
         "metric": METRIC, "value": round(r["value"], 4), "unit": "GDOF-stage/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "3-D Navier-Stokes Taylor-Green (Mach 0.1, Re 1600), %d^3 periodic hexes, P=%d, "
-                               "Rusanov+LDG, RK45 dt=1e-3, FP64" % (n, order),
+        "config": {"workload": describe(w, n, order), "name": args.workload,
                    "n_eles": s.n_eles, "order": order, "dof": r["dof"], "stages_per_step": 5,
                    "metrics": "per element (affine elements)" if r["per_element"] else "per point"},
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic["rk"] * s.n_eles) if traffic else None,
-                     "kernel": "fr_tensor_kernel ADVANCE (flux, divergence, RK update, next traces)",
+                     "kernel": "%s ADVANCE (flux, divergence, RK update, next traces)" % kname,
                      "bytes_per_launch": r["rk_b"] * s.n_eles, "bytes_per_element": r["rk_b"],
                      "avg_launch_ms": round(r["avg_rk_ms"], 5)},
-        "stage": stage_block(r, order, s, traffic),
+        "stage": stage_block(r, s, traffic, w),
         "mixed": {"value": round(mixed["value"], 4), "unit": "GDOF-stage/s",
                   "speedup_vs_fp64": round(mixed["value"] / r["value"], 3),
                   "ms_per_step": round(mixed["wall"] * 1e3 / args.steps, 4),
@@ -176,12 +232,12 @@ def main():
                   "err_after_steps": args.warmup + args.steps,
                   "roofline": {"achieved": round(mixed["achieved"], 1), "frac": round(mixed["achieved"] / HBM_PEAK_GBS, 4),
                                "bytes_per_element": mixed["rk_b"], "avg_launch_ms": round(mixed["avg_rk_ms"], 5)},
-                  "stage": stage_block(mixed, order, mixed["solver"], None)},
+                  "stage": stage_block(mixed, mixed["solver"], None, w)},
         "residual_L1": [float("%.6e" % v) for v in r["norms"]],
         "setup_s": round(r["setup"], 2),
     }
     if not args.no_cpu_baseline:
-        rec["cpu_baseline"] = cpu_baseline(s, order)
+        rec["cpu_baseline"] = cpu_baseline(s, w, order, fp64["g"])
     else:
         rec["cpu_baseline"] = None
     print(json.dumps(rec), flush=True)
@@ -194,12 +250,9 @@ def rel_errors(a, b):
     return np.sqrt((d ** 2).sum(0) / (ref ** 2).sum(0)), np.abs(d).max(0) / np.abs(ref).max(0)
 
 
-def measure(args, precision):
-    n, order = args.n, args.order
+def measure(args, w, n, order, precision):
     t_setup = time.perf_counter()
-    s = hifr.Solver.box(deck(order, args.steps, int(args.per_point_metrics), int(precision == "mixed")),
-                        hifr.HEX, (n, n, n), (TWO_PI,) * 3, (0.0,) * 3)
-    s.set_state(tgv_state(s.get("pos_upts")))
+    s, g = make_solver(w, n, order, args.steps, args.per_point_metrics, precision)
     t_setup = time.perf_counter() - t_setup
     s.run(args.warmup)
     s.synchronize()
@@ -212,26 +265,30 @@ def measure(args, precision):
         raise SystemExit("non-finite residual (%s)" % precision)
     dof = s.nu * s.n_eles * s.nf
     per_element = s.metrics_per_element()
-    grad_b, rk_b = algorithmic_bytes(order, per_element=per_element, mixed=precision == "mixed")
+    nfaces = 4 if w["kind"] == "tet" else 2 * s.nd
+    grad_b, rk_b = algorithmic_bytes(s.nu, s.nfp, nfaces, s.nd, per_element=per_element,
+                                     mixed=precision == "mixed")
     avg_rk_ms = kms[2] / kl[2]
     avg_grad_ms = kms[1] / kl[1] if kl[1] else 0.0
-    return {"solver": s, "value": dof * 5 * args.steps / wall / 1e9, "wall": wall, "dof": dof, "norms": norms,
-            "per_element": per_element, "grad_b": grad_b, "rk_b": rk_b, "avg_rk_ms": avg_rk_ms,
+    return {"solver": s, "g": g, "value": dof * 5 * args.steps / wall / 1e9, "wall": wall, "dof": dof,
+            "norms": norms, "per_element": per_element, "grad_b": grad_b, "rk_b": rk_b, "avg_rk_ms": avg_rk_ms,
             "avg_grad_ms": avg_grad_ms, "achieved": rk_b * s.n_eles / (avg_rk_ms * 1e-3) / 1e9,
             "total_ms": total_ms, "setup": t_setup}
 
 
-def stage_block(r, order, s, traffic):
+def stage_block(r, s, traffic, w):
     stage_ms = r["avg_rk_ms"] + r["avg_grad_ms"]
     stage_bytes = (r["grad_b"] + r["rk_b"]) * s.n_eles
+    model = survey_model_bytes(s.nu, s.nfp, s.nd, les=w["kind"] == "tet")
     return {"avg_grad_kernel_ms": round(r["avg_grad_ms"], 5), "avg_rk_kernel_ms": round(r["avg_rk_ms"], 5),
             "grad_kernel_bytes_per_element": r["grad_b"],
             "grad_kernel_traffic": round(traffic["grad"] * s.n_eles) if traffic else None,
             "algorithmic_bytes": stage_bytes,
             "achieved_GBs": round(stage_bytes / (stage_ms * 1e-3) / 1e9, 1),
-            "three_phase_model_bytes_per_dof": round(survey_model_bytes(order) / s.nu / s.nf, 2),
-            "three_phase_model_GBs": round(survey_model_bytes(order) * s.n_eles / (stage_ms * 1e-3) / 1e9, 1),
+            "three_phase_model_bytes_per_dof": round(model / s.nu / s.nf, 2),
+            "three_phase_model_GBs": round(model * s.n_eles / (stage_ms * 1e-3) / 1e9, 1),
             "event_total_ms": round(r["total_ms"], 3)}
+
 
 if __name__ == "__main__":
     main()

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libhifr.so")
 _LIB = None
 
-HEX, QUAD = 4, 1
+HEX, QUAD, TET = 4, 1, 2
 
 _GET = {"disu_upts": 0, "div_tconf_upts": 1, "disu_fpts": 2, "grad_disu_fpts": 3, "detjac_upts": 4,
         "pos_upts": 5, "opp_0": 6, "opp_3": 7, "opp_4": 8, "JGinv_upts": 9, "tdA_fpts": 10,
@@ -58,6 +58,16 @@ def lib():
         L.hifr_run.argtypes = [vp, C.c_int, C.c_int, C.c_double]
         L.hifr_status_string.argtypes = [C.c_int]
         L.hifr_status_string.restype = C.c_char_p
+        fp = C.POINTER(C.c_float)
+        L.hifr_mlp_create.argtypes = [C.POINTER(vp), C.c_ulonglong]
+        L.hifr_mlp_destroy.argtypes = [vp]
+        L.hifr_mlp_set_params.argtypes = [vp, fp]
+        L.hifr_mlp_get_params.argtypes = [vp, fp]
+        L.hifr_mlp_set_normalization.argtypes = [vp, fp, fp, fp, fp, C.c_float]
+        L.hifr_mlp_loss_grad.argtypes = [vp, C.c_long, fp, fp, fp, fp]
+        L.hifr_mlp_set_data.argtypes = [vp, C.c_long, fp, fp]
+        L.hifr_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, fp]
+        L.hifr_mlp_predict.argtypes = [vp, C.c_long, fp, fp]
         _LIB = L
     return _LIB
 
@@ -224,3 +234,90 @@ class Solver:
 
     def synchronize(self):
         lib().hifr_synchronize(self.ctx())
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if shape is not None and a.shape != shape:
+        raise ValueError("expected shape %s, got %s" % (shape, a.shape))
+    return a
+
+
+class MLP:
+    """The learned eddy-viscosity network of include/hifr_sgs.h (device-resident:
+    training, gradients and inference run in the library's HIP kernels)."""
+    N_IN, N_PARAMS = 10, 9089
+
+    def __init__(self, seed=0):
+        self._h = C.c_void_p()
+        self._check(lib().hifr_mlp_create(C.byref(self._h), int(seed)), "hifr_mlp_create")
+
+    @staticmethod
+    def _check(rc, what):
+        if rc != 0:
+            raise RuntimeError("%s: %s" % (what, lib().hifr_status_string(rc).decode()))
+
+    def close(self):
+        if self._h:
+            lib().hifr_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_params(self, p):
+        p = _f32(p, (self.N_PARAMS,))
+        self._check(lib().hifr_mlp_set_params(self._h, _fp(p)), "hifr_mlp_set_params")
+
+    def params(self):
+        out = np.zeros(self.N_PARAMS, dtype=np.float32)
+        self._check(lib().hifr_mlp_get_params(self._h, _fp(out)), "hifr_mlp_get_params")
+        return out
+
+    def set_normalization(self, mean, scale, lo, hi, out_scale):
+        a = [_f32(v, (self.N_IN,)) for v in (mean, scale, lo, hi)]
+        self._check(lib().hifr_mlp_set_normalization(self._h, *[_fp(v) for v in a], float(out_scale)),
+                    "hifr_mlp_set_normalization")
+
+    def loss_grad(self, x, t):
+        x = _f32(x)
+        t = _f32(t)
+        n = t.size
+        if x.shape != (n, self.N_IN):
+            raise ValueError("x must be (n, 10)")
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(self.N_PARAMS, dtype=np.float32)
+        self._check(lib().hifr_mlp_loss_grad(self._h, n, _fp(x), _fp(t), _fp(loss), _fp(g)), "hifr_mlp_loss_grad")
+        return float(loss[0]), g
+
+    def set_data(self, x, t):
+        x = _f32(x)
+        t = _f32(t)
+        if x.shape != (t.size, self.N_IN):
+            raise ValueError("x must be (n, 10)")
+        self._check(lib().hifr_mlp_set_data(self._h, t.size, _fp(x), _fp(t)), "hifr_mlp_set_data")
+
+    def train(self, epochs, batch, lr):
+        out = np.zeros(epochs, dtype=np.float32)
+        self._check(lib().hifr_mlp_train(self._h, int(epochs), int(batch), float(lr), _fp(out)), "hifr_mlp_train")
+        return out
+
+    def predict(self, x):
+        x = _f32(x)
+        n = x.shape[0]
+        if x.shape != (n, self.N_IN):
+            raise ValueError("x must be (n, 10)")
+        y = np.zeros(n, dtype=np.float32)
+        self._check(lib().hifr_mlp_predict(self._h, n, _fp(x), _fp(y)), "hifr_mlp_predict")
+        return y

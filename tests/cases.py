@@ -71,6 +71,8 @@ def product_deck(g, host_only=False, **over):
     kw.update(phys_of(g))
     if ele_type == 4:
         kw["vcjh_scheme_hexa"] = int(g["vcjh"])
+    elif ele_type == 2:
+        kw["vcjh_scheme_tet"] = int(g["vcjh"])
     else:
         kw["vcjh_scheme_quad"] = int(g["vcjh"])
     if int(g.get("les", 0)):
@@ -106,3 +108,37 @@ def field_rel_err(a, b):
         ak = a[..., k] if a.ndim == 3 else a[:, :, k]
         out.append(float(np.abs(ak - bk).max() / max(np.abs(bk).max(), 1e-300)))
     return np.array(out)
+
+
+def channel_tet_case(n=(24, 12, 12), order=3, perturb=0.1, mesh_seed=2024, dt=1e-3, steps=200, sgs_model=1,
+                     filter_ratio=2.0):
+    """The unstructured tetrahedral channel: an n[0] x n[1] x n[2] hex box over
+    [0,4pi] x [0,2] x [0,2pi], each hex split into 6 tets, interior vertices moved
+    by up to `perturb` of the y spacing, isothermal no-slip walls at y = 0 and 2,
+    periodic in x and z; Mach 0.2, Re_bulk 2800, WALE (sgs_model 1): a golden-like
+    record for solver_of / oracle_of (channel_tet_state gives the initial field)."""
+    xv, hexes = frmesh.box_hex(n[0], n[1], n[2], 2 * TWO_PI, 2.0, TWO_PI)
+    if perturb > 0:
+        xv = frmesh.perturb_interior(xv, perturb * 2.0 / n[1], seed=mesh_seed)
+    conn = frmesh.hex_to_tets(xv, hexes)
+    walls = frmesh.wall_faces(2, conn, xv, 1, (0.0, 2.0))
+    g = dict(ele_type=2, xv=xv, conn_gambit=conn, order=order, vcjh=1, viscous=1, tau=1.0, pen_fact=0.5,
+             cyc=np.array([2 * TWO_PI, float("inf"), TWO_PI]), dt=dt, kind="rk45", steps=steps, les=1,
+             sgs_model=sgs_model, filter_ratio=filter_ratio,
+             bfaces=np.array([(e, f, 11) for e, f in walls], dtype=np.int32).reshape(-1, 3), mach=0.2,
+             Re=2800.0, T_wall=300.0)
+    return g
+
+
+def channel_tet_state(g, pos, seed=7, noise=5e-2):
+    """Parabolic channel profile (rho = 1, uniform pressure) plus seeded random
+    velocity noise of `noise` times the bulk velocity, damped as y (2 - y) so that
+    the walls stay at rest; the energy carries the perturbed kinetic energy."""
+    u0 = frmesh.channel_state(pos, float(g["mach"]))
+    y = pos[..., 1]
+    dv = frmesh.seeded_perturbation(pos.shape, noise, seed) * (y * (2.0 - y))[..., None]
+    ke0 = 0.5 * (u0[..., 1] ** 2 + u0[..., 2] ** 2 + u0[..., 3] ** 2) / u0[..., 0]
+    u0[..., 1:4] += u0[..., :1] * dv
+    ke1 = 0.5 * (u0[..., 1] ** 2 + u0[..., 2] ** 2 + u0[..., 3] ** 2) / u0[..., 0]
+    u0[..., 4] += ke1 - ke0
+    return u0
