@@ -11,7 +11,8 @@
 namespace hifr {
 
 // vis: 0 Euler, 1 Navier-Stokes with constant viscosity, 2 Sutherland viscosity;
-// aff: metrics per element / per face; les: 0 none, 1 Smagorinsky, 2 WALE (vis 1)
+// aff: metrics per element / per face; les: 0 none, 1 Smagorinsky, 2 WALE (vis 1),
+// 3 learned eddy viscosity with WALE fallback (vis 1, hexes P = 2, 3)
 hipError_t launch_fp64(int nd, int P, int mode, int vis, bool aff, int les, const StageArgsT<double>& a,
                        hipStream_t s);
 hipError_t launch_mixed(int nd, int P, int mode, int vis, bool aff, int les, const StageArgsT<float>& a,

@@ -30,6 +30,9 @@ hipError_t launch_aff(const Args<MIX>& a, hipStream_t s, int les) {
   if constexpr (VIS == 1 && MODE != MODE_TRACE_U) {
     if (les == 1) return launch_les<P, ND, VIS, MODE, AFF, MIX, 1>(a, s);
     if (les == 2) return launch_les<P, ND, VIS, MODE, AFF, MIX, 2>(a, s);
+    // learned closure: hexes P = 2, 3
+    if constexpr (ND == 3 && (P == 2 || P == 3))
+      if (les == 3) return launch_les<P, ND, VIS, MODE, AFF, MIX, 3>(a, s);
   }
   if (les != 0) return hipErrorInvalidValue;
   return launch_les<P, ND, VIS, MODE, AFF, MIX, 0>(a, s);

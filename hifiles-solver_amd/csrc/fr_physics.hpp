@@ -264,6 +264,99 @@ __device__ __forceinline__ void sgs_flux(const T* u, const T (*g)[ND], T detj_vr
   }
 }
 
+// Eddy-viscosity closure in the form of the reference's GPU build
+// (SGS_flux_kernel, src/cuda_kernels.cu:1466-1660, with the velocity and energy
+// gradients of vis_NS_flux, :880-960): velocity gradient vg[i][j] = d v_i / d x_j,
+// deviatoric strain Sd, internal-energy gradient de.  Used by the learned closure
+// and its WALE fallback.
+template <int ND, class T>
+struct SgsPoint {
+  T rho;
+  T vg[ND][ND];
+  T Sd[ND][ND];
+  T de[ND];
+};
+
+template <int ND, class T>
+__device__ __forceinline__ void sgs_point(const T* u, const T (*g)[ND], SgsPoint<ND, T>& o) {
+  constexpr int NF = ND + 2;
+  const T rho = u[0], ir = T(1) / rho;
+  T v[ND], ke = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    v[i] = u[1 + i] * ir;
+    ke += v[i] * v[i];
+  }
+  o.rho = rho;
+#pragma unroll
+  for (int i = 0; i < ND; i++)
+#pragma unroll
+    for (int j = 0; j < ND; j++) o.vg[i][j] = (g[1 + i][j] - v[i] * g[0][j]) * ir;
+  // e = E/rho - |v|^2/2:  de_j = (dE_j - (E/rho) drho_j) / rho - v_i vg[i][j]
+  const T Er = u[NF - 1] * ir;
+#pragma unroll
+  for (int j = 0; j < ND; j++) {
+    T s = (g[NF - 1][j] - Er * g[0][j]) * ir;
+#pragma unroll
+    for (int i = 0; i < ND; i++) s -= v[i] * o.vg[i][j];
+    o.de[j] = s;
+  }
+  T tr = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) tr += o.vg[i][i];
+#pragma unroll
+  for (int i = 0; i < ND; i++)
+#pragma unroll
+    for (int j = 0; j < ND; j++) o.Sd[i][j] = T(0.5) * (o.vg[i][j] + o.vg[j][i]) - (i == j ? tr / T(3) : T(0));
+  (void)ke;
+}
+
+// WALE kinematic eddy viscosity (C_w = 0.5, eps = 1e-10) of the GPU form:
+// nu_t = (C_w delta)^2 (Sq:Sq)^(3/2) / ((Sd:Sd)^(5/2) + (Sq:Sq)^(5/4) + eps) with
+// Sq = sym(vg vg) - tr(vg vg)/3 I.
+template <int ND, class T>
+__device__ __forceinline__ T wale_nu(const SgsPoint<ND, T>& o, T delta) {
+  T num = 0, den = 0, d2 = 0;
+  T Sq[ND][ND];
+#pragma unroll
+  for (int i = 0; i < ND; i++)
+#pragma unroll
+    for (int j = 0; j < ND; j++) {
+      T q = 0;
+#pragma unroll
+      for (int k = 0; k < ND; k++) q += T(0.5) * (o.vg[i][k] * o.vg[k][j] + o.vg[j][k] * o.vg[k][i]);
+      Sq[i][j] = q;
+    }
+#pragma unroll
+  for (int i = 0; i < ND; i++)
+#pragma unroll
+    for (int k = 0; k < ND; k++) d2 += o.vg[i][k] * o.vg[k][i];
+#pragma unroll
+  for (int i = 0; i < ND; i++) Sq[i][i] -= d2 / T(3);
+#pragma unroll
+  for (int i = 0; i < ND; i++)
+#pragma unroll
+    for (int j = 0; j < ND; j++) {
+      num += Sq[i][j] * Sq[i][j];
+      den += o.Sd[i][j] * o.Sd[i][j];
+    }
+  const T cw = T(0.5);
+  return cw * cw * delta * delta * pow(num, T(1.5)) / (pow(den, T(2.5)) + pow(num, T(1.25)) + T(1.e-10));
+}
+
+// SGS flux of an eddy viscosity mu_t (GPU form): M[i][j] = -2 mu_t Sd[i][j],
+// E[j] = -gamma mu_t / Pr_t de[j] with Pr_t = 0.5.
+template <int ND, class T>
+__device__ __forceinline__ void sgs_flux_of(const SgsPoint<ND, T>& o, T mu_t, T gamma, T (*M)[ND], T* E) {
+  const T ek = -gamma * mu_t / T(0.5);
+#pragma unroll
+  for (int j = 0; j < ND; j++) {
+    E[j] = ek * o.de[j];
+#pragma unroll
+    for (int i = 0; i < ND; i++) M[i][j] = T(-2) * mu_t * o.Sd[i][j];
+  }
+}
+
 // Viscous flux projected on w: out[k] = sum_m F_v(k,m) w_m
 template <int ND, class T>
 __device__ __forceinline__ void visc_flux_dot(const ViscT<ND, T>& t, const T* w, T* out) {

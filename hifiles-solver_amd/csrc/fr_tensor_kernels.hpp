@@ -36,6 +36,7 @@
 #include <type_traits>
 
 #include "fr_physics.hpp"
+#include "sgs_mlp.hpp"
 
 namespace hifr {
 
@@ -83,6 +84,12 @@ struct StageArgsT {
   const R* opI;
   const R* opD;
   const R* opL;
+  // learned eddy viscosity (LES 3): network weights, input standardisation and
+  // training range, output scale; mlp_on 0 gives the WALE fallback everywhere
+  mlp::Weights mlpw;
+  float mlp_mean[mlp::NIN], mlp_scale[mlp::NIN], mlp_lo[mlp::NIN], mlp_hi[mlp::NIN];
+  float mlp_out_scale;
+  int mlp_on;
 };
 using StageArgs = StageArgsT<double>;
 
@@ -168,7 +175,9 @@ __device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base
 // VIS: 0 Euler, 1 Navier-Stokes with constant viscosity, 2 with Sutherland's law.
 // AFF: metrics stored per element / per face (every element affine).
 // MIX: mixed precision (FP32 arithmetic on deviations, FP64 state and update).
-// LES: 0 none, 1 Smagorinsky, 2 WALE eddy viscosity (viscous kernels only).
+// LES: 0 none, 1 Smagorinsky, 2 WALE eddy viscosity (viscous kernels only), as the
+// reference CPU build computes them; 3 learned eddy viscosity with the WALE model
+// of the reference GPU build as fallback (hexes).
 template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES>
 __global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK))
 fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
@@ -186,6 +195,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   R* s_g = s_q + S::Q;  // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD)
   R* s_f = s_g + S::G;  // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE)
   R* s_s = s_f + S::F;  // [EPB][NF*ND][NU] transformed SGS flux (TRACE_GRAD, LES)
+  // learned closure: network weights, standardised inputs and outputs per point
+  __shared__ std::conditional_t<LES == 3, mlp::Lds, char> s_mlp;
+  __shared__ float s_x[LES == 3 ? BLOCK * mlp::NIN : 1];
+  __shared__ float s_ml[LES == 3 ? BLOCK : 1];
   const R gam = R(a.ph.gamma);
   RefState<R> ref;
   ref.rho = R(a.ref.rho);
@@ -236,6 +249,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       s_u[(el * NF + k) * NU + p] = MIX ? R(v - ref_of(k)) : R(v);
     }
   }
+  if constexpr (LES == 3 && MODE != MODE_TRACE_U) mlp::stage(a.mlpw, s_mlp, tid, BLOCK);
   __syncthreads();
 
   if constexpr (MODE == MODE_TRACE_U) {
@@ -285,6 +299,98 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       __syncthreads();
     }
 
+    // reference-space corrected gradient of point p of local element el
+    // (reference gradient + LDG correction, src/eles.cpp:1667-1870)
+    [[maybe_unused]] auto point_grad = [&](R (*gr)[ND]) {
+      int pos[ND], lb[ND], fL[ND], fR[ND];
+#pragma unroll
+      for (int d = 0; d < ND; d++) line_of<P, ND>(p, d, pos[d], lb[d], fL[d], fR[d]);
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+          const int st = T::stride(d);
+          R r = 0;
+#pragma unroll
+          for (int q = 0; q < n; q++) r += ops.D[pos[d]][q] * s_u[(el * NF + k) * NU + lb[d] + q * st];
+          r += ops.gL[pos[d]] * s_q[(el * NFP + fL[d]) * NF + k] + ops.gR[pos[d]] * s_q[(el * NFP + fR[d]) * NF + k];
+          gr[k][d] = r;
+        }
+    };
+
+    // ---- learned closure: eddy viscosity of every point before the flux pass.
+    //      Inputs x = (g_ij / |g|, log(delta^2 |g| / nu)) of the velocity gradient
+    //      g; the network runs on the block's points as 16-point MFMA tiles;
+    //      outside the training range (or without a network) the WALE value ----
+    [[maybe_unused]] R nu_t_pt = R(0);
+    [[maybe_unused]] SgsPoint<ND, R> sp;
+    if constexpr (LES == 3 && VISC) {
+      bool use_net = false;
+      R scale_pt = R(0), nu_w = R(0);
+      if (active) {
+        const long vi = vol_idx(p + NU * (long)e, e);
+        constexpr long VOL = AFF ? 1 : NU;
+        R gr[NF][ND];
+        point_grad(gr);
+        const R inv_detj = R(1.0 / a.detj[vi]);
+        R gf[NF][ND];
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+#pragma unroll
+          for (int m = 0; m < ND; m++) {
+            R v = 0;
+#pragma unroll
+            for (int d = 0; d < ND; d++) v += gr[k][d] * (a.jg[(d * ND + m) * NN * VOL + vi] * inv_detj);
+            gf[k][m] = v;
+          }
+        R uo[NF], uu[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) uo[k] = s_u[(el * NF + k) * NU + p];
+        full(uo, uu);
+        sgs_point<ND, R>(uu, gf, sp);
+        const R vref = ND == 3 ? R(8) : R(4);
+        const R delta = R(a.filter_ratio) * pow(R(a.detj[vi]) * vref, R(1) / R(ND)) / R(P + 1);
+        R G2 = 0;
+#pragma unroll
+        for (int i = 0; i < ND; i++)
+#pragma unroll
+          for (int j = 0; j < ND; j++) G2 += sp.vg[i][j] * sp.vg[i][j];
+        const R G = sqrt(G2);
+        R x[mlp::NIN];
+        use_net = a.mlp_on != 0 && G > R(0);
+        if (use_net) {
+#pragma unroll
+          for (int i = 0; i < ND; i++)
+#pragma unroll
+            for (int j = 0; j < ND; j++) x[ND * i + j] = sp.vg[i][j] / G;
+          x[mlp::NIN - 1] = log(delta * delta * G * sp.rho / R(a.ph.mu_inf));
+#pragma unroll
+          for (int f = 0; f < mlp::NIN; f++) use_net = use_net && x[f] >= R(a.mlp_lo[f]) && x[f] <= R(a.mlp_hi[f]);
+        }
+#pragma unroll
+        for (int f = 0; f < mlp::NIN; f++)
+          s_x[tid * mlp::NIN + f] = use_net ? float((x[f] - R(a.mlp_mean[f])) / R(a.mlp_scale[f])) : 0.0f;
+        scale_pt = delta * delta * G;
+        nu_w = wale_nu<ND, R>(sp, delta);
+      } else {
+#pragma unroll
+        for (int f = 0; f < mlp::NIN; f++) s_x[tid * mlp::NIN + f] = 0.0f;
+      }
+      __syncthreads();
+      {
+        const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll 1
+        for (int t4 = 0; t4 < 4; t4++) {
+          const int pt = wv * 64 + t4 * 16 + (lane & 15);
+          mlp::Tile tl;
+          mlp::forward(s_mlp, tl, [&](int f) { return s_x[pt * mlp::NIN + f]; }, lane);
+          if ((lane >> 4) == 0) s_ml[pt] = mlp::softplusf(tl.z4);
+        }
+      }
+      __syncthreads();
+      if (active) nu_t_pt = use_net ? R(s_ml[tid]) * R(a.mlp_out_scale) * scale_pt : nu_w;
+    }
+
     // ---- per solution point: corrected physical gradient, flux, transformed flux ----
     if (active) {
       const long pe = p + NU * (long)e;
@@ -301,23 +407,9 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       ViscT<ND, R> vt;
       if constexpr (VISC) {
         const R inv_detj = R(1.0 / a.detj[vi]);
-        int pos[ND], lb[ND], fL[ND], fR[ND];
-#pragma unroll
-        for (int d = 0; d < ND; d++) line_of<P, ND>(p, d, pos[d], lb[d], fL[d], fR[d]);
         // physical gradient column by column: d/dx_m = sum_d (JGinv(d,m)/J) d/dr_d
-        // (reference gradient + LDG correction, src/eles.cpp:1667-1870)
         R gr[NF][ND];
-#pragma unroll
-        for (int k = 0; k < NF; k++)
-#pragma unroll
-          for (int d = 0; d < ND; d++) {
-            const int st = T::stride(d);
-            R r = 0;
-#pragma unroll
-            for (int q = 0; q < n; q++) r += ops.D[pos[d]][q] * s_u[(el * NF + k) * NU + lb[d] + q * st];
-            r += ops.gL[pos[d]] * s_q[(el * NFP + fL[d]) * NF + k] + ops.gR[pos[d]] * s_q[(el * NFP + fR[d]) * NF + k];
-            gr[k][d] = r;
-          }
+        point_grad(gr);
         R Ji[ND][ND];
 #pragma unroll
         for (int d = 0; d < ND; d++)
@@ -335,7 +427,9 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         // SGS eddy-viscosity flux (src/eles.cpp:2369-2417): physical momentum flux
         // M and energy flux E from the full corrected gradient
         R sgM[ND][ND], sgE[ND];
-        if constexpr (LES != 0) {
+        if constexpr (LES == 3) {
+          sgs_flux_of<ND, R>(sp, sp.rho * nu_t_pt, gam, sgM, sgE);
+        } else if constexpr (LES != 0) {
           R gf[NF][ND];
 #pragma unroll
           for (int m = 0; m < ND; m++) {

@@ -9,7 +9,10 @@
 #include <vector>
 
 #include "../../include/hifr.h"
+#include "../../include/hifr_sgs.h"
 #include "fr_launch.hpp"
+#include "hifr_ctx_view.hpp"
+#include "sgs_mlp_state.hpp"
 
 using hifr::Ops1D;
 using hifr::StageArgsT;
@@ -29,6 +32,10 @@ struct hifr_ctx {
   double *u, *res, *div, *detj, *fgeo;
   void *uf[2], *fv, *gf, *jg;  // element type double, or float when mixed
   void *opI, *opD, *opL;       // tets: dense operators (double, or float when mixed)
+  // learned SGS closure (sgs_model HIFR_SGS_LEARNED): attached network, if any
+  hifr::mlp::Weights mlpw;
+  hifr::mlp::Norm mlpn;
+  int mlp_on;
   int* nbr;
   double* red;  // reduction scratch
   int cur;      // which uf buffer holds the traces of the current state
@@ -68,7 +75,10 @@ bool order_supported(int ele_type, int P) {
 int vis_kind(const hifr_ctx* c) { return c->cfg.viscous == 0 ? 0 : (c->cfg.fix_vis == 1.0 ? 1 : 2); }
 
 // 0 no LES, 1 Smagorinsky, 2 WALE
-int les_kind(const hifr_ctx* c) { return c->cfg.les ? c->cfg.sgs_model + 1 : 0; }
+int les_kind(const hifr_ctx* c) {
+  if (!c->cfg.les) return 0;
+  return c->cfg.sgs_model == HIFR_SGS_LEARNED ? 3 : c->cfg.sgs_model + 1;
+}
 
 template <class R>
 StageArgsT<R> base_args(hifr_ctx* c) {
@@ -99,6 +109,15 @@ StageArgsT<R> base_args(hifr_ctx* c) {
   a.opI = (const R*)c->opI;
   a.opD = (const R*)c->opD;
   a.opL = (const R*)c->opL;
+  a.mlpw = c->mlpw;
+  for (int f = 0; f < hifr::mlp::NIN; f++) {
+    a.mlp_mean[f] = c->mlpn.mean[f];
+    a.mlp_scale[f] = c->mlpn.scale[f];
+    a.mlp_lo[f] = c->mlpn.lo[f];
+    a.mlp_hi[f] = c->mlpn.hi[f];
+  }
+  a.mlp_out_scale = c->mlpn.out_scale;
+  a.mlp_on = c->mlp_on;
   return a;
 }
 
@@ -229,7 +248,12 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   if (cfg->n_eles <= 0) return HIFR_ERR_ARG;
   if (cfg->riemann_solve_type != 0 || cfg->vis_riemann_solve_type != 0) return HIFR_ERR_UNSUPPORTED;
   if (cfg->precision != HIFR_FP64 && cfg->precision != HIFR_MIXED) return HIFR_ERR_UNSUPPORTED;
-  if (cfg->les && (!cfg->viscous || cfg->fix_vis != 1.0 || cfg->sgs_model < 0 || cfg->sgs_model > 1))
+  if (cfg->les && (!cfg->viscous || cfg->fix_vis != 1.0 ||
+                   !(cfg->sgs_model == HIFR_SGS_SMAGORINSKY || cfg->sgs_model == HIFR_SGS_WALE ||
+                     cfg->sgs_model == HIFR_SGS_LEARNED)))
+    return HIFR_ERR_UNSUPPORTED;
+  if (cfg->les && cfg->sgs_model == HIFR_SGS_LEARNED &&
+      !(cfg->ele_type == HIFR_HEX && (cfg->order == 2 || cfg->order == 3)))
     return HIFR_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HIFR_ERR_DEVICE;
@@ -272,6 +296,9 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   c->detj = c->fgeo = nullptr;  // sized by hifr_set_metrics / hifr_set_interfaces
   c->jg = nullptr;
   c->opI = c->opD = c->opL = nullptr;
+  std::memset(&c->mlpw, 0, sizeof(c->mlpw));
+  std::memset(&c->mlpn, 0, sizeof(c->mlpn));
+  c->mlp_on = 0;
   c->affine = false;
   for (auto& al : allocs) {
     if (hipMalloc(al.p, al.bytes) != hipSuccess) {
@@ -860,4 +887,35 @@ extern "C" hifr_status hifr_calibrate_copy(long n, int reps, double* ms) {
   (void)hipFree(a);
   (void)hipFree(b);
   return st;
+}
+
+extern "C" hifr_status hifr_set_sgs_network(hifr_ctx* c, const hifr_mlp* m) {
+  if (!c) return HIFR_ERR_ARG;
+  if (!(c->cfg.les && c->cfg.sgs_model == HIFR_SGS_LEARNED)) return HIFR_ERR_STATE;
+  if (!m) {
+    c->mlp_on = 0;
+    std::memset(&c->mlpw, 0, sizeof(c->mlpw));
+    return HIFR_OK;
+  }
+  c->mlpw = hifr::mlp::Weights{m->w1, m->w2, m->w3, m->w4, m->params};
+  c->mlpn = m->norm;
+  c->mlp_on = 1;
+  return HIFR_OK;
+}
+
+hifr_status hifr::ctx_view(const hifr_ctx* c, CtxView& v) {
+  if (!c) return HIFR_ERR_ARG;
+  v.ele_type = c->cfg.ele_type;
+  v.order = c->cfg.order;
+  v.N = c->N;
+  v.have_state = c->have_state;
+  v.have_metrics = c->have_metrics;
+  v.affine = c->affine;
+  v.mixed = c->mixed;
+  v.stream = c->stream;
+  v.u = c->u;
+  v.detj = c->detj;
+  v.jg = c->mixed ? nullptr : (const double*)c->jg;
+  v.mu = c->cfg.mu_inf;
+  return HIFR_OK;
 }

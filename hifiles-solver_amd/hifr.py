@@ -68,6 +68,8 @@ def lib():
         L.hifr_mlp_set_data.argtypes = [vp, C.c_long, fp, fp]
         L.hifr_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, fp]
         L.hifr_mlp_predict.argtypes = [vp, C.c_long, fp, fp]
+        L.hifr_set_sgs_network.argtypes = [vp, vp]
+        L.hifr_sgs_samples.argtypes = [vp, C.c_int, dp, dp, C.c_double, fp, fp, fp, dp]
         _LIB = L
     return _LIB
 
@@ -235,6 +237,35 @@ class Solver:
     def synchronize(self):
         lib().hifr_synchronize(self.ctx())
 
+    def set_sgs_network(self, mlp):
+        """Attach a learned eddy-viscosity network (MLP) to a deck with
+        hifr_learned_sgs 1; None: the WALE fallback everywhere."""
+        rc = lib().hifr_set_sgs_network(self.ctx(), mlp.handle if mlp is not None else None)
+        if rc != 0:
+            raise RuntimeError("hifr_set_sgs_network: " + lib().hifr_status_string(rc).decode())
+        self._mlp = mlp  # the context reads the network's weights in place
+
+    def sgs_samples(self, order_lo, filter_ratio=1.0, want_state=False):
+        """Training samples of the learned closure from the current state, filtered
+        onto degree order_lo: (x (n, 10) raw inputs, C_exact (n), C_wale (n)[,
+        filtered state (Nu_lo, N, 5)])."""
+        nl3 = (order_lo + 1) ** 3
+        n = nl3 * self.n_eles
+        order = int(round(self.nu ** (1.0 / self.nd))) - 1  # hexahedra: Nu = (P+1)^3
+        F = np.ascontiguousarray(modal_filter_1d(order, order_lo))
+        D = np.ascontiguousarray(lagrange_derivative_1d(gauss_points(order_lo)))
+        x = np.zeros((n, 10), dtype=np.float32)
+        ce = np.zeros(n, dtype=np.float32)
+        cw = np.zeros(n, dtype=np.float32)
+        ulo = np.zeros(nl3 * self.n_eles * 5) if want_state else None
+        rc = lib().hifr_sgs_samples(self.ctx(), int(order_lo), _dp(F), _dp(D), float(filter_ratio), _fp(x), _fp(ce),
+                                    _fp(cw), _dp(ulo) if want_state else None)
+        if rc != 0:
+            raise RuntimeError("hifr_sgs_samples: " + lib().hifr_status_string(rc).decode())
+        if want_state:
+            return x, ce, cw, ulo.reshape((nl3, self.n_eles, 5), order="F")
+        return x, ce, cw
+
 
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
@@ -321,3 +352,25 @@ class MLP:
         y = np.zeros(n, dtype=np.float32)
         self._check(lib().hifr_mlp_predict(self._h, n, _fp(x), _fp(y)), "hifr_mlp_predict")
         return y
+
+
+def gauss_points(order):
+    """The solver's 1-D Gauss-Legendre solution points of an order."""
+    return np.polynomial.legendre.leggauss(order + 1)[0]
+
+
+def modal_filter_1d(order_hi, order_lo):
+    """(n_lo, n_hi) row-major: nodal values at the order_hi Gauss points -> the
+    Legendre expansion truncated to degree order_lo, at the order_lo Gauss points."""
+    xh, xl = gauss_points(order_hi), gauss_points(order_lo)
+    modes = np.linalg.solve(np.polynomial.legendre.legvander(xh, order_hi), np.eye(order_hi + 1))
+    return np.polynomial.legendre.legvander(xl, order_lo) @ modes[: order_lo + 1, :]
+
+
+def lagrange_derivative_1d(x):
+    """(n, n) row-major: D[a][b] = derivative of the Lagrange polynomial of point b at x[a]."""
+    n = len(x)
+    V = np.polynomial.legendre.legvander(x, n - 1)
+    dV = np.stack([np.polynomial.legendre.legval(x, np.polynomial.legendre.legder(np.eye(n)[k]))
+                   for k in range(n)], axis=1)
+    return dV @ np.linalg.inv(V)

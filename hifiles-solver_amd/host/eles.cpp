@@ -277,7 +277,7 @@ std::string eles::mv_all_cpu_gpu(const input& in) {
   c.const_src = in.const_src;
   c.metric_storage = in.metric_storage;
   c.les = in.LES;
-  c.sgs_model = in.SGS_model;
+  c.sgs_model = in.learned_sgs ? HIFR_SGS_LEARNED : in.SGS_model;
   c.filter_ratio = in.filter_ratio;
   c.R_ref = in.R_ref;
   c.T_wall = in.T_wall;

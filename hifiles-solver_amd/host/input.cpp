@@ -149,6 +149,7 @@ std::string input::read_input_file(const std::string& path) {
   d.get("hifr_host_only", host_only);
   d.get("hifr_metrics", metric_storage);
   d.get("hifr_precision", precision);
+  d.get("hifr_learned_sgs", learned_sgs);
   return "";
 }
 

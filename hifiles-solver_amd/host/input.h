@@ -46,6 +46,9 @@ struct input {
   // Not a reference key: hifr_precision 1 selects the mixed-precision kernels
   // (HIFR_MIXED); 0 is FP64.
   int precision = 0;
+  // Not a reference key: hifr_learned_sgs 1 (with LES 1, SGS_model 1) selects the
+  // learned eddy viscosity with the WALE fallback (HIFR_SGS_LEARNED).
+  int learned_sgs = 0;
   // derived (setup_params)
   double mu_inf = 0., rt_inf = 0., c_sth = 0., R_ref = 0., prandtl_t = 0., T_ref = 0., L_ref = 0.;
   double uvw_ref = 0., rho_ref = 0., p_ref = 0., mu_ref = 0., time_ref = 0.;

@@ -167,6 +167,9 @@ std::string setup_solver(hifr_solver* s) {
     if (in.SGS_model != 0 && in.SGS_model != 1) return "only the Smagorinsky (0) and WALE (1) SGS models are built";
     if (in.wall_model != 0) return "LES wall models are not built";
     if (in.fix_vis != 1.0) return "LES is built for constant viscosity (fix_vis 1) only";
+    if (in.learned_sgs && in.SGS_model != 1) return "the learned SGS closure falls back to WALE: set SGS_model 1";
+  } else if (in.learned_sgs) {
+    return "hifr_learned_sgs needs LES 1";
   }
   if (in.dt_type != 0) return "only user-supplied time steps (dt_type 0) are built";
   in.setup_params();
@@ -177,6 +180,8 @@ std::string setup_solver(hifr_solver* s) {
   else return "mesh element type not supported";
   eles& e = *s->el;
   e.ele_type = m.ele_type;
+  if (in.learned_sgs && !(m.ele_type == 4 && (in.order == 2 || in.order == 3)))
+    return "the learned SGS closure is built for hexes of order 2 and 3";
   std::string err = e.setup(in, m.n_eles);
   if (!err.empty()) return err;
   for (int i = 0; i < m.n_eles; i++)

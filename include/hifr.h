@@ -71,7 +71,11 @@ typedef struct {
   /* LES (src/eles.cpp:2321-2778, 3294-3368; src/int_inters.cpp:415-427): les 1 with
    * sgs_model 0 Smagorinsky or 1 WALE, eddy viscosity at the solution points with
    * filter width filter_ratio * vol^(1/d) / (P+1); requires viscous and constant
-   * viscosity (fix_vis 1).  No wall model. */
+   * viscosity (fix_vis 1).  No wall model.  sgs_model HIFR_SGS_LEARNED (no
+   * counterpart in the reference; hexes, P = 2, 3): the eddy viscosity of a
+   * learned network (include/hifr_sgs.h), with the WALE model of the reference's
+   * GPU build (src/cuda_kernels.cu:1466-1660) where no network is attached or its
+   * inputs leave the training range. */
   int les, sgs_model;
   double filter_ratio;
   /* isothermal walls (bctype 11): gas constant R_ref and wall temperature T_wall,
@@ -86,6 +90,8 @@ typedef struct {
  * and per face is kept and read by the kernels; otherwise, and with PER_POINT,
  * the per-point arrays are used as given. */
 enum { HIFR_METRICS_AUTO = 0, HIFR_METRICS_PER_POINT = 1 };
+
+enum { HIFR_SGS_SMAGORINSKY = 0, HIFR_SGS_WALE = 1, HIFR_SGS_LEARNED = 10 };
 
 typedef struct hifr_ctx hifr_ctx;
 

@@ -65,6 +65,30 @@ hifr_status hifr_mlp_train(hifr_mlp* m, int epochs, int batch, float lr, float* 
 /* Network output y (n) for n standardised samples x (n x 10). */
 hifr_status hifr_mlp_predict(hifr_mlp* m, long n, const float* x, float* y);
 
+/* Training samples from the state of a FP64 hexahedral context with per-element
+ * metrics (affine hexes).  Per element, the state is filtered onto degree
+ * order_lo: filter_1d (n_lo x n_hi, row-major, n = order + 1) maps nodal values at
+ * the context's 1-D solution points onto the truncated Legendre expansion
+ * evaluated at the degree-order_lo Gauss points, applied along r, s, t;
+ * deriv_1d (n_lo x n_lo, row-major) differentiates the Lagrange interpolant on
+ * those points.  From the filtered rho, momenta and products m_i m_j / rho: the
+ * Favre velocity v~, its gradient g, the SGS stress tau and
+ *   c_exact = -(tau^d : S~d) / (2 rho-bar S~d : S~d) / (delta^2 |g|),
+ *   c_wale  = nu_WALE(g) / (delta^2 |g|)   (the fallback model, for comparison),
+ *   x       = the closure's raw inputs (g_ij / |g|, log(delta^2 |g| / nu)),
+ * with delta = filter_ratio (8 detJ)^(1/3) / (order_lo + 1).  Outputs are host
+ * arrays of N n_lo^3 samples (sample = low-order point + n_lo^3 element; x is
+ * (samples x 10)); u_lo (optional, NULL to skip) receives the filtered
+ * conservative state (n_lo^3, N, 5). */
+hifr_status hifr_sgs_samples(hifr_ctx* c, int order_lo, const double* filter_1d, const double* deriv_1d,
+                             double filter_ratio, float* x, float* c_exact, float* c_wale, double* u_lo);
+
+/* Attaches a network to a context created with sgs_model HIFR_SGS_LEARNED: its
+ * weights (read in place by the residual kernels, so the network must outlive
+ * the context's use of it) and a copy of its normalisation (attach again after
+ * changing it).  m = NULL detaches: the WALE fallback everywhere. */
+hifr_status hifr_set_sgs_network(hifr_ctx* c, const hifr_mlp* m);
+
 #ifdef __cplusplus
 }
 #endif

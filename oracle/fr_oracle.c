@@ -169,6 +169,7 @@ struct or_solver {
   double *disu_upts[2], *div_tconf_upts, *disu_fpts, *tdisf_upts, *norm_tdisf_fpts,
       *norm_tconf_fpts, *delta_disu_fpts, *grad_disu_upts, *grad_disu_fpts, *src_upts;
   double *sgsf_upts, *sgsf_fpts; /* LES: transformed SGS flux (Nu,N,nf,d), (Nfp,N,nf,d) */
+  double *sgs_nu;                /* SGS_model 10: kinematic eddy viscosity per point (Nu,N) */
   int n_inters;
   double* inv_vdm;           /* tets: inverse Vandermonde of the Dubiner basis (Nu,Nu) */
   int n_bdy;                 /* boundary flux points */
@@ -946,7 +947,8 @@ or_solver* or_create_bc(const or_deck* deck, int ele_type, int n_verts, const do
                         const int* bf_type) {
   if (ele_type != 1 && ele_type != 2 && ele_type != 4) { fprintf(stderr, "fr_oracle: ele_type %d\n", ele_type); return NULL; }
   if (deck->order < 0 || deck->order > 6 || deck->upts_type != 0 || deck->equation != 0 ||
-      (deck->LES && (!deck->viscous || deck->SGS_model < 0 || deck->SGS_model > 1)) ||
+      (deck->LES && (!deck->viscous || !(deck->SGS_model == 0 || deck->SGS_model == 1 ||
+                                         deck->SGS_model == 10))) ||
       (ele_type == 2 && (deck->order < 1 || deck->order > 5 || !(deck->vcjh_scheme == 1 ||
                                                                  (deck->vcjh_scheme == 0 && deck->eta == 0.0))))) {
     /* tets: orders 1-5 of the tabulated point sets, DG correction (c_tet = 0) */
@@ -1022,6 +1024,7 @@ or_solver* or_create_bc(const or_deck* deck, int ele_type, int n_verts, const do
   if (deck->LES) {
     s->sgsf_upts = xcalloc(nuN * nf * nd, sizeof(double));
     s->sgsf_fpts = xcalloc(nfpN * nf * nd, sizeof(double));
+    s->sgs_nu = xcalloc(nuN, sizeof(double));
   }
   return s;
 }
@@ -1035,7 +1038,7 @@ void or_destroy(or_solver* s) {
   free(s->disu_upts[0]); free(s->disu_upts[1]); free(s->div_tconf_upts); free(s->src_upts);
   free(s->disu_fpts); free(s->tdisf_upts); free(s->norm_tdisf_fpts); free(s->norm_tconf_fpts);
   free(s->delta_disu_fpts); free(s->grad_disu_upts); free(s->grad_disu_fpts);
-  free(s->sgsf_upts); free(s->sgsf_fpts);
+  free(s->sgsf_upts); free(s->sgsf_fpts); free(s->sgs_nu);
   free(s->face_l); free(s->face_r); free(s->bdy_fpt); free(s->bdy_type); free(s->inv_vdm);
   free(s);
 }
@@ -1490,6 +1493,39 @@ static void calc_sgsf(const or_solver* s, const double* u, const double* g, doub
   }
 }
 
+/* SGS flux of a supplied kinematic eddy viscosity nu_t, in the form of the
+ * reference's GPU build (SGS_flux_kernel, src/cuda_kernels.cu:1466-1660; velocity
+ * and energy gradients of vis_NS_flux, :880-960): v_i = U_{i+1}/rho,
+ * dv_i/dx_j = (dU_{i+1}/dx_j - v_i drho/dx_j)/rho, Sd the deviatoric strain,
+ * de_j = (dE/dx_j - (E/rho) drho/dx_j)/rho - v_i dv_i/dx_j;
+ * F(1+i, j) = -2 rho nu_t Sd(i,j), F(nf-1, j) = -gamma rho nu_t / 0.5 de_j. */
+static void calc_sgsf_nu(const or_solver* s, const double* u, const double* g, double nu_t, double* sgsf) {
+  int nd = s->nd, nf = s->nf;
+  double rho = u[0], v[3], vg[3][3], de[3], Sd[3][3], tr = 0.0;
+  for (int i = 0; i < nd; i++) v[i] = u[1 + i] / rho;
+  for (int i = 0; i < nd; i++)
+    for (int j = 0; j < nd; j++) vg[i][j] = (g[(1 + i) + nf * j] - v[i] * g[0 + nf * j]) / rho;
+  for (int j = 0; j < nd; j++) {
+    double t = (g[(nf - 1) + nf * j] - (u[nf - 1] / rho) * g[0 + nf * j]) / rho;
+    for (int i = 0; i < nd; i++) t -= v[i] * vg[i][j];
+    de[j] = t;
+  }
+  for (int i = 0; i < nd; i++) tr += vg[i][i];
+  for (int i = 0; i < nd; i++)
+    for (int j = 0; j < nd; j++) Sd[i][j] = 0.5 * (vg[i][j] + vg[j][i]) - (i == j ? tr / 3.0 : 0.0);
+  double mu_t = rho * nu_t;
+  for (int j = 0; j < nd; j++) {
+    sgsf[0 + nf * j] = 0.0;
+    for (int i = 0; i < nd; i++) sgsf[(1 + i) + nf * j] = -2.0 * mu_t * Sd[i][j];
+    sgsf[(nf - 1) + nf * j] = -s->ph.gamma * mu_t / 0.5 * de[j];
+  }
+}
+
+void or_set_sgs_nu(or_solver* s, const double* nu_t) {
+  if (!s->sgs_nu) return;
+  memcpy(s->sgs_nu, nu_t, sizeof(double) * (size_t)s->nu * s->n_eles);
+}
+
 /* src/eles.cpp:2321-2453 (static mesh, no wall model) */
 static void evaluate_viscFlux(or_solver* s) {
   int nu = s->nu, N = s->n_eles, nf = s->nf, nd = s->nd;
@@ -1503,7 +1539,8 @@ static void evaluate_viscFlux(or_solver* s) {
       }
       calc_visf(nd, &s->ph, u, g, f);
       if (s->deck.LES) {
-        calc_sgsf(s, u, g, s->detjac_upts[j + (long)nu * i], sg);
+        if (s->deck.SGS_model == 10) calc_sgsf_nu(s, u, g, s->sgs_nu[j + (long)nu * i], sg);
+        else calc_sgsf(s, u, g, s->detjac_upts[j + (long)nu * i], sg);
         for (int k = 0; k < nf; k++)
           for (int l = 0; l < nd; l++) f[k + nf * l] += sg[k + nf * l];
         for (int k = 0; k < nf; k++)

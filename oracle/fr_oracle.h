@@ -44,7 +44,8 @@ typedef struct {
   int upts_type;              /* 0 Gauss (only supported point set) */
   int vcjh_scheme;            /* 0 user eta, 1 DG, 2 SD, 3 Hu, 4 c+ */
   double eta;                 /* used when vcjh_scheme == 0 */
-  int LES, SGS_model;         /* LES 1 with SGS_model 0 Smagorinsky, 1 WALE (no wall model) */
+  int LES, SGS_model;         /* LES 1 with SGS_model 0 Smagorinsky, 1 WALE (no wall model);
+                                 10: supplied eddy viscosity (or_set_sgs_nu), GPU-build form */
   double filter_ratio;
   double const_src;
   double dx_cyclic, dy_cyclic, dz_cyclic;
@@ -109,6 +110,8 @@ void or_advance(or_solver* s, int rk_stage, int adv_type);
 void or_run_steps(or_solver* s, int n_steps);
 /* CalcNormResidual (src/output.cpp:1452-1522), norm_type 0 inf, 1 L1, 2 L2. */
 void or_norm_residual(const or_solver* s, int norm_type, double* out5);
+/* SGS_model 10: kinematic eddy viscosity at every solution point (Nu,N). */
+void or_set_sgs_nu(or_solver* s, const double* nu_t);
 /* Number of OpenMP threads used by the dense loops (1 = serial). */
 void or_set_threads(int n);
 

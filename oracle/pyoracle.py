@@ -62,6 +62,7 @@ def lib():
         L.or_get.argtypes = [vp, C.c_int, dp]
         L.or_get.restype = C.c_long
         L.or_set_state.argtypes = [vp, dp]
+        L.or_set_sgs_nu.argtypes = [vp, dp]
         L.or_zero_rk_register.argtypes = [vp]
         L.or_set_ics_tgv.argtypes = [vp, C.c_double]
         L.or_get_face_pairs.argtypes = [vp, ip, ip]
@@ -184,6 +185,13 @@ class Oracle:
 
     def advance(self, stage, adv_type):
         lib().or_advance(self._h, int(stage), int(adv_type))
+
+    def set_sgs_nu(self, nu_t):
+        """SGS_model 10: kinematic eddy viscosity at the solution points, (Nu, N)."""
+        flat = np.ascontiguousarray(np.asarray(nu_t, dtype=np.float64).reshape(-1, order="F"))
+        if flat.size != self.nu * self.n_eles:
+            raise ValueError("nu_t size")
+        lib().or_set_sgs_nu(self._h, _dp(flat))
 
     def run_steps(self, n):
         lib().or_run_steps(self._h, int(n))
