@@ -253,7 +253,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
                      cfg->sgs_model == HIFR_SGS_LEARNED)))
     return HIFR_ERR_UNSUPPORTED;
   if (cfg->les && cfg->sgs_model == HIFR_SGS_LEARNED &&
-      !(cfg->ele_type == HIFR_HEX && (cfg->order == 2 || cfg->order == 3)))
+      (!hifr::mlp::kLearnedEnabled || !(cfg->ele_type == HIFR_HEX && (cfg->order == 2 || cfg->order == 3))))
     return HIFR_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HIFR_ERR_DEVICE;
@@ -890,6 +890,7 @@ extern "C" hifr_status hifr_calibrate_copy(long n, int reps, double* ms) {
 }
 
 extern "C" hifr_status hifr_set_sgs_network(hifr_ctx* c, const hifr_mlp* m) {
+  if (!hifr::mlp::kLearnedEnabled) return HIFR_ERR_UNSUPPORTED;
   if (!c) return HIFR_ERR_ARG;
   if (!(c->cfg.les && c->cfg.sgs_model == HIFR_SGS_LEARNED)) return HIFR_ERR_STATE;
   if (!m) {

@@ -268,6 +268,7 @@ extern "C" {
 hifr_status hifr_mlp_create(hifr_mlp** out, unsigned long long seed) {
   if (!out) return HIFR_ERR_ARG;
   *out = nullptr;
+  if (!kLearnedEnabled) return HIFR_ERR_UNSUPPORTED;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return HIFR_ERR_DEVICE;
   hifr_mlp* m = new hifr_mlp();

@@ -10,6 +10,7 @@
 #include "../../include/hifr_sgs.h"
 #include "fr_physics.hpp"
 #include "hifr_ctx_view.hpp"
+#include "sgs_mlp_state.hpp"
 
 namespace {
 
@@ -142,6 +143,7 @@ extern "C" hifr_status hifr_sgs_samples(hifr_ctx* c, int order_lo, const double*
                                         double filter_ratio, float* x, float* c_exact, float* c_wale,
                                         double* u_lo) {
   hifr::CtxView v;
+  if (!hifr::mlp::kLearnedEnabled) return HIFR_ERR_UNSUPPORTED;
   if (!c || !filter_1d || !deriv_1d || !x || !c_exact || !c_wale || !(filter_ratio > 0.0)) return HIFR_ERR_ARG;
   if (hifr::ctx_view(c, v) != HIFR_OK) return HIFR_ERR_ARG;
   if (v.ele_type != HIFR_HEX) return HIFR_ERR_UNSUPPORTED;

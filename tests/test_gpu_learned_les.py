@@ -1,0 +1,50 @@
+"""The learned LES closure end to end on the GPU, at a reduced size: a P=5
+Taylor-Green run at Re 5000 on 16^3 hexes generates filtered training samples
+(modal cutoff to P=2) at t = 1, 2, 3; the library's HIP kernels train the network
+(Adam, BF16 MFMA forward/backward); then
+
+- a priori, on the held-out snapshot at t = 4: the network's eddy-viscosity
+  coefficient is closer to the exact one (relative RMS error, correlation) than
+  the WALE coefficient, which is the method without the network;
+- a posteriori: P=2 LES on 16^3 hexes without a model, with WALE and with the
+  learned closure; the relative L2 error of their kinetic-energy dissipation rate
+  against the filtered P=5 run is reported (the full-size experiment, 32^3 P=5 ->
+  64^3 P=2, runs through hifiles-solver_amd/learned_sgs.py).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import learned_sgs as ls
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason="the learned-closure device entry points are switched off (hifr::mlp::kLearnedEnabled): their first device run coincided with a device fault whose cause is not established")]
+
+
+def test_learned_les_pipeline(require_gpu):
+    log = lambda msg: print(msg, flush=True)
+    gen = ls.generate(16, 5, 5000.0, 4.0, 50, 2, (1.0, 2.0, 3.0, 4.0), log=None)
+    X, C, CW, tagged = gen["samples"]
+    assert len(tagged) == 4, tagged
+    x, c = np.concatenate(X[:3]), np.concatenate(C[:3])
+    log("P=5 16^3: %.3f ms per step; %d training samples" % (gen["ms_per_step"], len(c)))
+    m, norm, hist = ls.train(x, c, epochs=20, log=log)
+    assert np.all(np.isfinite(hist)) and hist[-1] < hist[0]
+    ap = ls.apriori(m, norm, X[3], C[3], CW[3])
+    log("a priori on the held-out snapshot: %s" % json.dumps(ap))
+    assert ap["rel_rms_learned"] < ap["rel_rms_wale"]
+    assert ap["corr_learned"] > ap["corr_wale"]
+    eps_ref = ls.dissipation(gen["t"], gen["ke_filtered"])
+    res = {"apriori": ap, "les": {}}
+    for model in ("none", "wale", "learned"):
+        t, ke, ms = ls.deploy(16, 2, 5000.0, 4.0, 25, model, m)
+        err = ls.eps_error(gen["t"], eps_ref, t, ls.dissipation(t, ke))
+        res["les"][model] = {"eps_rel_err": err, "ms_per_step": ms, "ke_end": float(ke[-1])}
+        log("LES %-7s: dissipation-rate error %.4f, KE(t=4) %.5f (filtered P=5: %.5f), %.3f ms per step"
+            % (model, err, ke[-1], gen["ke_filtered"][-1], ms))
+        assert np.all(np.isfinite(ke)) and ke[-1] < ke[0]
+    out = os.environ.get("HIFR_LEARNED_REPORT")
+    if out:
+        with open(out, "w") as f:
+            json.dump(res, f)
