@@ -407,9 +407,23 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       ViscT<ND, R> vt;
       if constexpr (VISC) {
         const R inv_detj = R(1.0 / a.detj[vi]);
+        int pos[ND], lb[ND], fL[ND], fR[ND];
+#pragma unroll
+        for (int d = 0; d < ND; d++) line_of<P, ND>(p, d, pos[d], lb[d], fL[d], fR[d]);
         // physical gradient column by column: d/dx_m = sum_d (JGinv(d,m)/J) d/dr_d
+        // (reference gradient + LDG correction, src/eles.cpp:1667-1870)
         R gr[NF][ND];
-        point_grad(gr);
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+#pragma unroll
+          for (int d = 0; d < ND; d++) {
+            const int st = T::stride(d);
+            R r = 0;
+#pragma unroll
+            for (int q = 0; q < n; q++) r += ops.D[pos[d]][q] * s_u[(el * NF + k) * NU + lb[d] + q * st];
+            r += ops.gL[pos[d]] * s_q[(el * NFP + fL[d]) * NF + k] + ops.gR[pos[d]] * s_q[(el * NFP + fR[d]) * NF + k];
+            gr[k][d] = r;
+          }
         R Ji[ND][ND];
 #pragma unroll
         for (int d = 0; d < ND; d++)
