@@ -120,8 +120,13 @@ def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
 # and WRITE_SIZE (x1.0), KiB per dispatch, the factors read off a 1 GiB copy kernel;
 # keyed by (workload, per-element metrics, precision).  Sources: the summaries
 # under profiles/ named for each kernel and workload.
-MEASURED_KIB = {("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (227408.0, 122880.0),
-                                           "rk": (388554.0, 297642.7)}}
+MEASURED_KIB = {
+    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (227408.0, 122880.0), "rk": (388554.0, 297642.7)},
+    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (123474.3, 122880.0), "rk": (250421.0, 297642.7)},
+    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (99110.6, 61440.0), "rk": (241103.8, 236202.7)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (51113.1, 32400.0), "rk": (103201.2, 66960.0)},
+    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (39773.6, 16204.5), "rk": (85853.2, 50760.0)},
+}
 
 
 def measured_traffic(workload, per_element, precision):
@@ -209,6 +214,7 @@ def main():
     l2, mx = rel_errors(mixed["solver"].get("disu_upts"), s.get("disu_upts"))
     traffic = measured_traffic(args.workload, r["per_element"], "fp64") if (n, order) == (w.get("n", 0), w["order"]) \
         else None
+    mtraffic = measured_traffic(args.workload, mixed["per_element"], "mixed") if traffic is not None else None
     kname = "fr_dense_kernel" if w["kind"] == "tet" else "fr_tensor_kernel"
     rec = {
         "metric": METRIC, "value": round(r["value"], 4), "unit": "GDOF-stage/s", "n_gpus": 1, "steps": args.steps,
@@ -231,8 +237,9 @@ def main():
                   "rel_max_err_vs_fp64": [float("%.3e" % v) for v in mx],
                   "err_after_steps": args.warmup + args.steps,
                   "roofline": {"achieved": round(mixed["achieved"], 1), "frac": round(mixed["achieved"] / HBM_PEAK_GBS, 4),
+                               "traffic": round(mtraffic["rk"] * s.n_eles) if mtraffic else None,
                                "bytes_per_element": mixed["rk_b"], "avg_launch_ms": round(mixed["avg_rk_ms"], 5)},
-                  "stage": stage_block(mixed, mixed["solver"], None, w)},
+                  "stage": stage_block(mixed, mixed["solver"], mtraffic, w)},
         "residual_L1": [float("%.6e" % v) for v in r["norms"]],
         "setup_s": round(r["setup"], 2),
     }
