@@ -41,10 +41,11 @@ def test_samples_match_oracle(require_gpu):
     u0 = _state(s.get("pos_upts"))
     s.set_state(u0)
     x, ce, cw, ulo = s.sgs_samples(2, filter_ratio=1.0, want_state=True)
-    h = TWO_PI / 3
     mu = cases.oracle_of(g, LES=0).phys().mu_inf
-    drdx = np.repeat((np.eye(3) * 2.0 / h)[None], s.n_eles, axis=0)
-    detj = np.full(s.n_eles, (h / 2) ** 3)
+    # dr_d/dx_m = JGinv(d, m) / detJ of each (affine) element, from its first point
+    JG = s.get("JGinv_upts")[:, :, 0, :]        # (d, m, N)
+    detj = s.get("detjac_upts")[0, :]
+    drdx = JG.transpose(2, 0, 1) / detj[:, None, None]
     xo, ceo, cwo, ulo_o = sc.samples(u0, hifr.gauss_points(5), hifr.gauss_points(2), drdx, detj, 1.0, mu)
     print("state", np.abs(ulo - ulo_o).max(), "x", np.abs(x - xo).max(), "c_exact", np.abs(ce - ceo).max(),
           np.abs(ceo).max(), "c_wale", np.abs(cw - cwo).max(), np.abs(cwo).max(), flush=True)
