@@ -124,7 +124,7 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       R uo[NF];
       interp_u(tl, fp, uo);
 #pragma unroll
-      for (int k = 0; k < NF; k++) a.uf_out[fp + NFP * (te + NN * k)] = uo[k];
+      for (int k = 0; k < NF; k++) a.uf_out[(fp + NFP * (long)te) * NF + k] = uo[k];
     }
     return;
   } else {
@@ -147,7 +147,7 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         } else {
 #pragma unroll
           for (int k = 0; k < NF; k++) {
-            const R un = a.uf_in[nb + NFP * NN * k];
+            const R un = a.uf_in[(long)nb * NF + k];
             const R uc = R(0.5) * (uo[k] + un) - pen * (uo[k] - un);
             s_q[(tl * NFP + fp) * NF + k] = uc - uo[k];
           }
@@ -318,7 +318,7 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           }
         }
 #pragma unroll
-        for (int k = 0; k < NF; k++) a.fv_out[fe + NFP * NN * k] = fa[k];
+        for (int k = 0; k < NF; k++) a.fv_out[fe * NF + k] = fa[k];
       }
       return;
     } else {
@@ -354,7 +354,7 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           for (int k = 0; k < NF; k++) fn[k] = R(0.5) * (fo[k] + fg[k]);
         } else {
 #pragma unroll
-          for (int k = 0; k < NF; k++) un[k] = a.uf_in[nb + NFP * NN * k];
+          for (int k = 0; k < NF; k++) un[k] = a.uf_in[(long)nb * NF + k];
           if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
           else rusanov<ND, R>(uo, un, nrm, gam, fn);
         }
@@ -369,14 +369,14 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           if (wall) {
 #pragma unroll
             for (int k = 0; k < NF; k++) {
-              const R fa = a.fv_in[fe + NFP * NN * k];
+              const R fa = a.fv_in[fe * NF + k];
               jump[k] += (fa + tau * nn * (uo[k] - un[k])) * tdA;
             }
           } else {
 #pragma unroll
             for (int k = 0; k < NF; k++) {
-              const R fa = a.fv_in[fe + NFP * NN * k];
-              const R fb = -a.fv_in[nb + NFP * NN * k];
+              const R fa = a.fv_in[fe * NF + k];
+              const R fb = -a.fv_in[(long)nb * NF + k];
               jump[k] += ldg_normal<R>(fa, fb, uo[k], un[k], nn, pen, tau) * tdA;
             }
           }
@@ -465,7 +465,7 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           R uo[NF];
           interp_u(tl, fp, uo);
 #pragma unroll
-          for (int k = 0; k < NF; k++) a.uf_out[fp + NFP * (te + NN * k)] = uo[k];
+          for (int k = 0; k < NF; k++) a.uf_out[(fp + NFP * (long)te) * NF + k] = uo[k];
         }
       }
     }

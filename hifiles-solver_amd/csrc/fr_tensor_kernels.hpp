@@ -63,6 +63,9 @@ struct StageArgsT {
   double* u;             // state (read; written by ADVANCE)
   double* res;           // RK register (ADVANCE)
   double* div;           // div_tconf_upts output (RESID; ADVANCE if non-null)
+  // Traces (state u_f, viscous normal flux F_v.n) are stored field-interleaved:
+  // value of field k at global flux point f = fpt + Nfp * element is at [f * nf + k],
+  // so a face's points and fields form one contiguous run for the neighbour's gather.
   const R* uf_in;        // traces of the current state (deviations on the mixed path)
   R* uf_out;             // traces written (TRACE_U, ADVANCE)
   const R* fv_in;        // viscous normal flux traces (RESID/ADVANCE, viscous)
@@ -263,7 +266,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       R uo[NF];
       interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
 #pragma unroll
-      for (int k = 0; k < NF; k++) a.uf_out[fp + NFP * (te + NN * k)] = uo[k];
+      for (int k = 0; k < NF; k++) a.uf_out[(fp + NFP * (long)te) * NF + k] = uo[k];
     }
     return;
   } else {
@@ -290,7 +293,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         } else {
 #pragma unroll
           for (int k = 0; k < NF; k++) {
-            const R un = a.uf_in[nb + NFP * NN * k];
+            const R un = a.uf_in[(long)nb * NF + k];
             const R uc = R(0.5) * (uo[k] + un) - pen * (uo[k] - un);
             s_q[(tl * NFP + fp) * NF + k] = uc - uo[k];
           }
@@ -569,7 +572,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           }
         }
 #pragma unroll
-        for (int k = 0; k < NF; k++) a.fv_out[fe + NFP * NN * k] = fa[k];
+        for (int k = 0; k < NF; k++) a.fv_out[fe * NF + k] = fa[k];
       }
       return;
     } else {
@@ -611,7 +614,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           for (int k = 0; k < NF; k++) fn[k] = R(0.5) * (fo[k] + fg[k]);
         } else {
 #pragma unroll
-          for (int k = 0; k < NF; k++) un[k] = a.uf_in[nb + NFP * NN * k];
+          for (int k = 0; k < NF; k++) un[k] = a.uf_in[(long)nb * NF + k];
           if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
           else rusanov<ND, R>(uo, un, nrm, gam, fn);
         }
@@ -627,15 +630,15 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
             // Dirichlet LDG flux f_c = f_l + tau n (u_l - u_r) (src/inters.cpp:605-619)
 #pragma unroll
             for (int k = 0; k < NF; k++) {
-              const R fa = a.fv_in[fe + NFP * NN * k];
+              const R fa = a.fv_in[fe * NF + k];
               jump[k] += (fa + tau * nn * (uo[k] - un[k])) * tdA;
             }
           } else {
 #pragma unroll
             for (int k = 0; k < NF; k++) {
               // own F_v.n and the neighbour's, whose normal is the negated own normal
-              const R fa = a.fv_in[fe + NFP * NN * k];
-              const R fb = -a.fv_in[nb + NFP * NN * k];
+              const R fa = a.fv_in[fe * NF + k];
+              const R fb = -a.fv_in[(long)nb * NF + k];
               jump[k] += ldg_normal<R>(fa, fb, uo[k], un[k], nn, pen, tau) * tdA;
             }
           }
@@ -726,7 +729,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           R uo[NF];
           interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
 #pragma unroll
-          for (int k = 0; k < NF; k++) a.uf_out[fp + NFP * (te + NN * k)] = uo[k];
+          for (int k = 0; k < NF; k++) a.uf_out[(fp + NFP * (long)te) * NF + k] = uo[k];
         }
       }
     }

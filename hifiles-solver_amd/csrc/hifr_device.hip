@@ -642,22 +642,32 @@ hifr_status hifr_download(hifr_ctx* c, int what, double* host) {
       src = c->gf; n = nfp * N * nf * nd; esz = c->tsz; break;
     default: return HIFR_ERR_ARG;
   }
-  if (esz == sizeof(double)) {
+  if (esz == sizeof(double) && what != HIFR_DISU_FPTS) {
     HIP_TRY(hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return HIFR_OK;
   }
-  // mixed path: FP32 buffers; state traces are deviations from the reference state
-  std::vector<float> tmp(n);
-  HIP_TRY(hipMemcpyAsync(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  // FP32 buffers (mixed path), and the field-interleaved state traces [f * nf + k],
+  // which go out in the reference layout (Nfp, N, nf); mixed state traces are
+  // deviations from the reference state
+  std::vector<double> tmp(n);
+  if (esz == sizeof(double)) {
+    HIP_TRY(hipMemcpyAsync(tmp.data(), src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  } else {
+    std::vector<float> tf(n);
+    HIP_TRY(hipMemcpyAsync(tf.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) tmp[i] = tf[i];
+  }
   HIP_TRY(hipStreamSynchronize(c->stream));
-  for (size_t i = 0; i < n; i++) host[i] = tmp[i];
   if (what == HIFR_DISU_FPTS) {
     const size_t np = nfp * N;
-    for (size_t i = 0; i < np; i++) {
-      host[i] += c->ref.rho;
-      host[i + np * (nf - 1)] += c->ref.E;
-    }
+    const double add_rho = c->mixed ? c->ref.rho : 0.0, add_E = c->mixed ? c->ref.E : 0.0;
+    for (size_t f = 0; f < np; f++)
+      for (size_t k = 0; k < nf; k++)
+        host[f + np * k] = tmp[f * nf + k] + (k == 0 ? add_rho : (k == nf - 1 ? add_E : 0.0));
+  } else {
+    for (size_t i = 0; i < n; i++) host[i] = tmp[i];
   }
   return HIFR_OK;
 }
