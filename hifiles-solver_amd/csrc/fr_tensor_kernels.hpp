@@ -244,6 +244,32 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
                                                              : (long)fpt + NFP * (long)ee; };
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
+  // ---- this thread's flux-point tasks (task = tid + t * BLOCK): in FP64 the
+  //      neighbour index is fetched before u, so that its latency overlaps the u
+  //      load and the face passes start with the neighbour-trace load (the FP32
+  //      kernels would lose a wave per SIMD to the extra registers) ----
+  constexpr int MT = (EPB * NFP + BLOCK - 1) / BLOCK;
+  constexpr bool PREF = !MIX && MODE != MODE_TRACE_U;
+  [[maybe_unused]] int pnb[MT];
+  if constexpr (PREF) {
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+      const int task = tid + t * BLOCK, tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+      pnb[t] = (task < EPB * NFP && te < N) ? a.nbr[fp + NFP * te] : BDY_FLAG;
+    }
+  }
+  // body(t, task) over this thread's flux-point tasks: unrolled when pnb holds the
+  // prefetched indices (registers need a constant t), a plain loop otherwise
+  [[maybe_unused]] auto each_task = [&](auto&& body) {
+    if constexpr (PREF) {
+#pragma unroll
+      for (int t = 0; t < MT; t++) body(t, tid + t * BLOCK);
+    } else {
+#pragma unroll 1
+      for (int task = tid; task < EPB * NFP; task += BLOCK) body(0, task);
+    }
+  };
+
   // ---- load u ----
   if (active) {
 #pragma unroll
@@ -273,13 +299,14 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     // ---- LDG common solution -> delta = u_c - u_own at each flux point
     //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346) ----
     if constexpr (VISC) {
-#pragma unroll 1
-      for (int task = tid; task < EPB * NFP; task += BLOCK) {
+      each_task([&](const int t, const int task) {
         const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
-        if (te >= N) continue;
+        if (task >= EPB * NFP || te >= N) return;
         int dir, side, base;
         face_line<P, ND>(fp, dir, side, base);
-        int nb = a.nbr[fp + NFP * te];
+        int nb;
+        if constexpr (PREF) nb = pnb[t];
+        else nb = a.nbr[fp + NFP * te];
         const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
         nb &= 0x7fffffff;
         R uo[NF];
@@ -298,7 +325,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
             s_q[(tl * NFP + fp) * NF + k] = uc - uo[k];
           }
         }
-      }
+      });
       __syncthreads();
     }
 
@@ -578,16 +605,17 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     } else {
       // ---- interface fluxes and jumps at the flux points
       //      (common_invFlux / common_viscFlux, extrapolate_totalFlux, daxpy) ----
-#pragma unroll 1
-      for (int task = tid; task < EPB * NFP; task += BLOCK) {
+      each_task([&](const int t, const int task) {
         const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
-        if (te >= N) continue;
+        if (task >= EPB * NFP || te >= N) return;
         int dir, side, base;
         face_line<P, ND>(fp, dir, side, base);
         const int st = T::stride(dir);
         const R* lw = side ? ops.lR : ops.lL;
         const long fe = fp + NFP * (long)te;
-        int nb = a.nbr[fe];
+        int nb;
+        if constexpr (PREF) nb = pnb[t];
+        else nb = a.nbr[fe];
         const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
         nb &= 0x7fffffff;
         const long gi = geo_idx(fp, te);
@@ -652,7 +680,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           for (int m = 0; m < n; m++) v += lw[m] * s_f[(tl * NF * ND + k * ND + dir) * NU + base + m * st];
           s_q[(tl * NFP + fp) * NF + k] = jump[k] - sgn * v;
         }
-      }
+      });
       __syncthreads();
 
       // ---- divergence of the transformed flux (calculate_divergence) plus the
