@@ -151,6 +151,9 @@ __device__ __forceinline__ void line_of(int p, int dir, int& pos, int& base, int
   }
 }
 
+// TRACE_GRAD: the gradient is written over u and the LDG delta once every point has
+// read them (the face pass takes the own state trace from the trace array), so the
+// block holds max(U + Q, G) + SG values instead of U + Q + G + SG.
 template <int P, int ND, int EPB, int MODE, int LES>
 struct SmemLayout {
   using T = TP<P, ND, EPB>;
@@ -160,7 +163,9 @@ struct SmemLayout {
   static constexpr int F = (MODE == MODE_RESID || MODE == MODE_ADVANCE) ? EPB * T::NF * ND * T::NU : 0;
   // LES: transformed SGS flux at the solution points (extrapolated to the faces)
   static constexpr int SG = (MODE == MODE_TRACE_GRAD && LES) ? EPB * T::NF * ND * T::NU : 0;
-  static constexpr int TOTAL = U + Q + G + F + SG;
+  static constexpr int G_AT = (MODE == MODE_TRACE_GRAD) ? 0 : U + Q;  // offset of the gradient
+  static constexpr int UQG = (MODE == MODE_TRACE_GRAD) ? (U + Q > G ? U + Q : G) : U + Q + G;
+  static constexpr int TOTAL = UQG + F + SG;
 };
 
 // Interpolate NF fields from LDS rows (stride NU apart) to the end of a line.
@@ -193,11 +198,11 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   constexpr int n = T::n, NU = T::NU, NFP = T::NFP, NF = T::NF, BLOCK = T::BLOCK;
   constexpr int NFPF = T::NFPF, NFACE = 2 * ND;
   __shared__ R smem[S::TOTAL];
-  R* s_u = smem;        // [EPB][NF][NU]  u, or u - u_ref (MIX)
-  R* s_q = s_u + S::U;  // [EPB][NFP][NF]   LDG delta, then interface jumps
-  R* s_g = s_q + S::Q;  // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD)
-  R* s_f = s_g + S::G;  // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE)
-  R* s_s = s_f + S::F;  // [EPB][NF*ND][NU] transformed SGS flux (TRACE_GRAD, LES)
+  R* s_u = smem;             // [EPB][NF][NU]  u, or u - u_ref (MIX)
+  R* s_q = s_u + S::U;       // [EPB][NFP][NF]   LDG delta, then interface jumps
+  R* s_g = smem + S::G_AT;   // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD, over s_u and s_q)
+  R* s_f = smem + S::UQG;    // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE)
+  R* s_s = s_f + S::F;       // [EPB][NF*ND][NU] transformed SGS flux (TRACE_GRAD, LES)
   // learned closure: network weights, standardised inputs and outputs per point
   __shared__ std::conditional_t<LES == 3, mlp::Lds, char> s_mlp;
   __shared__ float s_x[LES == 3 ? BLOCK * mlp::NIN : 1];
@@ -422,6 +427,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     }
 
     // ---- per solution point: corrected physical gradient, flux, transformed flux ----
+    // TRACE_GRAD keeps the gradient (and the transformed SGS flux) in registers until
+    // every point has read u and the delta, then writes it over them
+    [[maybe_unused]] R gx[NF][ND];
+    [[maybe_unused]] R sx[LES != 0 && MODE == MODE_TRACE_GRAD ? NF - 1 : 1][ND];
     if (active) {
       const long pe = p + NU * (long)e;
       const long vi = vol_idx(pe, e);
@@ -493,24 +502,23 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
             R gc[NF];
             gcol(m, gc);
 #pragma unroll
-            for (int k = 0; k < NF; k++) s_g[(el * NF * ND + k * ND + m) * NU + p] = gc[k];
+            for (int k = 0; k < NF; k++) gx[k][m] = gc[k];
           }
           if constexpr (LES != 0) {
-            // transformed SGS flux sgsf(k,l) = sum_m JGinv(l,m) F_sgs(k,m)
+            // transformed SGS flux sgsf(k,l) = sum_m JGinv(l,m) F_sgs(k,m), k >= 1
 #pragma unroll
             for (int l = 0; l < ND; l++) {
-              s_s[(el * NF * ND + 0 * ND + l) * NU + p] = R(0);
 #pragma unroll
               for (int i = 0; i < ND; i++) {
                 R v = 0;
 #pragma unroll
                 for (int m = 0; m < ND; m++) v += Jg[l][m] * sgM[i][m];
-                s_s[(el * NF * ND + (1 + i) * ND + l) * NU + p] = v;
+                sx[i][l] = v;
               }
               R v = 0;
 #pragma unroll
               for (int m = 0; m < ND; m++) v += Jg[l][m] * sgE[m];
-              s_s[(el * NF * ND + (NF - 1) * ND + l) * NU + p] = v;
+              sx[NF - 2][l] = v;
             }
           }
         } else {
@@ -546,6 +554,23 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         }
       }
     }
+    if constexpr (MODE == MODE_TRACE_GRAD) {
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+#pragma unroll
+          for (int m = 0; m < ND; m++) s_g[(el * NF * ND + k * ND + m) * NU + p] = gx[k][m];
+        if constexpr (LES != 0) {
+#pragma unroll
+          for (int l = 0; l < ND; l++) {
+            s_s[(el * NF * ND + 0 * ND + l) * NU + p] = R(0);
+#pragma unroll
+            for (int k = 1; k < NF; k++) s_s[(el * NF * ND + k * ND + l) * NU + p] = sx[k - 1][l];
+          }
+        }
+      }
+    }
     __syncthreads();
 
     if constexpr (MODE == MODE_TRACE_GRAD) {
@@ -560,8 +585,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         const int st = T::stride(dir);
         const R* lw = side ? ops.lR : ops.lL;
         const long fe = fp + NFP * (long)te;
+        // own state trace, as the state pass wrote it (s_u now holds the gradient)
         R uo[NF];
-        interp_line<n, NF, NU>(s_u + tl * NF * NU, lw, base, st, uo);
+#pragma unroll
+        for (int k = 0; k < NF; k++) uo[k] = a.uf_in[fe * NF + k];
         const long gi = geo_idx(fp, te);
         R nrm[ND];
 #pragma unroll
