@@ -151,21 +151,32 @@ __device__ __forceinline__ void line_of(int p, int dir, int& pos, int& base, int
   }
 }
 
-// TRACE_GRAD: the gradient is written over u and the LDG delta once every point has
-// read them (the face pass takes the own state trace from the trace array), so the
-// block holds max(U + Q, G) + SG values instead of U + Q + G + SG.
-template <int P, int ND, int EPB, int MODE, int LES>
+// LDS of one block (VB bytes per value).  A pass may write its result over what no
+// later pass reads: TRACE_GRAD always writes the gradient over u and the LDG delta.
+// RESID / ADVANCE write the transformed flux over u (ALIAS_F: the face pass then
+// takes the own state trace from the trace array, and the new state goes back into
+// u after the divergence pass) only where the plain layout would fit fewer than 6
+// blocks in a CU's 160 KB, the most that the registers of those kernels allow;
+// elsewhere the extra barriers cost more than they gain.
+template <int P, int ND, int EPB, int MODE, int LES, int VB>
 struct SmemLayout {
   using T = TP<P, ND, EPB>;
+  static constexpr int mx(int x, int y) { return x > y ? x : y; }
+  static constexpr bool FLUX = MODE == MODE_RESID || MODE == MODE_ADVANCE;
   static constexpr int U = EPB * T::NF * T::NU;
   static constexpr int Q = (MODE == MODE_TRACE_U) ? 0 : EPB * T::NFP * T::NF;
   static constexpr int G = (MODE == MODE_TRACE_GRAD) ? EPB * T::NF * ND * T::NU : 0;
-  static constexpr int F = (MODE == MODE_RESID || MODE == MODE_ADVANCE) ? EPB * T::NF * ND * T::NU : 0;
+  static constexpr int F = FLUX ? EPB * T::NF * ND * T::NU : 0;
   // LES: transformed SGS flux at the solution points (extrapolated to the faces)
   static constexpr int SG = (MODE == MODE_TRACE_GRAD && LES) ? EPB * T::NF * ND * T::NU : 0;
-  static constexpr int G_AT = (MODE == MODE_TRACE_GRAD) ? 0 : U + Q;  // offset of the gradient
-  static constexpr int UQG = (MODE == MODE_TRACE_GRAD) ? (U + Q > G ? U + Q : G) : U + Q + G;
-  static constexpr int TOTAL = UQG + F + SG;
+  static constexpr bool ALIAS_F = FLUX && (U + Q + F) * VB > 163840 / 6;
+  static constexpr int Q_AT = FLUX ? 0 : U;
+  static constexpr int U_AT = FLUX ? Q : 0;
+  static constexpr int G_AT = 0;
+  static constexpr int F_AT = ALIAS_F ? Q : Q + U;
+  static constexpr int S_AT = mx(U + Q, G);
+  static constexpr int TOTAL = FLUX ? (ALIAS_F ? Q + mx(U, F) : Q + U + F)
+                                    : (MODE == MODE_TRACE_GRAD ? S_AT + SG : U);
 };
 
 // Interpolate NF fields from LDS rows (stride NU apart) to the end of a line.
@@ -194,15 +205,15 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   constexpr bool SUTH = VIS == 2;
   static_assert(LES == 0 || VISC, "LES needs the viscous kernels");
   using T = TP<P, ND, EPB>;
-  using S = SmemLayout<P, ND, EPB, MODE, LES>;
+  using S = SmemLayout<P, ND, EPB, MODE, LES, (int)sizeof(R)>;
   constexpr int n = T::n, NU = T::NU, NFP = T::NFP, NF = T::NF, BLOCK = T::BLOCK;
   constexpr int NFPF = T::NFPF, NFACE = 2 * ND;
   __shared__ R smem[S::TOTAL];
-  R* s_u = smem;             // [EPB][NF][NU]  u, or u - u_ref (MIX)
-  R* s_q = s_u + S::U;       // [EPB][NFP][NF]   LDG delta, then interface jumps
+  R* s_u = smem + S::U_AT;   // [EPB][NF][NU]  u, or u - u_ref (MIX)
+  R* s_q = smem + S::Q_AT;   // [EPB][NFP][NF]   LDG delta, then interface jumps
   R* s_g = smem + S::G_AT;   // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD, over s_u and s_q)
-  R* s_f = smem + S::UQG;    // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE)
-  R* s_s = s_f + S::F;       // [EPB][NF*ND][NU] transformed SGS flux (TRACE_GRAD, LES)
+  R* s_f = smem + S::F_AT;   // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE, over s_u)
+  R* s_s = smem + S::S_AT;   // [EPB][NF*ND][NU] transformed SGS flux (TRACE_GRAD, LES)
   // learned closure: network weights, standardised inputs and outputs per point
   __shared__ std::conditional_t<LES == 3, mlp::Lds, char> s_mlp;
   __shared__ float s_x[LES == 3 ? BLOCK * mlp::NIN : 1];
@@ -431,6 +442,9 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     // every point has read u and the delta, then writes it over them
     [[maybe_unused]] R gx[NF][ND];
     [[maybe_unused]] R sx[LES != 0 && MODE == MODE_TRACE_GRAD ? NF - 1 : 1][ND];
+    // RESID / ADVANCE likewise keep the transformed flux, and FP64 ADVANCE the state
+    [[maybe_unused]] R fx[ND][NF];
+    [[maybe_unused]] R ukeep[NF];
     if (active) {
       const long pe = p + NU * (long)e;
       const long vi = vol_idx(pe, e);
@@ -443,6 +457,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       R u_own[NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) u_own[k] = s_u[(el * NF + k) * NU + p];
+      if constexpr (MODE == MODE_ADVANCE && !MIX && S::ALIAS_F) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) ukeep[k] = u_own[k];
+      }
       ViscT<ND, R> vt;
       if constexpr (VISC) {
         const R inv_detj = R(1.0 / a.detj[vi]);
@@ -550,8 +568,20 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
             for (int k = 0; k < NF; k++) fi[k] += fv[k];
           }
 #pragma unroll
-          for (int k = 0; k < NF; k++) s_f[(el * NF * ND + k * ND + l) * NU + p] = fi[k];
+          for (int k = 0; k < NF; k++) {
+            if constexpr (S::ALIAS_F) fx[l][k] = fi[k];
+            else s_f[(el * NF * ND + k * ND + l) * NU + p] = fi[k];
+          }
         }
+      }
+    }
+    if constexpr (S::ALIAS_F) {
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int l = 0; l < ND; l++)
+#pragma unroll
+          for (int k = 0; k < NF; k++) s_f[(el * NF * ND + k * ND + l) * NU + p] = fx[l][k];
       }
     }
     if constexpr (MODE == MODE_TRACE_GRAD) {
@@ -650,8 +680,14 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
         for (int l = 0; l < ND; l++) nrm[l] = R(a.fgeo[l * NN * GEO_PER_ELE + gi]);
         const R tdA = R(a.fgeo[ND * NN * GEO_PER_ELE + gi]);
+        // own state trace: with ALIAS_F as the state pass wrote it (s_u holds the flux)
         R uo[NF], un[NF], fn[NF];
-        interp_line<n, NF, NU>(s_u + tl * NF * NU, lw, base, st, uo);
+        if constexpr (S::ALIAS_F) {
+#pragma unroll
+          for (int k = 0; k < NF; k++) uo[k] = a.uf_in[fe * NF + k];
+        } else {
+          interp_line<n, NF, NU>(s_u + tl * NF * NU, lw, base, st, uo);
+        }
         const bool wall = (nb & BDY_FLAG) != 0;
         if (wall) {
           // isothermal wall: central flux between the trace and the ghost state
@@ -712,6 +748,19 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 
       // ---- divergence of the transformed flux (calculate_divergence) plus the
       //      correction lift (calculate_corrected_divergence), then output ----
+      [[maybe_unused]] R unew_r[NF];
+      // state before the update (FP64: as loaded; MIX: the FP64 array) and the new
+      // state's LDS copy (in place, or after the barrier below with ALIAS_F)
+      [[maybe_unused]] auto old_u = [&](int k, long idx) -> double {
+        if constexpr (MIX) return a.u[idx];
+        else if constexpr (S::ALIAS_F) return ukeep[k];
+        else return s_u[(el * NF + k) * NU + p];
+      };
+      [[maybe_unused]] auto put_u = [&](int k, double unew) {
+        const R v = R(unew - (MIX ? ref_of(k) : 0.0));
+        if constexpr (S::ALIAS_F) unew_r[k] = v;
+        else s_u[(el * NF + k) * NU + p] = v;
+      };
       if (active) {
         R div[NF];
 #pragma unroll
@@ -744,35 +793,31 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
               const double rhs = -double(div[k]) / detj + a.ph.const_src;
               const double r = a.rk_a * a.res[idx] + a.dt * rhs;
               a.res[idx] = r;
-              if constexpr (MIX) {
-                const double unew = a.u[idx] + a.rk_b * r;
-                a.u[idx] = unew;
-                s_u[(el * NF + k) * NU + p] = R(unew - ref_of(k));
-              } else {
-                const double unew = s_u[(el * NF + k) * NU + p] + a.rk_b * r;
-                a.u[idx] = unew;
-                s_u[(el * NF + k) * NU + p] = unew;
-              }
+              const double unew = old_u(k, idx) + a.rk_b * r;
+              a.u[idx] = unew;
+              put_u(k, unew);
             }
           } else {
             // forward Euler (src/eles.cpp:1067-1079)
 #pragma unroll
             for (int k = 0; k < NF; k++) {
               const long idx = pe + NU * NN * k;
-              if constexpr (MIX) {
-                const double unew = a.u[idx] - a.dt * (double(div[k]) / detj - a.ph.const_src);
-                a.u[idx] = unew;
-                s_u[(el * NF + k) * NU + p] = R(unew - ref_of(k));
-              } else {
-                const double unew = s_u[(el * NF + k) * NU + p] - a.dt * (div[k] / detj - a.ph.const_src);
-                a.u[idx] = unew;
-                s_u[(el * NF + k) * NU + p] = unew;
-              }
+              const double unew = old_u(k, idx) - a.dt * (double(div[k]) / detj - a.ph.const_src);
+              a.u[idx] = unew;
+              put_u(k, unew);
             }
           }
         }
       }
       if constexpr (MODE == MODE_ADVANCE) {
+        if constexpr (S::ALIAS_F) {
+          // the new state goes over the flux once every point has read it
+          __syncthreads();
+          if (active) {
+#pragma unroll
+            for (int k = 0; k < NF; k++) s_u[(el * NF + k) * NU + p] = unew_r[k];
+          }
+        }
         __syncthreads();
         // ---- traces of the new state for the next stage ----
 #pragma unroll 1
