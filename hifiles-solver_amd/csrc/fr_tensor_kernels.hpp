@@ -177,6 +177,10 @@ struct SmemLayout {
   static constexpr int S_AT = mx(U + Q, G);
   static constexpr int TOTAL = FLUX ? (ALIAS_F ? Q + mx(U, F) : Q + U + F)
                                     : (MODE == MODE_TRACE_GRAD ? S_AT + SG : U);
+  // waves per SIMD that the LDS allows (rounded up, at most 8)
+  static constexpr int LDS_BLOCKS = 163840 / (TOTAL * VB);
+  static constexpr int WAVES = mx(1, (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4 > 8 ? 8
+                                         : (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4);
 };
 
 // Interpolate NF fields from LDS rows (stride NU apart) to the end of a line.
@@ -197,8 +201,19 @@ __device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base
 // LES: 0 none, 1 Smagorinsky, 2 WALE eddy viscosity (viscous kernels only), as the
 // reference CPU build computes them; 3 learned eddy viscosity with the WALE model
 // of the reference GPU build as fallback (hexes).
+// Register budget (waves per SIMD) of a kernel variant.  The FP64 advance kernels
+// of affine hexes up to P = 4 (constant viscosity, no LES) are compiled for the
+// number of waves that their LDS allows, so that registers do not bind first (at
+// P = 3: 128 VGPRs and 7 blocks per CU instead of 151 and 6); every other variant
+// would spill under that budget and keeps the compiler's choice.
 template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES>
-__global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK))
+constexpr int kernel_waves() {
+  using S = SmemLayout<P, ND, EPB, MODE, LES, MIX ? 4 : 8>;
+  return (MODE == MODE_ADVANCE && ND == 3 && P <= 4 && AFF && !MIX && LES == 0 && VIS != 2) ? S::WAVES : 1;
+}
+
+template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES>
+__global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK), (kernel_waves<P, ND, EPB, VIS, MODE, AFF, MIX, LES>()))
 fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
   constexpr bool VISC = VIS != 0;
