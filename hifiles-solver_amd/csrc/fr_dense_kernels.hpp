@@ -171,7 +171,7 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       for (int k = 0; k < NF; k++) u_own[k] = s_u[(el * NF + k) * NU + p];
       ViscT<ND, R> vt;
       if constexpr (VISC) {
-        const R inv_detj = R(1.0 / a.detj[vi]);
+        const R inv_detj = R(rcp(a.detj[vi]));
         // reference gradient D u plus the LDG correction (L o tnorm) delta
         // (src/eles.cpp:1667-1870)
         R gr[NF][ND];
@@ -421,12 +421,12 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = double(div[k]);
         }
         if constexpr (MODE == MODE_ADVANCE) {
-          const double detj = a.detj[vol_idx(pe, e)];
+          const double idetj = rcp(a.detj[vol_idx(pe, e)]);
           if (a.adv_type == 3) {
 #pragma unroll
             for (int k = 0; k < NF; k++) {
               const long idx = pe + NU * NN * k;
-              const double rhs = -double(div[k]) / detj + a.ph.const_src;
+              const double rhs = -double(div[k]) * idetj + a.ph.const_src;
               const double r = a.rk_a * a.res[idx] + a.dt * rhs;
               a.res[idx] = r;
               if constexpr (MIX) {
@@ -444,11 +444,11 @@ fr_dense_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
             for (int k = 0; k < NF; k++) {
               const long idx = pe + NU * NN * k;
               if constexpr (MIX) {
-                const double unew = a.u[idx] - a.dt * (double(div[k]) / detj - a.ph.const_src);
+                const double unew = a.u[idx] - a.dt * (double(div[k]) * idetj - a.ph.const_src);
                 a.u[idx] = unew;
                 s_u[(el * NF + k) * NU + p] = R(unew - ref_of(k));
               } else {
-                const double unew = s_u[(el * NF + k) * NU + p] - a.dt * (div[k] / detj - a.ph.const_src);
+                const double unew = s_u[(el * NF + k) * NU + p] - a.dt * (div[k] * idetj - a.ph.const_src);
                 a.u[idx] = unew;
                 s_u[(el * NF + k) * NU + p] = unew;
               }

@@ -19,6 +19,23 @@
 
 namespace hifr {
 
+// Reciprocal of a positive normal number without the IEEE division sequence.
+// FP64: the hardware estimate refined by two Newton-Raphson steps
+// r <- r + r (1 - x r), within about an ulp of 1/x.  FP32 (mixed path): the
+// hardware estimate, within an ulp.
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Square root: FP64 correctly rounded; FP32 the hardware estimate, within an ulp.
+__device__ __forceinline__ double sqrt_fast(double x) { return sqrt(x); }
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 struct Phys {
   double gamma, prandtl, mu_inf, rt_inf, c_sth, fix_vis, tau, pen_fact, const_src;
   double R_ref, T_wall;  // isothermal walls
@@ -32,15 +49,15 @@ __device__ __forceinline__ void wall_ghost(const T* ul, T gamma, T R_ref, T T_wa
   T v_sq = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
-    const T v = ul[i + 1] / ul[0];
+    const T v = ul[i + 1] * rcp(ul[0]);
     v_sq += v * v;
   }
   const T p = (gamma - T(1)) * (ul[ND + 1] - T(0.5) * ul[0] * v_sq);
-  const T rho = p / (R_ref * T_wall);
+  const T rho = p * rcp(R_ref * T_wall);
   ur[0] = rho;
 #pragma unroll
   for (int i = 0; i < ND; i++) ur[i + 1] = T(0);
-  ur[ND + 1] = p / (gamma - T(1));
+  ur[ND + 1] = p * rcp(gamma - T(1));
 }
 
 // Reference state at rest of the mixed-precision path (rho_r, E_r; p_r and the
@@ -54,7 +71,7 @@ struct RefState {
 // (src/flux.cpp:36-128).
 template <int ND, class T>
 __device__ __forceinline__ void inv_flux_dot(const T* u, const T* w, T gamma, T* out) {
-  const T ir = T(1) / u[0];
+  const T ir = rcp(u[0]);
   T vw = 0, ke = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
@@ -73,7 +90,7 @@ __device__ __forceinline__ void inv_flux_dot(const T* u, const T* w, T gamma, T*
 // p - p_r = (gamma-1)(dE - rho|v|^2/2); the energy flux is v.w (H_r + dE + p - p_r).
 template <int ND, class T>
 __device__ __forceinline__ void inv_flux_dot_dev(const T* du, const T* w, T gamma, const RefState<T>& r, T* out) {
-  const T ir = T(1) / (r.rho + du[0]);
+  const T ir = rcp(r.rho + du[0]);
   T vw = 0, mv = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
@@ -129,7 +146,7 @@ struct ViscT {
 template <int ND, bool SUTH, class T, class GCol>
 __device__ __forceinline__ void visc_tensor_cols(const T* u, GCol&& gcol, const Phys& ph, ViscT<ND, T>& t) {
   constexpr int NF = ND + 2;
-  const T rho = u[0], irho = T(1) / rho;
+  const T rho = u[0], irho = rcp(rho);
   T v[ND];
   T ke = 0;
 #pragma unroll
@@ -158,13 +175,13 @@ __device__ __forceinline__ void visc_tensor_cols(const T* u, GCol&& gcol, const 
   T diag = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) diag += dv[i][i];
-  diag = diag / T(3);
+  diag = diag * T(1.0 / 3.0);
 #pragma unroll
   for (int i = 0; i < ND; i++)
 #pragma unroll
     for (int j = 0; j < ND; j++)
       t.tau[i][j] = (i == j) ? T(2) * mu * (dv[i][i] - diag) : mu * (dv[i][j] + dv[j][i]);
-  const T kq = mu / T(ph.prandtl) * T(ph.gamma);
+  const T kq = mu * T(ph.gamma / ph.prandtl);
 #pragma unroll
   for (int m = 0; m < ND; m++) {
     T w = 0;
@@ -382,7 +399,7 @@ __device__ __forceinline__ void rusanov(const T* uo, const T* un, const T* n, T 
   inv_flux_dot<ND, T>(uo, n, gamma, fo);
   inv_flux_dot<ND, T>(un, n, gamma, fb);
   T vno = 0, vnn = 0, ko = 0, kn = 0;
-  const T iro = T(1) / uo[0], irn = T(1) / un[0];
+  const T iro = rcp(uo[0]), irn = rcp(un[0]);
 #pragma unroll
   for (int i = 0; i < ND; i++) {
     const T a = uo[1 + i] * iro, b = un[1 + i] * irn;
@@ -394,7 +411,7 @@ __device__ __forceinline__ void rusanov(const T* uo, const T* un, const T* n, T 
   const T po = (gamma - T(1)) * (uo[ND + 1] - T(0.5) * uo[0] * ko);
   const T pn = (gamma - T(1)) * (un[ND + 1] - T(0.5) * un[0] * kn);
   const T vn_av = fabs(T(0.5) * (vno + vnn));
-  const T c_av = sqrt(gamma * (po + pn) / (uo[0] + un[0]));
+  const T c_av = sqrt_fast(gamma * (po + pn) * rcp(uo[0] + un[0]));
   const T eig = fabs(vn_av + c_av);
 #pragma unroll
   for (int k = 0; k < NF; k++) fn[k] = T(0.5) * ((fo[k] + fb[k]) - eig * (un[k] - uo[k]));
@@ -409,7 +426,7 @@ __device__ __forceinline__ void rusanov_dev(const T* duo, const T* dun, const T*
   inv_flux_dot_dev<ND, T>(duo, n, gamma, r, fo);
   inv_flux_dot_dev<ND, T>(dun, n, gamma, r, fb);
   const T ro = r.rho + duo[0], rn = r.rho + dun[0];
-  const T iro = T(1) / ro, irn = T(1) / rn;
+  const T iro = rcp(ro), irn = rcp(rn);
   T vno = 0, vnn = 0, mvo = 0, mvn = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
@@ -422,7 +439,7 @@ __device__ __forceinline__ void rusanov_dev(const T* duo, const T* dun, const T*
   const T po = r.p + (gamma - T(1)) * (duo[ND + 1] - T(0.5) * mvo);
   const T pn = r.p + (gamma - T(1)) * (dun[ND + 1] - T(0.5) * mvn);
   const T vn_av = fabs(T(0.5) * (vno + vnn));
-  const T c_av = sqrt(gamma * (po + pn) / (ro + rn));
+  const T c_av = sqrt_fast(gamma * (po + pn) * rcp(ro + rn));
   const T eig = fabs(vn_av + c_av);
 #pragma unroll
   for (int k = 0; k < NF; k++) fn[k] = T(0.5) * ((fo[k] + fb[k]) - eig * (dun[k] - duo[k]));

@@ -393,7 +393,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         constexpr long VOL = AFF ? 1 : NU;
         R gr[NF][ND];
         point_grad(gr);
-        const R inv_detj = R(1.0 / a.detj[vi]);
+        const R inv_detj = R(rcp(a.detj[vi]));
         R gf[NF][ND];
 #pragma unroll
         for (int k = 0; k < NF; k++)
@@ -478,7 +478,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       }
       ViscT<ND, R> vt;
       if constexpr (VISC) {
-        const R inv_detj = R(1.0 / a.detj[vi]);
+        const R inv_detj = R(rcp(a.detj[vi]));
         int pos[ND], lb[ND], fL[ND], fR[ND];
 #pragma unroll
         for (int d = 0; d < ND; d++) line_of<P, ND>(p, d, pos[d], lb[d], fL[d], fR[d]);
@@ -799,13 +799,13 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = double(div[k]);
         }
         if constexpr (MODE == MODE_ADVANCE) {
-          const double detj = a.detj[vol_idx(pe, e)];
+          const double idetj = rcp(a.detj[vol_idx(pe, e)]);
           if (a.adv_type == 3) {
             // low-storage RK45 stage (src/eles.cpp:1155-1178)
 #pragma unroll
             for (int k = 0; k < NF; k++) {
               const long idx = pe + NU * NN * k;
-              const double rhs = -double(div[k]) / detj + a.ph.const_src;
+              const double rhs = -double(div[k]) * idetj + a.ph.const_src;
               const double r = a.rk_a * a.res[idx] + a.dt * rhs;
               a.res[idx] = r;
               const double unew = old_u(k, idx) + a.rk_b * r;
@@ -817,7 +817,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
             for (int k = 0; k < NF; k++) {
               const long idx = pe + NU * NN * k;
-              const double unew = old_u(k, idx) - a.dt * (double(div[k]) / detj - a.ph.const_src);
+              const double unew = old_u(k, idx) - a.dt * (double(div[k]) * idetj - a.ph.const_src);
               a.u[idx] = unew;
               put_u(k, unew);
             }
