@@ -121,11 +121,12 @@ def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
 # keyed by (workload, per-element metrics, precision).  Sources: the summaries
 # under profiles/ named for each kernel and workload.
 MEASURED_KIB = {
-    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (227408.0, 122880.0), "rk": (388554.0, 297642.7)},
-    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (123474.3, 122880.0), "rk": (250421.0, 297642.7)},
-    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (99110.6, 61440.0), "rk": (241103.8, 236202.7)},
-    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (51113.1, 32400.0), "rk": (103201.2, 66960.0)},
-    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (39773.6, 16204.5), "rk": (85853.2, 50760.0)},
+    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (280706.3, 122881.4), "rk": (410887.3, 297642.8)},
+    ("tgv32", False, "mixed"): {"n_eles": 32768, "grad": (181605.0, 61440.4), "rk": (332173.5, 236202.7)},
+    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (156171.6, 122880.7), "rk": (274734.0, 297642.8)},
+    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (99757.1, 61440.0), "rk": (223869.5, 236202.7)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (38242.3, 32400.0), "rk": (78310.6, 66960.0)},
+    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (28909.5, 16200.1), "rk": (59480.8, 50760.0)},
 }
 
 
