@@ -7,6 +7,7 @@
 
 #include "fr_dense_kernels.hpp"
 #include "fr_tensor_kernels.hpp"
+#include "fr_tet_mfma.hpp"
 
 namespace hifr {
 

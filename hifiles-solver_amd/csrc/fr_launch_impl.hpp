@@ -90,7 +90,13 @@ template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES>
 hipError_t tet_les(const Args<MIX>& a, hipStream_t s) {
   using T = TetP<P>;
   const int grid = (a.N + T::EPB - 1) / T::EPB;
-  hipLaunchKernelGGL((fr_dense_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
+  if constexpr (MODE == MODE_RESID || MODE == MODE_ADVANCE) {
+    // flux pass: operator products on MFMA (one wavefront of EPB elements per block)
+    static_assert(T::BLOCK == 64, "one wavefront per block");
+    hipLaunchKernelGGL((fr_tet_mfma_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(64), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((fr_dense_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
+  }
   return hipGetLastError();
 }
 

@@ -87,6 +87,12 @@ struct StageArgsT {
   const R* opI;
   const R* opD;
   const R* opL;
+  // the same operators combined and laid out as 16x16x4 MFMA A-operand tiles (tets,
+  // flux pass): I, [D_c | L o tnorm_c], [tnorm_c o I], [D_0 D_1 D_2 | L]
+  const R* tI;
+  const R* tG;
+  const R* tN;
+  const R* tD;
   // learned eddy viscosity (LES 3): network weights, input standardisation and
   // training range, output scale; mlp_on 0 gives the WALE fallback everywhere
   mlp::Weights mlpw;

@@ -1,0 +1,448 @@
+// Flux / divergence / update pass of the FR residual for tetrahedra on gfx950 with
+// the dense element operators applied as MFMA tiles (RESID and ADVANCE stages).
+//
+// A workgroup is one wavefront holding EPB = 64 / Nu elements.  Every operator
+// product of the pass is one small GEMM  C (M x NC) = A (M x K) * B (K x NC)  whose
+// columns are the block's (element, field) pairs, n = 5 el + k, padded to NC = 16 NT:
+//   uo   = I                      * u          traces for the LDG common solution
+//   gr   = [D_c | L o tnorm_c]    * [u; delta] corrected reference gradient, row c Nu + p
+//   fn   = [tnorm_c o I]          * Ft         normal transformed flux at the flux points
+//   div  = [D_0 D_1 D_2 | L]      * [Ft; jump] divergence plus correction lift
+//   uf   = I                      * u_new      traces of the new state
+// (the reference's opp_0, opp_4 / opp_2, opp_3 / opp_5 / opp_6 products,
+// src/eles.cpp:3620-4052, with tnorm the reference tet's face normals,
+// src/eles_tets.cpp:589-620).  A is streamed from global memory in the MFMA operand
+// order (16 x 4 tiles, 64 values per tile, prepared once by hifr_set_operators), so
+// the operators take no LDS and every load is one coalesced 512 B (FP64) access; B
+// comes from LDS as [row][column] matrices.  The pointwise physics (fluxes, LES,
+// Rusanov and LDG interface fluxes, RK update) is the per-point / per-flux-point
+// code of fr_dense_kernels.hpp.
+#pragma once
+#include "fr_dense_kernels.hpp"
+
+namespace hifr {
+
+// 16x16x4 MFMA per precision.  Lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; element i of its result is C[row(l, i)][l % 16].
+template <class R>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int i) { return (l >> 4) + 4 * i; }
+};
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int l, int i) { return 4 * (l >> 4) + i; }
+};
+
+// Sizes of the tiled products for order P (shared with the host, which prepares
+// the operator tiles in hifr_set_operators).
+template <int P>
+struct TetMfma {
+  using T = TetP<P>;
+  static constexpr int NU = T::NU, NFP = T::NFP, EPB = T::EPB;
+  static constexpr int KU = (NU + 3) / 4 * 4;          // rows of u (zero-padded)
+  static constexpr int KF = (3 * NU + 3) / 4 * 4;      // rows of the flux / gradient
+  static constexpr int NT = (5 * EPB + 15) / 16, NC = 16 * NT;
+  static constexpr int MT_I = (NFP + 15) / 16, KS_I = KU / 4;              // I
+  static constexpr int MT_G = (3 * NU + 15) / 16, KS_G = (KU + NFP) / 4;  // [D_c | L tnorm_c]
+  static constexpr int MT_N = (NFP + 15) / 16, KS_N = KF / 4;              // [tnorm_c I]
+  static constexpr int MT_D = (NU + 15) / 16, KS_D = (KF + NFP) / 4;       // [D | L]
+  static constexpr int SZ_I = MT_I * KS_I * 64, SZ_G = MT_G * KS_G * 64, SZ_N = MT_N * KS_N * 64,
+                       SZ_D = MT_D * KS_D * 64;
+  static constexpr int ROWS = KU + NFP + KF;  // LDS rows: u, q (delta / fn / jump / div), flux
+};
+
+// acc[mt][nt] += A (tiles mt, k-steps ks0 .. ks0 + KS - 1 of an operator with KSA
+// k-steps) * B (KS * 4 rows of an LDS matrix with NC columns).
+template <class R, int MT, int NT, int KSA, int KS>
+__device__ __forceinline__ void mfma_seg(const R* __restrict__ A, int ks0, const R* B, int lane,
+                                         typename Mfma<R>::v4 (&acc)[MT][NT]) {
+  constexpr int NC = 16 * NT;
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    R b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) b[nt] = B[(ks * 4 + (lane >> 4)) * NC + nt * 16 + (lane & 15)];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+      const R av = A[(mt * KSA + ks0 + ks) * 64 + lane];
+#pragma unroll
+      for (int nt = 0; nt < NT; nt++) acc[mt][nt] = Mfma<R>::op(av, b[nt], acc[mt][nt]);
+    }
+  }
+}
+
+template <class R, int MT, int NT>
+__device__ __forceinline__ void mfma_zero(typename Mfma<R>::v4 (&acc)[MT][NT]) {
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = typename Mfma<R>::v4{0, 0, 0, 0};
+}
+
+// C rows < M into an LDS matrix with NC columns
+template <class R, int MT, int NT>
+__device__ __forceinline__ void mfma_store(const typename Mfma<R>::v4 (&acc)[MT][NT], int M, R* C, int lane) {
+  constexpr int NC = 16 * NT;
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = mt * 16 + Mfma<R>::row(lane, i);
+      if (r < M) {
+#pragma unroll
+        for (int nt = 0; nt < NT; nt++) C[r * NC + nt * 16 + (lane & 15)] = acc[mt][nt][i];
+      }
+    }
+}
+
+template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES>
+__global__ void __launch_bounds__(64)
+fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
+  using R = std::conditional_t<MIX, float, double>;
+  using V4 = typename Mfma<R>::v4;
+  constexpr int ND = 3, NF = 5, NFACE = 4;
+  constexpr bool VISC = VIS != 0;
+  constexpr bool SUTH = VIS == 2;
+  static_assert(MODE == MODE_RESID || MODE == MODE_ADVANCE, "flux pass only");
+  static_assert(LES == 0 || VISC, "LES needs the viscous kernels");
+  using M = TetMfma<P>;
+  constexpr int NU = M::NU, NFP = M::NFP, NFPF = TetP<P>::NFPF, EPB = M::EPB;
+  constexpr int KU = M::KU, KF = M::KF, NT = M::NT, NC = M::NC;
+  __shared__ R smem[M::ROWS * NC];
+  R* s_u = smem;              // [KU][NC]   u (deviations on MIX), then the new state
+  R* s_q = s_u + KU * NC;     // [NFP][NC]  uo -> LDG delta; fn -> jump - fn; div (rows < Nu)
+  R* s_f = s_q + NFP * NC;    // [KF][NC]   corrected reference gradient -> transformed flux
+
+  const R gam = R(a.ph.gamma);
+  RefState<R> ref;
+  ref.rho = R(a.ref.rho);
+  ref.E = R(a.ref.E);
+  ref.p = R(a.ref.p);
+  ref.H = R(a.ref.H);
+  auto ref_of = [&](int k) -> double { return k == 0 ? a.ref.rho : (k == NF - 1 ? a.ref.E : 0.0); };
+  auto full = [&](const R* w, R* out) {
+    if constexpr (MIX) {
+      add_ref<ND, R>(w, ref, out);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; k++) out[k] = w[k];
+    }
+  };
+  auto ghost = [&](const R* wo, R* wr) {
+    R uo_full[NF];
+    full(wo, uo_full);
+    wall_ghost<ND, R>(uo_full, gam, R(a.ph.R_ref), R(a.ph.T_wall), wr);
+    if constexpr (MIX) {
+      wr[0] -= ref.rho;
+      wr[NF - 1] -= ref.E;
+    }
+  };
+
+  const int N = a.N;
+  const long NN = (long)N;
+  const int tid = threadIdx.x;
+  const int lane = tid;
+  const int el = tid / NU;
+  const int p = tid - el * NU;
+  const int e = blockIdx.x * EPB + el;
+  const bool active = (tid < EPB * NU) && (e < N);
+  auto vol_idx = [&](long pe, int ee) -> long { return AFF ? (long)ee : pe; };
+  auto geo_idx = [&](int fpt, int ee) -> long { return AFF ? (long)(fpt / NFPF) + NFACE * (long)ee
+                                                             : (long)fpt + NFP * (long)ee; };
+  constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
+
+  // ---- zero the LDS (padding rows and columns stay zero), then u ----
+  for (int i = tid; i < M::ROWS * NC; i += 64) smem[i] = R(0);
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const double v = a.u[p + NU * (e + NN * k)];
+      s_u[p * NC + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
+    }
+  }
+  __syncthreads();
+
+  if constexpr (VISC) {
+    // ---- own traces uo = I u, then the LDG delta = u_c - uo at each flux point
+    //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346) ----
+    {
+      V4 acc[M::MT_I][NT];
+      mfma_zero<R>(acc);
+      mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_u, lane, acc);
+      mfma_store<R>(acc, NFP, s_q, lane);
+    }
+    __syncthreads();
+#pragma unroll 1
+    for (int task = tid; task < EPB * NFP; task += 64) {
+      const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+      if (te >= N) continue;
+      int nb = a.nbr[fp + NFP * te];
+      const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
+      nb &= 0x7fffffff;
+      R* q = s_q + fp * NC + tl * NF;
+      R uo[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) uo[k] = q[k];
+      if (nb & BDY_FLAG) {
+        // wall: Dirichlet common solution u_c = (u_l + u_r)/2 (src/inters.cpp:664-668)
+        R ur[NF];
+        ghost(uo, ur);
+#pragma unroll
+        for (int k = 0; k < NF; k++) q[k] = R(0.5) * (ur[k] + uo[k]) - uo[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          const R un = a.uf_in[(long)nb * NF + k];
+          const R uc = R(0.5) * (uo[k] + un) - pen * (uo[k] - un);
+          q[k] = uc - uo[k];
+        }
+      }
+    }
+    __syncthreads();
+    // ---- corrected reference gradient: D_c u + (L o tnorm_c) delta (src/eles.cpp:1667-1870) ----
+    {
+      V4 acc[M::MT_G][NT];
+      mfma_zero<R>(acc);
+      mfma_seg<R, M::MT_G, NT, M::KS_G, KU / 4>(a.tG, 0, s_u, lane, acc);
+      mfma_seg<R, M::MT_G, NT, M::KS_G, NFP / 4>(a.tG, KU / 4, s_q, lane, acc);
+      mfma_store<R>(acc, 3 * NU, s_f, lane);
+    }
+    __syncthreads();
+  }
+
+  // ---- per solution point: physical gradient, flux, transformed flux (written
+  //      over the gradient of the same point and element) ----
+  if (active) {
+    const long pe = p + NU * (long)e;
+    const long vi = vol_idx(pe, e);
+    constexpr long VOL = AFF ? 1 : NU;
+    R Jg[ND][ND];
+#pragma unroll
+    for (int l = 0; l < ND; l++)
+#pragma unroll
+      for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * VOL + vi];
+    R u_own[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) u_own[k] = s_u[p * NC + el * NF + k];
+    ViscT<ND, R> vt;
+    if constexpr (VISC) {
+      const R inv_detj = R(rcp(a.detj[vi]));
+      R gr[NF][ND];
+#pragma unroll
+      for (int k = 0; k < NF; k++)
+#pragma unroll
+        for (int c = 0; c < ND; c++) gr[k][c] = s_f[(c * NU + p) * NC + el * NF + k];
+      R Ji[ND][ND];
+#pragma unroll
+      for (int d = 0; d < ND; d++)
+#pragma unroll
+        for (int m = 0; m < ND; m++) Ji[d][m] = Jg[d][m] * inv_detj;
+      auto gcol = [&](int m, R* gc) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          R v = 0;
+#pragma unroll
+          for (int d = 0; d < ND; d++) v += gr[k][d] * Ji[d][m];
+          gc[k] = v;
+        }
+      };
+      R sgM[ND][ND], sgE[ND];
+      if constexpr (LES != 0) {
+        R gf[NF][ND];
+#pragma unroll
+        for (int m = 0; m < ND; m++) {
+          R gc[NF];
+          gcol(m, gc);
+#pragma unroll
+          for (int k = 0; k < NF; k++) gf[k][m] = gc[k];
+        }
+        R uu[NF];
+        full(u_own, uu);
+        // calc_ele_vol for tets: detJ * 8/6
+        sgs_flux<ND, LES, R>(uu, gf, R(a.detj[vi] * 8. / 6.), R(a.filter_ratio), P, gam, sgM, sgE);
+      }
+      R uu[NF];
+      full(u_own, uu);
+      visc_tensor_cols<ND, SUTH, R>(uu, gcol, a.ph, vt);
+      if constexpr (LES != 0) {
+#pragma unroll
+        for (int i = 0; i < ND; i++)
+#pragma unroll
+          for (int j = 0; j < ND; j++) vt.tau[i][j] -= sgM[i][j];
+#pragma unroll
+        for (int j = 0; j < ND; j++) vt.qe[j] += sgE[j];
+      }
+    }
+    // transformed flux tdisf(k,l) = sum_m JGinv(l,m) F(k,m)  (src/eles.cpp:1397-1405, 2437-2444)
+#pragma unroll
+    for (int l = 0; l < ND; l++) {
+      R fi[NF];
+      if constexpr (MIX) inv_flux_dot_dev<ND, R>(u_own, Jg[l], gam, ref, fi);
+      else inv_flux_dot<ND, R>(u_own, Jg[l], gam, fi);
+      if constexpr (VISC) {
+        R fv[NF];
+        visc_flux_dot<ND, R>(vt, Jg[l], fv);
+#pragma unroll
+        for (int k = 0; k < NF; k++) fi[k] += fv[k];
+      }
+#pragma unroll
+      for (int k = 0; k < NF; k++) s_f[(l * NU + p) * NC + el * NF + k] = fi[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- normal transformed flux at the flux points: fn = (tnorm o I) Ft ----
+  {
+    V4 acc[M::MT_N][NT];
+    mfma_zero<R>(acc);
+    mfma_seg<R, M::MT_N, NT, M::KS_N, M::KS_N>(a.tN, 0, s_f, lane, acc);
+    mfma_store<R>(acc, NFP, s_q, lane);
+  }
+  __syncthreads();
+
+  // ---- interface fluxes and jumps at the flux points
+  //      (common_invFlux / common_viscFlux, extrapolate_totalFlux, daxpy) ----
+#pragma unroll 1
+  for (int task = tid; task < EPB * NFP; task += 64) {
+    const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+    if (te >= N) continue;
+    const long fe = fp + NFP * (long)te;
+    int nb = a.nbr[fe];
+    const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
+    nb &= 0x7fffffff;
+    const long gi = geo_idx(fp, te);
+    R nrm[ND];
+#pragma unroll
+    for (int l = 0; l < ND; l++) nrm[l] = R(a.fgeo[l * NN * GEO_PER_ELE + gi]);
+    const R tdA = R(a.fgeo[ND * NN * GEO_PER_ELE + gi]);
+    // own state trace, as the previous pass wrote it
+    R uo[NF], un[NF], fn[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) uo[k] = a.uf_in[fe * NF + k];
+    const bool wall = (nb & BDY_FLAG) != 0;
+    if (wall) {
+      // isothermal wall: central flux between the trace and the ghost state
+      // (convective_flux_boundary, src/inters.cpp:368-387)
+      ghost(uo, un);
+      R fo[NF], fg[NF];
+      if constexpr (MIX) {
+        inv_flux_dot_dev<ND, R>(uo, nrm, gam, ref, fo);
+        inv_flux_dot_dev<ND, R>(un, nrm, gam, ref, fg);
+      } else {
+        inv_flux_dot<ND, R>(uo, nrm, gam, fo);
+        inv_flux_dot<ND, R>(un, nrm, gam, fg);
+      }
+#pragma unroll
+      for (int k = 0; k < NF; k++) fn[k] = R(0.5) * (fo[k] + fg[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; k++) un[k] = a.uf_in[(long)nb * NF + k];
+      if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
+      else rusanov<ND, R>(uo, un, nrm, gam, fn);
+    }
+    R jump[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) jump[k] = fn[k] * tdA;
+    if constexpr (VISC) {
+      R nn = 0;
+#pragma unroll
+      for (int l = 0; l < ND; l++) nn += nrm[l] * nrm[l];
+      const R tau = R(a.ph.tau);
+      if (wall) {
+        // Dirichlet LDG flux f_c = f_l + tau n (u_l - u_r) (src/inters.cpp:605-619)
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          const R fa = a.fv_in[fe * NF + k];
+          jump[k] += (fa + tau * nn * (uo[k] - un[k])) * tdA;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NF; k++) {
+          const R fa = a.fv_in[fe * NF + k];
+          const R fb = -a.fv_in[(long)nb * NF + k];
+          jump[k] += ldg_normal<R>(fa, fb, uo[k], un[k], nn, pen, tau) * tdA;
+        }
+      }
+    }
+    // minus the normal transformed discontinuous flux (opp_1 = I o tnorm)
+    R* q = s_q + fp * NC + tl * NF;
+#pragma unroll
+    for (int k = 0; k < NF; k++) q[k] = jump[k] - q[k];
+  }
+  __syncthreads();
+
+  // ---- divergence plus correction lift: [D_0 D_1 D_2 | L] [Ft; jump] ----
+  {
+    V4 acc[M::MT_D][NT];
+    mfma_zero<R>(acc);
+    mfma_seg<R, M::MT_D, NT, M::KS_D, KF / 4>(a.tD, 0, s_f, lane, acc);
+    mfma_seg<R, M::MT_D, NT, M::KS_D, NFP / 4>(a.tD, KF / 4, s_q, lane, acc);
+    __syncthreads();
+    mfma_store<R>(acc, NU, s_q, lane);
+  }
+  __syncthreads();
+
+  // ---- output: div, and the RK update with the new state into LDS ----
+  if (active) {
+    R div[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) div[k] = s_q[p * NC + el * NF + k];
+    const long pe = p + NU * (long)e;
+    if (MODE == MODE_RESID || a.div) {
+#pragma unroll
+      for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = double(div[k]);
+    }
+    if constexpr (MODE == MODE_ADVANCE) {
+      const double idetj = rcp(a.detj[vol_idx(pe, e)]);
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const long idx = pe + NU * NN * k;
+        const double uold = MIX ? a.u[idx] : double(s_u[p * NC + el * NF + k]);
+        double unew;
+        if (a.adv_type == 3) {
+          // low-storage RK45 stage (src/eles.cpp:1155-1178)
+          const double rhs = -double(div[k]) * idetj + a.ph.const_src;
+          const double r = a.rk_a * a.res[idx] + a.dt * rhs;
+          a.res[idx] = r;
+          unew = uold + a.rk_b * r;
+        } else {
+          // forward Euler (src/eles.cpp:1067-1079)
+          unew = uold - a.dt * (double(div[k]) * idetj - a.ph.const_src);
+        }
+        a.u[idx] = unew;
+        s_u[p * NC + el * NF + k] = R(unew - (MIX ? ref_of(k) : 0.0));
+      }
+    }
+  }
+  if constexpr (MODE == MODE_ADVANCE) {
+    __syncthreads();
+    // ---- traces of the new state for the next stage: I u_new, staged in LDS and
+    //      written element by element (each element's traces are contiguous) ----
+    {
+      V4 acc[M::MT_I][NT];
+      mfma_zero<R>(acc);
+      mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_u, lane, acc);
+      mfma_store<R>(acc, NFP, s_q, lane);
+    }
+    __syncthreads();
+    const int nel = (N - blockIdx.x * EPB) < EPB ? (N - blockIdx.x * EPB) : EPB;
+    R* out = a.uf_out + (long)blockIdx.x * EPB * NFP * NF;
+    for (int i = tid; i < nel * NFP * NF; i += 64) {
+      const int tl = i / (NFP * NF), r = i - tl * NFP * NF, f = r / NF, k = r - f * NF;
+      out[i] = s_q[f * NC + tl * NF + k];
+    }
+  }
+}
+
+}  // namespace hifr

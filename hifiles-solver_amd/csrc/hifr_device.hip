@@ -32,6 +32,7 @@ struct hifr_ctx {
   double *u, *res, *div, *detj, *fgeo;
   void *uf[2], *fv, *gf, *jg;  // element type double, or float when mixed
   void *opI, *opD, *opL;       // tets: dense operators (double, or float when mixed)
+  void *tI, *tG, *tN, *tD;     // tets: combined operators as MFMA A-operand tiles
   // learned SGS closure (sgs_model HIFR_SGS_LEARNED): attached network, if any
   hifr::mlp::Weights mlpw;
   hifr::mlp::Norm mlpn;
@@ -109,6 +110,10 @@ StageArgsT<R> base_args(hifr_ctx* c) {
   a.opI = (const R*)c->opI;
   a.opD = (const R*)c->opD;
   a.opL = (const R*)c->opL;
+  a.tI = (const R*)c->tI;
+  a.tG = (const R*)c->tG;
+  a.tN = (const R*)c->tN;
+  a.tD = (const R*)c->tD;
   a.mlpw = c->mlpw;
   for (int f = 0; f < hifr::mlp::NIN; f++) {
     a.mlp_mean[f] = c->mlpn.mean[f];
@@ -296,6 +301,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   c->detj = c->fgeo = nullptr;  // sized by hifr_set_metrics / hifr_set_interfaces
   c->jg = nullptr;
   c->opI = c->opD = c->opL = nullptr;
+  c->tI = c->tG = c->tN = c->tD = nullptr;
   std::memset(&c->mlpw, 0, sizeof(c->mlpw));
   std::memset(&c->mlpn, 0, sizeof(c->mlpn));
   c->mlp_on = 0;
@@ -317,7 +323,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
 hifr_status hifr_destroy(hifr_ctx* c) {
   if (!c) return HIFR_OK;
   void* ps[] = {c->u, c->res, c->div, c->uf[0], c->uf[1], c->fv, c->gf, c->detj, c->jg, c->fgeo, c->nbr, c->red,
-                c->opI, c->opD, c->opL};
+                c->opI, c->opD, c->opL, c->tI, c->tG, c->tN, c->tD};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -346,6 +352,49 @@ namespace {
 
 // Tets: the dense operators in the kernel layouts I [Nfp][Nu], D [d][Nu][Nu],
 // L [Nu][Nfp], in the context's arithmetic type.
+// Operator A (M x K, a(row, col); K a multiple of 4) in the 16x16x4 MFMA A-operand
+// order: tile (mt, ks) holds, for lane l, A[16 mt + l % 16][4 ks + l / 16] (zero for
+// rows >= M), 64 values per tile, tiles mt-major.
+template <class R, class F>
+void mfma_tiles(int M, int K, F a, std::vector<R>& out) {
+  const int MT = (M + 15) / 16, KS = K / 4;
+  out.assign((size_t)MT * KS * 64, R(0));
+  for (int mt = 0; mt < MT; mt++)
+    for (int ks = 0; ks < KS; ks++)
+      for (int l = 0; l < 64; l++) {
+        const int r = 16 * mt + l % 16, col = 4 * ks + l / 16;
+        if (r < M) out[((size_t)mt * KS + ks) * 64 + l] = R(a(r, col));
+      }
+}
+
+// The four combined tet operators of the MFMA flux pass (fr_tet_mfma.hpp), from the
+// reference's opp_0 (I), opp_4 (D) and opp_3 (L) and the reference tet's face
+// normals (face 0: (1,1,1)/sqrt(3); faces 1-3: -e_r, -e_s, -e_t).
+template <int P, class R>
+void tet_tiles(const double* opp_0, const double* opp_3, const double* opp_4, std::vector<R>& tI,
+               std::vector<R>& tG, std::vector<R>& tN, std::vector<R>& tD) {
+  using M = hifr::TetMfma<P>;
+  constexpr int NU = M::NU, NFP = M::NFP, KU = M::KU, KF = M::KF, NFPF = NFP / 4;
+  auto Iv = [&](int f, int m) { return opp_0[f + NFP * m]; };
+  auto Dv = [&](int c, int p, int m) { return opp_4[p + NU * (m + NU * c)]; };
+  auto Lv = [&](int p, int f) { return opp_3[p + NU * f]; };
+  auto tn = [](int face, int c) { return face == 0 ? 0.57735026918962576 : (face - 1 == c ? -1.0 : 0.0); };
+  mfma_tiles<R>(NFP, KU, [&](int f, int m) { return m < NU ? Iv(f, m) : 0.0; }, tI);
+  mfma_tiles<R>(3 * NU, KU + NFP, [&](int r, int col) {
+    const int c = r / NU, p = r % NU;
+    if (col < KU) return col < NU ? Dv(c, p, col) : 0.0;
+    const int f = col - KU;
+    return Lv(p, f) * tn(f / NFPF, c);
+  }, tG);
+  mfma_tiles<R>(NFP, KF, [&](int f, int col) {
+    return col < 3 * NU ? tn(f / NFPF, col / NU) * Iv(f, col % NU) : 0.0;
+  }, tN);
+  mfma_tiles<R>(NU, KF + NFP, [&](int p, int col) {
+    if (col < 3 * NU) return Dv(col / NU, p, col % NU);
+    return col < KF ? 0.0 : Lv(p, col - KF);
+  }, tD);
+}
+
 template <class R>
 hifr_status upload_dense(hifr_ctx* c, const double* opp_0, const double* opp_3, const double* opp_4) {
   const size_t nu = c->nu, nfp = c->nfp, nd = c->nd;
@@ -360,7 +409,15 @@ hifr_status upload_dense(hifr_ctx* c, const double* opp_0, const double* opp_3, 
   for (double v : I) if (!std::isfinite(v)) return HIFR_ERR_ARG;
   for (double v : D) if (!std::isfinite(v)) return HIFR_ERR_ARG;
   for (double v : L) if (!std::isfinite(v)) return HIFR_ERR_ARG;
-  struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->opI, &I}, {&c->opD, &D}, {&c->opL, &L}};
+  std::vector<R> tI, tG, tN, tD;
+  switch (c->cfg.order) {
+    case 1: tet_tiles<1>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+    case 2: tet_tiles<2>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+    default: tet_tiles<3>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+  }
+  struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->opI, &I}, {&c->opD, &D}, {&c->opL, &L},
+                                                          {&c->tI, &tI}, {&c->tG, &tG}, {&c->tN, &tN},
+                                                          {&c->tD, &tD}};
   for (auto& b : bufs) {
     if (*b.p) (void)hipFree(*b.p);
     *b.p = nullptr;
