@@ -94,6 +94,9 @@ hipError_t tet_les(const Args<MIX>& a, hipStream_t s) {
     // flux pass: operator products on MFMA (one wavefront of EPB elements per block)
     static_assert(T::BLOCK == 64, "one wavefront per block");
     hipLaunchKernelGGL((fr_tet_mfma_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(64), 0, s, a);
+  } else if constexpr (MODE == MODE_TRACE_GRAD) {
+    static_assert(T::BLOCK == 64, "one wavefront per block");
+    hipLaunchKernelGGL((fr_tet_mfma_grad_kernel<P, VIS, AFF, MIX, LES>), dim3(grid), dim3(64), 0, s, a);
   } else {
     hipLaunchKernelGGL((fr_dense_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
   }

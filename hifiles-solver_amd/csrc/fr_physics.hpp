@@ -191,6 +191,61 @@ __device__ __forceinline__ void visc_tensor_cols(const T* u, GCol&& gcol, const 
   }
 }
 
+// The viscous tensor of visc_tensor_cols in three steps, for callers that receive
+// the gradient columns in separate passes: primitives of the state, one gradient
+// column at a time, then stress and heat flux.  (visc_tensor_cols keeps its fused
+// form: its callers are register-bound kernels.)
+template <int ND, class T>
+struct ViscPrim {
+  T rho, irho, v[ND], ke, inte;
+};
+template <int ND, class T>
+__device__ __forceinline__ void visc_prim(const T* u, ViscPrim<ND, T>& q) {
+  q.rho = u[0];
+  q.irho = rcp(q.rho);
+  q.ke = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    q.v[i] = u[1 + i] * q.irho;
+    q.ke += q.v[i] * q.v[i];
+  }
+  q.inte = u[ND + 1] * q.irho - T(0.5) * q.ke;
+}
+// dv[i][m] = d v_i / d x_m and de = d e / d x_m from g[k] = d u_k / d x_m
+template <int ND, class T>
+__device__ __forceinline__ void visc_column(const ViscPrim<ND, T>& q, const T* g, T (&dv)[ND][ND], int m, T& de) {
+  T rdv = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    dv[i][m] = (g[1 + i] - g[0] * q.v[i]) * q.irho;
+    rdv += q.v[i] * dv[i][m];
+  }
+  const T dke = T(0.5) * q.ke * g[0] + q.rho * rdv;
+  de = (g[ND + 1] - dke - g[0] * q.inte) * q.irho;
+}
+template <int ND, bool SUTH, class T>
+__device__ __forceinline__ void visc_finish(const ViscPrim<ND, T>& q, const T (&dv)[ND][ND], const T* de,
+                                            const Phys& ph, ViscT<ND, T>& t) {
+  const T mu = viscosity<SUTH, T>(ph, q.inte);
+  T diag = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) diag += dv[i][i];
+  diag = diag * T(1.0 / 3.0);
+#pragma unroll
+  for (int i = 0; i < ND; i++)
+#pragma unroll
+    for (int j = 0; j < ND; j++)
+      t.tau[i][j] = (i == j) ? T(2) * mu * (dv[i][i] - diag) : mu * (dv[i][j] + dv[j][i]);
+  const T kq = mu * T(ph.gamma / ph.prandtl);
+#pragma unroll
+  for (int m = 0; m < ND; m++) {
+    T w = 0;
+#pragma unroll
+    for (int i = 0; i < ND; i++) w += q.v[i] * t.tau[i][m];
+    t.qe[m] = -(w + kq * de[m]);
+  }
+}
+
 // Eddy-viscosity SGS flux at a solution point, as the reference CPU build
 // computes it (src/eles.cpp:2456-2778, no wall model): MODEL 1 Smagorinsky
 // (C_s = 0.1), 2 WALE (C_w = 0.5, eps = 1e-12), turbulent Prandtl number 0.5,

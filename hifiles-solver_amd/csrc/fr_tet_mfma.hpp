@@ -445,4 +445,272 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   }
 }
 
+// Gradient-trace pass (TRACE_GRAD) for tetrahedra: LDG delta, corrected physical
+// gradient at the solution points, then the viscous normal flux F_v(u_f, grad u_f).n
+// at the flux points (plus, with LES, the extrapolated transformed SGS flux dotted
+// with the normal, src/int_inters.cpp:392-427).  The gradient reaches the flux
+// points one direction m at a time: I * (d u / d x_m) is one MFMA product into a
+// [Nfp][NC] LDS buffer, and each flux-point task folds that column into its
+// velocity / energy gradients (registers) before the next product.
+template <int P, int VIS, bool AFF, bool MIX, int LES>
+__global__ void __launch_bounds__(64)
+fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
+  using R = std::conditional_t<MIX, float, double>;
+  using V4 = typename Mfma<R>::v4;
+  constexpr int ND = 3, NF = 5, NFACE = 4;
+  constexpr bool SUTH = VIS == 2;
+  static_assert(VIS != 0, "viscous only");
+  using M = TetMfma<P>;
+  constexpr int NU = M::NU, NFP = M::NFP, NFPF = TetP<P>::NFPF, EPB = M::EPB;
+  constexpr int KU = M::KU, KF = M::KF, NT = M::NT, NC = M::NC;
+  constexpr int KA = KU + NFP;                       // rows of u + delta
+  constexpr int MT = (EPB * NFP + 63) / 64;          // flux-point tasks per thread
+  static_assert(2 * NU + KU <= KF && 2 * NU + KU <= KA, "gradient / SGS blocks fit their regions");
+  __shared__ R smem[(KA + KF + NFP) * NC];
+  R* s_u = smem;              // [KU][NC]  u (deviations on MIX)
+  R* s_q = s_u + KU * NC;     // [NFP][NC] uo -> LDG delta
+  R* s_s = smem;              // [3 Nu][NC] transformed SGS flux, row l Nu + p (LES; over u and delta)
+  R* s_f = smem + KA * NC;    // [KF][NC]  corrected reference gradient -> physical gradient, row m Nu + p
+  R* s_o = s_f + KF * NC;     // [NFP][NC] one column of the gradient (or SGS flux) at the flux points
+
+  const R gam = R(a.ph.gamma);
+  RefState<R> ref;
+  ref.rho = R(a.ref.rho);
+  ref.E = R(a.ref.E);
+  ref.p = R(a.ref.p);
+  ref.H = R(a.ref.H);
+  auto ref_of = [&](int k) -> double { return k == 0 ? a.ref.rho : (k == NF - 1 ? a.ref.E : 0.0); };
+  auto full = [&](const R* w, R* out) {
+    if constexpr (MIX) {
+      add_ref<ND, R>(w, ref, out);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; k++) out[k] = w[k];
+    }
+  };
+  auto ghost = [&](const R* wo, R* wr) {
+    R uo_full[NF];
+    full(wo, uo_full);
+    wall_ghost<ND, R>(uo_full, gam, R(a.ph.R_ref), R(a.ph.T_wall), wr);
+    if constexpr (MIX) {
+      wr[0] -= ref.rho;
+      wr[NF - 1] -= ref.E;
+    }
+  };
+
+  const int N = a.N;
+  const long NN = (long)N;
+  const int tid = threadIdx.x;
+  const int lane = tid;
+  const int el = tid / NU;
+  const int p = tid - el * NU;
+  const int e = blockIdx.x * EPB + el;
+  const bool active = (tid < EPB * NU) && (e < N);
+  auto geo_idx = [&](int fpt, int ee) -> long { return AFF ? (long)(fpt / NFPF) + NFACE * (long)ee
+                                                             : (long)fpt + NFP * (long)ee; };
+  constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
+
+  for (int i = tid; i < (KA + KF + NFP) * NC; i += 64) smem[i] = R(0);
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const double v = a.u[p + NU * (e + NN * k)];
+      s_u[p * NC + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
+    }
+  }
+  __syncthreads();
+
+  // ---- own traces, LDG delta (src/int_inters.cpp:325-346) ----
+  {
+    V4 acc[M::MT_I][NT];
+    mfma_zero<R>(acc);
+    mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_u, lane, acc);
+    mfma_store<R>(acc, NFP, s_q, lane);
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int task = tid; task < EPB * NFP; task += 64) {
+    const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+    if (te >= N) continue;
+    int nb = a.nbr[fp + NFP * te];
+    const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
+    nb &= 0x7fffffff;
+    R* q = s_q + fp * NC + tl * NF;
+    R uo[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) uo[k] = q[k];
+    if (nb & BDY_FLAG) {
+      R ur[NF];
+      ghost(uo, ur);
+#pragma unroll
+      for (int k = 0; k < NF; k++) q[k] = R(0.5) * (ur[k] + uo[k]) - uo[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const R un = a.uf_in[(long)nb * NF + k];
+        const R uc = R(0.5) * (uo[k] + un) - pen * (uo[k] - un);
+        q[k] = uc - uo[k];
+      }
+    }
+  }
+  __syncthreads();
+  // ---- corrected reference gradient (src/eles.cpp:1667-1870) ----
+  {
+    V4 acc[M::MT_G][NT];
+    mfma_zero<R>(acc);
+    mfma_seg<R, M::MT_G, NT, M::KS_G, KU / 4>(a.tG, 0, s_u, lane, acc);
+    mfma_seg<R, M::MT_G, NT, M::KS_G, NFP / 4>(a.tG, KU / 4, s_q, lane, acc);
+    mfma_store<R>(acc, 3 * NU, s_f, lane);
+  }
+  __syncthreads();
+
+  // ---- per solution point: physical gradient over the reference one (same point
+  //      and element), LES: transformed SGS flux over u (own entries) and delta ----
+  if (active) {
+    const long pe = p + NU * (long)e;
+    const long vi = AFF ? (long)e : pe;
+    constexpr long VOL = AFF ? 1 : NU;
+    R gr[NF][ND];
+#pragma unroll
+    for (int k = 0; k < NF; k++)
+#pragma unroll
+      for (int c = 0; c < ND; c++) gr[k][c] = s_f[(c * NU + p) * NC + el * NF + k];
+    R u_own[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) u_own[k] = s_u[p * NC + el * NF + k];
+    R Jg[ND][ND];
+#pragma unroll
+    for (int l = 0; l < ND; l++)
+#pragma unroll
+      for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * VOL + vi];
+    const R inv_detj = R(rcp(a.detj[vi]));
+    R gf[NF][ND];
+#pragma unroll
+    for (int k = 0; k < NF; k++)
+#pragma unroll
+      for (int m = 0; m < ND; m++) {
+        R v = 0;
+#pragma unroll
+        for (int d = 0; d < ND; d++) v += gr[k][d] * (Jg[d][m] * inv_detj);
+        gf[k][m] = v;
+      }
+#pragma unroll
+    for (int k = 0; k < NF; k++)
+#pragma unroll
+      for (int m = 0; m < ND; m++) s_f[(m * NU + p) * NC + el * NF + k] = gf[k][m];
+    if constexpr (LES != 0) {
+      R uu[NF], sgM[ND][ND], sgE[ND];
+      full(u_own, uu);
+      sgs_flux<ND, LES, R>(uu, gf, R(a.detj[vi] * 8. / 6.), R(a.filter_ratio), P, gam, sgM, sgE);
+      // sgsf(k,l) = sum_m JGinv(l,m) F_sgs(k,m)
+#pragma unroll
+      for (int l = 0; l < ND; l++) {
+        R* srow = s_s + (l * NU + p) * NC + el * NF;
+        srow[0] = R(0);
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+          R v = 0;
+#pragma unroll
+          for (int m = 0; m < ND; m++) v += Jg[l][m] * sgM[i][m];
+          srow[1 + i] = v;
+        }
+        R v = 0;
+#pragma unroll
+        for (int m = 0; m < ND; m++) v += Jg[l][m] * sgE[m];
+        srow[NF - 1] = v;
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- the gradient at the flux points, one direction at a time ----
+  R dv[MT][ND][ND], de[MT][ND], fs[MT][NF];
+#pragma unroll
+  for (int t = 0; t < MT; t++)
+#pragma unroll
+    for (int k = 0; k < NF; k++) fs[t][k] = R(0);
+  auto own_prim = [&](int task, ViscPrim<ND, R>& q) {
+    const int tl = task / NFP, fp = task - tl * NFP;
+    const long fe = fp + NFP * (long)(blockIdx.x * EPB + tl);
+    R uo[NF], uu[NF];
+#pragma unroll
+    for (int k = 0; k < NF; k++) uo[k] = a.uf_in[fe * NF + k];
+    full(uo, uu);
+    visc_prim<ND, R>(uu, q);
+  };
+  auto valid = [&](int task) { return task < EPB * NFP && blockIdx.x * EPB + task / NFP < N; };
+#pragma unroll
+  for (int m = 0; m < ND; m++) {
+    {
+      V4 acc[M::MT_I][NT];
+      mfma_zero<R>(acc);
+      mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_f + m * NU * NC, lane, acc);
+      mfma_store<R>(acc, NFP, s_o, lane);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+      const int task = tid + 64 * t;
+      if (!valid(task)) continue;
+      const int tl = task / NFP, fp = task - tl * NFP;
+      R g[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) g[k] = s_o[fp * NC + tl * NF + k];
+      if (a.gf_out) {
+        const long fe = fp + NFP * (long)(blockIdx.x * EPB + tl);
+#pragma unroll
+        for (int k = 0; k < NF; k++) a.gf_out[fe + NFP * NN * (k + NF * m)] = g[k];
+      }
+      ViscPrim<ND, R> q;
+      own_prim(task, q);
+      visc_column<ND, R>(q, g, dv[t], m, de[t][m]);
+    }
+    __syncthreads();
+  }
+  if constexpr (LES != 0) {
+    // extrapolated transformed SGS flux dotted with the physical normal
+#pragma unroll
+    for (int l = 0; l < ND; l++) {
+      {
+        V4 acc[M::MT_I][NT];
+        mfma_zero<R>(acc);
+        mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_s + l * NU * NC, lane, acc);
+        mfma_store<R>(acc, NFP, s_o, lane);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < MT; t++) {
+        const int task = tid + 64 * t;
+        if (!valid(task)) continue;
+        const int tl = task / NFP, fp = task - tl * NFP;
+        const R nl = R(a.fgeo[l * NN * GEO_PER_ELE + geo_idx(fp, blockIdx.x * EPB + tl)]);
+#pragma unroll
+        for (int k = 1; k < NF; k++) fs[t][k] += nl * s_o[fp * NC + tl * NF + k];
+      }
+      __syncthreads();
+    }
+  }
+  // ---- viscous normal flux traces ----
+#pragma unroll
+  for (int t = 0; t < MT; t++) {
+    const int task = tid + 64 * t;
+    if (!valid(task)) continue;
+    const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+    const long fe = fp + NFP * (long)te;
+    const long gi = geo_idx(fp, te);
+    R nrm[ND];
+#pragma unroll
+    for (int l = 0; l < ND; l++) nrm[l] = R(a.fgeo[l * NN * GEO_PER_ELE + gi]);
+    ViscPrim<ND, R> q;
+    own_prim(task, q);
+    ViscT<ND, R> vt;
+    visc_finish<ND, SUTH, R>(q, dv[t], de[t], a.ph, vt);
+    R fa[NF];
+    visc_flux_dot<ND, R>(vt, nrm, fa);
+#pragma unroll
+    for (int k = 0; k < NF; k++) a.fv_out[fe * NF + k] = fa[k] + fs[t][k];
+  }
+}
+
 }  // namespace hifr
