@@ -216,7 +216,7 @@ def main():
     traffic = measured_traffic(args.workload, r["per_element"], "fp64") if (n, order) == (w.get("n", 0), w["order"]) \
         else None
     mtraffic = measured_traffic(args.workload, mixed["per_element"], "mixed") if traffic is not None else None
-    kname = "fr_dense_kernel" if w["kind"] == "tet" else "fr_tensor_kernel"
+    kname = "fr_tet_mfma_kernel" if w["kind"] == "tet" else "fr_tensor_kernel"
     rec = {
         "metric": METRIC, "value": round(r["value"], 4), "unit": "GDOF-stage/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(r["wall"] * 1e3 / args.steps, 4), "higher_is_better": True,

@@ -5,7 +5,6 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "fr_dense_kernels.hpp"
 #include "fr_tensor_kernels.hpp"
 #include "fr_tet_mfma.hpp"
 
@@ -18,7 +17,7 @@ hipError_t launch_fp64(int nd, int P, int mode, int vis, bool aff, int les, cons
                        hipStream_t s);
 hipError_t launch_mixed(int nd, int P, int mode, int vis, bool aff, int les, const StageArgsT<float>& a,
                         hipStream_t s);
-// tetrahedra, P = 1..3 (dense operators)
+// tetrahedra, P = 1..3 (dense operators on MFMA)
 hipError_t launch_tet_fp64(int P, int mode, int vis, bool aff, int les, const StageArgsT<double>& a,
                            hipStream_t s);
 hipError_t launch_tet_mixed(int P, int mode, int vis, bool aff, int les, const StageArgsT<float>& a,

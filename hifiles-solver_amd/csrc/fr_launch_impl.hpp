@@ -98,7 +98,8 @@ hipError_t tet_les(const Args<MIX>& a, hipStream_t s) {
     static_assert(T::BLOCK == 64, "one wavefront per block");
     hipLaunchKernelGGL((fr_tet_mfma_grad_kernel<P, VIS, AFF, MIX, LES>), dim3(grid), dim3(64), 0, s, a);
   } else {
-    hipLaunchKernelGGL((fr_dense_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
+    static_assert(T::BLOCK == 64, "one wavefront per block");
+    hipLaunchKernelGGL((fr_tet_mfma_trace_kernel<P, MIX>), dim3(grid), dim3(64), 0, s, a);
   }
   return hipGetLastError();
 }

@@ -83,12 +83,9 @@ struct StageArgsT {
   Phys ph;
   RefState<double> ref;  // mixed path: reference state of the deviations
   double filter_ratio;   // LES filter width factor
-  // dense element operators (tets): I [Nfp][Nu], D [d][Nu][Nu], L [Nu][Nfp]
-  const R* opI;
-  const R* opD;
-  const R* opL;
-  // the same operators combined and laid out as 16x16x4 MFMA A-operand tiles (tets,
-  // flux pass): I, [D_c | L o tnorm_c], [tnorm_c o I], [D_0 D_1 D_2 | L]
+  // dense element operators of tets (I = opp_0, D = opp_4, L = opp_3), combined and
+  // laid out as 16x16x4 MFMA A-operand tiles: I, [D_c | L o tnorm_c], [tnorm_c o I],
+  // [D_0 D_1 D_2 | L]
   const R* tI;
   const R* tG;
   const R* tN;

@@ -11,16 +11,28 @@
 //   uf   = I                      * u_new      traces of the new state
 // (the reference's opp_0, opp_4 / opp_2, opp_3 / opp_5 / opp_6 products,
 // src/eles.cpp:3620-4052, with tnorm the reference tet's face normals,
-// src/eles_tets.cpp:589-620).  A is streamed from global memory in the MFMA operand
-// order (16 x 4 tiles, 64 values per tile, prepared once by hifr_set_operators), so
-// the operators take no LDS and every load is one coalesced 512 B (FP64) access; B
-// comes from LDS as [row][column] matrices.  The pointwise physics (fluxes, LES,
-// Rusanov and LDG interface fluxes, RK update) is the per-point / per-flux-point
-// code of fr_dense_kernels.hpp.
+// src/eles_tets.cpp:589-620: face 0 (1,1,1)/sqrt(3), faces 1-3 -e_r, -e_s, -e_t).
+// A is streamed from global memory in the MFMA operand order (16 x 4 tiles, 64
+// values per tile, prepared once by hifr_set_operators), so the operators take no
+// LDS and every load is one coalesced 512 B (FP64) access; B comes from LDS as
+// [row][column] matrices.  The pointwise physics between the products (fluxes, LES,
+// Rusanov and LDG interface fluxes, RK update) runs one thread per solution point
+// or per flux point, as in the tensor-product kernels.
 #pragma once
-#include "fr_dense_kernels.hpp"
+#include "fr_tensor_kernels.hpp"
 
 namespace hifr {
+
+// Tet of order P: Nu solution points, 4 faces of Nfpf flux points; one wavefront
+// per workgroup holding EPB = 64 / Nu elements (16, 6, 3 at P = 1, 2, 3).
+template <int P>
+struct TetP {
+  static constexpr int NU = (P + 1) * (P + 2) * (P + 3) / 6;
+  static constexpr int NFPF = (P + 1) * (P + 2) / 2;
+  static constexpr int NFP = 4 * NFPF;
+  static constexpr int EPB = 64 / NU > 0 ? 64 / NU : 1;
+  static constexpr int BLOCK = ((EPB * NU + 63) / 64) * 64;
+};
 
 // 16x16x4 MFMA per precision.  Lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16]; element i of its result is C[row(l, i)][l % 16].
@@ -103,6 +115,53 @@ __device__ __forceinline__ void mfma_store(const typename Mfma<R>::v4 (&acc)[MT]
         for (int nt = 0; nt < NT; nt++) C[r * NC + nt * 16 + (lane & 15)] = acc[mt][nt][i];
       }
     }
+}
+
+// Traces staged in LDS as [f][5 el + k] written to the trace array: the block's
+// elements' traces are one contiguous run, written in order.
+template <int P, class R>
+__device__ __forceinline__ void tet_write_traces(const R* s_q, R* uf_out, int N, int tid) {
+  using M = TetMfma<P>;
+  constexpr int NFP = M::NFP, EPB = M::EPB, NC = M::NC, NF = 5;
+  const int nel = (N - (int)blockIdx.x * EPB) < EPB ? (N - (int)blockIdx.x * EPB) : EPB;
+  R* out = uf_out + (long)blockIdx.x * EPB * NFP * NF;
+  for (int i = tid; i < nel * NFP * NF; i += 64) {
+    const int tl = i / (NFP * NF), r = i - tl * NFP * NF, f = r / NF, k = r - f * NF;
+    out[i] = s_q[f * NC + tl * NF + k];
+  }
+}
+
+// First-stage traces (TRACE_U): uf = I u.
+template <int P, bool MIX>
+__global__ void __launch_bounds__(64)
+fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
+  using R = std::conditional_t<MIX, float, double>;
+  using V4 = typename Mfma<R>::v4;
+  using M = TetMfma<P>;
+  constexpr int NU = M::NU, NFP = M::NFP, EPB = M::EPB, KU = M::KU, NT = M::NT, NC = M::NC, NF = 5;
+  __shared__ R smem[(KU + NFP) * NC];
+  R* s_u = smem;
+  R* s_q = smem + KU * NC;
+  const int tid = threadIdx.x, el = tid / NU, p = tid - el * NU, e = blockIdx.x * EPB + el;
+  const long NN = a.N;
+  for (int i = tid; i < (KU + NFP) * NC; i += 64) smem[i] = R(0);
+  __syncthreads();
+  if (tid < EPB * NU && e < a.N) {
+#pragma unroll
+    for (int k = 0; k < NF; k++) {
+      const double v = a.u[p + NU * (e + NN * k)];
+      s_u[p * NC + el * NF + k] = MIX ? R(v - (k == 0 ? a.ref.rho : (k == NF - 1 ? a.ref.E : 0.0))) : R(v);
+    }
+  }
+  __syncthreads();
+  {
+    V4 acc[M::MT_I][NT];
+    mfma_zero<R>(acc);
+    mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_u, tid, acc);
+    mfma_store<R>(acc, NFP, s_q, tid);
+  }
+  __syncthreads();
+  tet_write_traces<P, R>(s_q, a.uf_out, a.N, tid);
 }
 
 template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES>
@@ -436,12 +495,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       mfma_store<R>(acc, NFP, s_q, lane);
     }
     __syncthreads();
-    const int nel = (N - blockIdx.x * EPB) < EPB ? (N - blockIdx.x * EPB) : EPB;
-    R* out = a.uf_out + (long)blockIdx.x * EPB * NFP * NF;
-    for (int i = tid; i < nel * NFP * NF; i += 64) {
-      const int tl = i / (NFP * NF), r = i - tl * NFP * NF, f = r / NF, k = r - f * NF;
-      out[i] = s_q[f * NC + tl * NF + k];
-    }
+    tet_write_traces<P, R>(s_q, a.uf_out, N, tid);
   }
 }
 

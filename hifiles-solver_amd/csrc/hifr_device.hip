@@ -31,8 +31,8 @@ struct hifr_ctx {
   // device buffers
   double *u, *res, *div, *detj, *fgeo;
   void *uf[2], *fv, *gf, *jg;  // element type double, or float when mixed
-  void *opI, *opD, *opL;       // tets: dense operators (double, or float when mixed)
-  void *tI, *tG, *tN, *tD;     // tets: combined operators as MFMA A-operand tiles
+  void *tI, *tG, *tN, *tD;     // tets: combined dense operators as MFMA A-operand tiles
+                               // (double, or float when mixed)
   // learned SGS closure (sgs_model HIFR_SGS_LEARNED): attached network, if any
   hifr::mlp::Weights mlpw;
   hifr::mlp::Norm mlpn;
@@ -107,9 +107,6 @@ StageArgsT<R> base_args(hifr_ctx* c) {
   a.ph.T_wall = c->cfg.T_wall;
   a.ref = c->ref;
   a.filter_ratio = c->cfg.filter_ratio;
-  a.opI = (const R*)c->opI;
-  a.opD = (const R*)c->opD;
-  a.opL = (const R*)c->opL;
   a.tI = (const R*)c->tI;
   a.tG = (const R*)c->tG;
   a.tN = (const R*)c->tN;
@@ -300,7 +297,6 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   for (auto& al : allocs) *al.p = nullptr;
   c->detj = c->fgeo = nullptr;  // sized by hifr_set_metrics / hifr_set_interfaces
   c->jg = nullptr;
-  c->opI = c->opD = c->opL = nullptr;
   c->tI = c->tG = c->tN = c->tD = nullptr;
   std::memset(&c->mlpw, 0, sizeof(c->mlpw));
   std::memset(&c->mlpn, 0, sizeof(c->mlpn));
@@ -323,7 +319,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
 hifr_status hifr_destroy(hifr_ctx* c) {
   if (!c) return HIFR_OK;
   void* ps[] = {c->u, c->res, c->div, c->uf[0], c->uf[1], c->fv, c->gf, c->detj, c->jg, c->fgeo, c->nbr, c->red,
-                c->opI, c->opD, c->opL, c->tI, c->tG, c->tN, c->tD};
+                c->tI, c->tG, c->tN, c->tD};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -398,25 +394,17 @@ void tet_tiles(const double* opp_0, const double* opp_3, const double* opp_4, st
 template <class R>
 hifr_status upload_dense(hifr_ctx* c, const double* opp_0, const double* opp_3, const double* opp_4) {
   const size_t nu = c->nu, nfp = c->nfp, nd = c->nd;
-  std::vector<R> I(nfp * nu), D(nd * nu * nu), L(nu * nfp);
-  for (size_t f = 0; f < nfp; f++)
-    for (size_t j = 0; j < nu; j++) I[f * nu + j] = R(opp_0[f + nfp * j]);
-  for (size_t d = 0; d < nd; d++)
-    for (size_t p = 0; p < nu; p++)
-      for (size_t m = 0; m < nu; m++) D[(d * nu + p) * nu + m] = R(opp_4[p + nu * (m + nu * d)]);
-  for (size_t p = 0; p < nu; p++)
-    for (size_t f = 0; f < nfp; f++) L[p * nfp + f] = R(opp_3[p + nu * f]);
-  for (double v : I) if (!std::isfinite(v)) return HIFR_ERR_ARG;
-  for (double v : D) if (!std::isfinite(v)) return HIFR_ERR_ARG;
-  for (double v : L) if (!std::isfinite(v)) return HIFR_ERR_ARG;
+  for (size_t i = 0; i < nfp * nu; i++)
+    if (!std::isfinite(opp_0[i]) || !std::isfinite(opp_3[i])) return HIFR_ERR_ARG;
+  for (size_t i = 0; i < nd * nu * nu; i++)
+    if (!std::isfinite(opp_4[i])) return HIFR_ERR_ARG;
   std::vector<R> tI, tG, tN, tD;
   switch (c->cfg.order) {
     case 1: tet_tiles<1>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
     case 2: tet_tiles<2>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
     default: tet_tiles<3>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
   }
-  struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->opI, &I}, {&c->opD, &D}, {&c->opL, &L},
-                                                          {&c->tI, &tI}, {&c->tG, &tG}, {&c->tN, &tN},
+  struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->tI, &tI}, {&c->tG, &tG}, {&c->tN, &tN},
                                                           {&c->tD, &tD}};
   for (auto& b : bufs) {
     if (*b.p) (void)hipFree(*b.p);
