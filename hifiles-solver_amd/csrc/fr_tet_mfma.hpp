@@ -64,6 +64,9 @@ struct TetMfma {
   static constexpr int KU = (NU + 3) / 4 * 4;          // rows of u (zero-padded)
   static constexpr int KF = (3 * NU + 3) / 4 * 4;      // rows of the flux / gradient
   static constexpr int NT = (5 * EPB + 15) / 16, NC = 16 * NT;
+  // row pitch of the LDS matrices: one more than NC, so that the per-point and
+  // per-flux-point passes (one row per lane) spread over the LDS banks
+  static constexpr int LD = NC + 1;
   static constexpr int MT_I = (NFP + 15) / 16, KS_I = KU / 4;              // I
   static constexpr int MT_G = (3 * NU + 15) / 16, KS_G = (KU + NFP) / 4;  // [D_c | L tnorm_c]
   static constexpr int MT_N = (NFP + 15) / 16, KS_N = KF / 4;              // [tnorm_c I]
@@ -74,16 +77,16 @@ struct TetMfma {
 };
 
 // acc[mt][nt] += A (tiles mt, k-steps ks0 .. ks0 + KS - 1 of an operator with KSA
-// k-steps) * B (KS * 4 rows of an LDS matrix with NC columns).
+// k-steps) * B (KS * 4 rows of an LDS matrix with NC = 16 NT columns, pitch NC + 1).
 template <class R, int MT, int NT, int KSA, int KS>
 __device__ __forceinline__ void mfma_seg(const R* __restrict__ A, int ks0, const R* B, int lane,
                                          typename Mfma<R>::v4 (&acc)[MT][NT]) {
-  constexpr int NC = 16 * NT;
+  constexpr int LD = 16 * NT + 1;
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
     R b[NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) b[nt] = B[(ks * 4 + (lane >> 4)) * NC + nt * 16 + (lane & 15)];
+    for (int nt = 0; nt < NT; nt++) b[nt] = B[(ks * 4 + (lane >> 4)) * LD + nt * 16 + (lane & 15)];
 #pragma unroll
     for (int mt = 0; mt < MT; mt++) {
       const R av = A[(mt * KSA + ks0 + ks) * 64 + lane];
@@ -101,10 +104,10 @@ __device__ __forceinline__ void mfma_zero(typename Mfma<R>::v4 (&acc)[MT][NT]) {
     for (int nt = 0; nt < NT; nt++) acc[mt][nt] = typename Mfma<R>::v4{0, 0, 0, 0};
 }
 
-// C rows < M into an LDS matrix with NC columns
+// C rows < M into an LDS matrix with NC = 16 NT columns (pitch NC + 1)
 template <class R, int MT, int NT>
 __device__ __forceinline__ void mfma_store(const typename Mfma<R>::v4 (&acc)[MT][NT], int M, R* C, int lane) {
-  constexpr int NC = 16 * NT;
+  constexpr int LD = 16 * NT + 1;
 #pragma unroll
   for (int mt = 0; mt < MT; mt++)
 #pragma unroll
@@ -112,7 +115,7 @@ __device__ __forceinline__ void mfma_store(const typename Mfma<R>::v4 (&acc)[MT]
       const int r = mt * 16 + Mfma<R>::row(lane, i);
       if (r < M) {
 #pragma unroll
-        for (int nt = 0; nt < NT; nt++) C[r * NC + nt * 16 + (lane & 15)] = acc[mt][nt][i];
+        for (int nt = 0; nt < NT; nt++) C[r * LD + nt * 16 + (lane & 15)] = acc[mt][nt][i];
       }
     }
 }
@@ -122,12 +125,12 @@ __device__ __forceinline__ void mfma_store(const typename Mfma<R>::v4 (&acc)[MT]
 template <int P, class R>
 __device__ __forceinline__ void tet_write_traces(const R* s_q, R* uf_out, int N, int tid) {
   using M = TetMfma<P>;
-  constexpr int NFP = M::NFP, EPB = M::EPB, NC = M::NC, NF = 5;
+  constexpr int NFP = M::NFP, EPB = M::EPB, LD = M::LD, NF = 5;
   const int nel = (N - (int)blockIdx.x * EPB) < EPB ? (N - (int)blockIdx.x * EPB) : EPB;
   R* out = uf_out + (long)blockIdx.x * EPB * NFP * NF;
   for (int i = tid; i < nel * NFP * NF; i += 64) {
     const int tl = i / (NFP * NF), r = i - tl * NFP * NF, f = r / NF, k = r - f * NF;
-    out[i] = s_q[f * NC + tl * NF + k];
+    out[i] = s_q[f * LD + tl * NF + k];
   }
 }
 
@@ -138,19 +141,19 @@ fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>
   using R = std::conditional_t<MIX, float, double>;
   using V4 = typename Mfma<R>::v4;
   using M = TetMfma<P>;
-  constexpr int NU = M::NU, NFP = M::NFP, EPB = M::EPB, KU = M::KU, NT = M::NT, NC = M::NC, NF = 5;
-  __shared__ R smem[(KU + NFP) * NC];
+  constexpr int NU = M::NU, NFP = M::NFP, EPB = M::EPB, KU = M::KU, NT = M::NT, LD = M::LD, NF = 5;
+  __shared__ R smem[(KU + NFP) * LD];
   R* s_u = smem;
-  R* s_q = smem + KU * NC;
+  R* s_q = smem + KU * LD;
   const int tid = threadIdx.x, el = tid / NU, p = tid - el * NU, e = blockIdx.x * EPB + el;
   const long NN = a.N;
-  for (int i = tid; i < (KU + NFP) * NC; i += 64) smem[i] = R(0);
+  for (int i = tid; i < (KU + NFP) * LD; i += 64) smem[i] = R(0);
   __syncthreads();
   if (tid < EPB * NU && e < a.N) {
 #pragma unroll
     for (int k = 0; k < NF; k++) {
       const double v = a.u[p + NU * (e + NN * k)];
-      s_u[p * NC + el * NF + k] = MIX ? R(v - (k == 0 ? a.ref.rho : (k == NF - 1 ? a.ref.E : 0.0))) : R(v);
+      s_u[p * LD + el * NF + k] = MIX ? R(v - (k == 0 ? a.ref.rho : (k == NF - 1 ? a.ref.E : 0.0))) : R(v);
     }
   }
   __syncthreads();
@@ -176,11 +179,11 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   static_assert(LES == 0 || VISC, "LES needs the viscous kernels");
   using M = TetMfma<P>;
   constexpr int NU = M::NU, NFP = M::NFP, NFPF = TetP<P>::NFPF, EPB = M::EPB;
-  constexpr int KU = M::KU, KF = M::KF, NT = M::NT, NC = M::NC;
-  __shared__ R smem[M::ROWS * NC];
-  R* s_u = smem;              // [KU][NC]   u (deviations on MIX), then the new state
-  R* s_q = s_u + KU * NC;     // [NFP][NC]  uo -> LDG delta; fn -> jump - fn; div (rows < Nu)
-  R* s_f = s_q + NFP * NC;    // [KF][NC]   corrected reference gradient -> transformed flux
+  constexpr int KU = M::KU, KF = M::KF, NT = M::NT, LD = M::LD;
+  __shared__ R smem[M::ROWS * LD];
+  R* s_u = smem;              // [KU][LD]   u (deviations on MIX), then the new state
+  R* s_q = s_u + KU * LD;     // [NFP][LD]  uo -> LDG delta; fn -> jump - fn; div (rows < Nu)
+  R* s_f = s_q + NFP * LD;    // [KF][LD]   corrected reference gradient -> transformed flux
 
   const R gam = R(a.ph.gamma);
   RefState<R> ref;
@@ -221,13 +224,13 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
   // ---- zero the LDS (padding rows and columns stay zero), then u ----
-  for (int i = tid; i < M::ROWS * NC; i += 64) smem[i] = R(0);
+  for (int i = tid; i < M::ROWS * LD; i += 64) smem[i] = R(0);
   __syncthreads();
   if (active) {
 #pragma unroll
     for (int k = 0; k < NF; k++) {
       const double v = a.u[p + NU * (e + NN * k)];
-      s_u[p * NC + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
+      s_u[p * LD + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
     }
   }
   __syncthreads();
@@ -249,7 +252,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       int nb = a.nbr[fp + NFP * te];
       const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
       nb &= 0x7fffffff;
-      R* q = s_q + fp * NC + tl * NF;
+      R* q = s_q + fp * LD + tl * NF;
       R uo[NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) uo[k] = q[k];
@@ -293,7 +296,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * VOL + vi];
     R u_own[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) u_own[k] = s_u[p * NC + el * NF + k];
+    for (int k = 0; k < NF; k++) u_own[k] = s_u[p * LD + el * NF + k];
     ViscT<ND, R> vt;
     if constexpr (VISC) {
       const R inv_detj = R(rcp(a.detj[vi]));
@@ -301,7 +304,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
       for (int k = 0; k < NF; k++)
 #pragma unroll
-        for (int c = 0; c < ND; c++) gr[k][c] = s_f[(c * NU + p) * NC + el * NF + k];
+        for (int c = 0; c < ND; c++) gr[k][c] = s_f[(c * NU + p) * LD + el * NF + k];
       R Ji[ND][ND];
 #pragma unroll
       for (int d = 0; d < ND; d++)
@@ -356,7 +359,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         for (int k = 0; k < NF; k++) fi[k] += fv[k];
       }
 #pragma unroll
-      for (int k = 0; k < NF; k++) s_f[(l * NU + p) * NC + el * NF + k] = fi[k];
+      for (int k = 0; k < NF; k++) s_f[(l * NU + p) * LD + el * NF + k] = fi[k];
     }
   }
   __syncthreads();
@@ -435,7 +438,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       }
     }
     // minus the normal transformed discontinuous flux (opp_1 = I o tnorm)
-    R* q = s_q + fp * NC + tl * NF;
+    R* q = s_q + fp * LD + tl * NF;
 #pragma unroll
     for (int k = 0; k < NF; k++) q[k] = jump[k] - q[k];
   }
@@ -456,7 +459,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   if (active) {
     R div[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) div[k] = s_q[p * NC + el * NF + k];
+    for (int k = 0; k < NF; k++) div[k] = s_q[p * LD + el * NF + k];
     const long pe = p + NU * (long)e;
     if (MODE == MODE_RESID || a.div) {
 #pragma unroll
@@ -467,7 +470,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         const long idx = pe + NU * NN * k;
-        const double uold = MIX ? a.u[idx] : double(s_u[p * NC + el * NF + k]);
+        const double uold = MIX ? a.u[idx] : double(s_u[p * LD + el * NF + k]);
         double unew;
         if (a.adv_type == 3) {
           // low-storage RK45 stage (src/eles.cpp:1155-1178)
@@ -480,7 +483,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           unew = uold - a.dt * (double(div[k]) * idetj - a.ph.const_src);
         }
         a.u[idx] = unew;
-        s_u[p * NC + el * NF + k] = R(unew - (MIX ? ref_of(k) : 0.0));
+        s_u[p * LD + el * NF + k] = R(unew - (MIX ? ref_of(k) : 0.0));
       }
     }
   }
@@ -504,7 +507,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 // at the flux points (plus, with LES, the extrapolated transformed SGS flux dotted
 // with the normal, src/int_inters.cpp:392-427).  The gradient reaches the flux
 // points one direction m at a time: I * (d u / d x_m) is one MFMA product into a
-// [Nfp][NC] LDS buffer, and each flux-point task folds that column into its
+// [Nfp][LD] LDS buffer, and each flux-point task folds that column into its
 // velocity / energy gradients (registers) before the next product.
 template <int P, int VIS, bool AFF, bool MIX, int LES>
 __global__ void __launch_bounds__(64)
@@ -516,16 +519,16 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   static_assert(VIS != 0, "viscous only");
   using M = TetMfma<P>;
   constexpr int NU = M::NU, NFP = M::NFP, NFPF = TetP<P>::NFPF, EPB = M::EPB;
-  constexpr int KU = M::KU, KF = M::KF, NT = M::NT, NC = M::NC;
+  constexpr int KU = M::KU, KF = M::KF, NT = M::NT, LD = M::LD;
   constexpr int KA = KU + NFP;                       // rows of u + delta
   constexpr int MT = (EPB * NFP + 63) / 64;          // flux-point tasks per thread
   static_assert(2 * NU + KU <= KF && 2 * NU + KU <= KA, "gradient / SGS blocks fit their regions");
-  __shared__ R smem[(KA + KF + NFP) * NC];
-  R* s_u = smem;              // [KU][NC]  u (deviations on MIX)
-  R* s_q = s_u + KU * NC;     // [NFP][NC] uo -> LDG delta
-  R* s_s = smem;              // [3 Nu][NC] transformed SGS flux, row l Nu + p (LES; over u and delta)
-  R* s_f = smem + KA * NC;    // [KF][NC]  corrected reference gradient -> physical gradient, row m Nu + p
-  R* s_o = s_f + KF * NC;     // [NFP][NC] one column of the gradient (or SGS flux) at the flux points
+  __shared__ R smem[(KA + KF + NFP) * LD];
+  R* s_u = smem;              // [KU][LD]  u (deviations on MIX)
+  R* s_q = s_u + KU * LD;     // [NFP][LD] uo -> LDG delta
+  R* s_s = smem;              // [3 Nu][LD] transformed SGS flux, row l Nu + p (LES; over u and delta)
+  R* s_f = smem + KA * LD;    // [KF][LD]  corrected reference gradient -> physical gradient, row m Nu + p
+  R* s_o = s_f + KF * LD;     // [NFP][LD] one column of the gradient (or SGS flux) at the flux points
 
   const R gam = R(a.ph.gamma);
   RefState<R> ref;
@@ -564,13 +567,13 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
                                                              : (long)fpt + NFP * (long)ee; };
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
-  for (int i = tid; i < (KA + KF + NFP) * NC; i += 64) smem[i] = R(0);
+  for (int i = tid; i < (KA + KF + NFP) * LD; i += 64) smem[i] = R(0);
   __syncthreads();
   if (active) {
 #pragma unroll
     for (int k = 0; k < NF; k++) {
       const double v = a.u[p + NU * (e + NN * k)];
-      s_u[p * NC + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
+      s_u[p * LD + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
     }
   }
   __syncthreads();
@@ -590,7 +593,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
     int nb = a.nbr[fp + NFP * te];
     const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
     nb &= 0x7fffffff;
-    R* q = s_q + fp * NC + tl * NF;
+    R* q = s_q + fp * LD + tl * NF;
     R uo[NF];
 #pragma unroll
     for (int k = 0; k < NF; k++) uo[k] = q[k];
@@ -629,10 +632,10 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
 #pragma unroll
     for (int k = 0; k < NF; k++)
 #pragma unroll
-      for (int c = 0; c < ND; c++) gr[k][c] = s_f[(c * NU + p) * NC + el * NF + k];
+      for (int c = 0; c < ND; c++) gr[k][c] = s_f[(c * NU + p) * LD + el * NF + k];
     R u_own[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) u_own[k] = s_u[p * NC + el * NF + k];
+    for (int k = 0; k < NF; k++) u_own[k] = s_u[p * LD + el * NF + k];
     R Jg[ND][ND];
 #pragma unroll
     for (int l = 0; l < ND; l++)
@@ -652,7 +655,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
 #pragma unroll
     for (int k = 0; k < NF; k++)
 #pragma unroll
-      for (int m = 0; m < ND; m++) s_f[(m * NU + p) * NC + el * NF + k] = gf[k][m];
+      for (int m = 0; m < ND; m++) s_f[(m * NU + p) * LD + el * NF + k] = gf[k][m];
     if constexpr (LES != 0) {
       R uu[NF], sgM[ND][ND], sgE[ND];
       full(u_own, uu);
@@ -660,7 +663,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
       // sgsf(k,l) = sum_m JGinv(l,m) F_sgs(k,m)
 #pragma unroll
       for (int l = 0; l < ND; l++) {
-        R* srow = s_s + (l * NU + p) * NC + el * NF;
+        R* srow = s_s + (l * NU + p) * LD + el * NF;
         srow[0] = R(0);
 #pragma unroll
         for (int i = 0; i < ND; i++) {
@@ -699,7 +702,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
     {
       V4 acc[M::MT_I][NT];
       mfma_zero<R>(acc);
-      mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_f + m * NU * NC, lane, acc);
+      mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_f + m * NU * LD, lane, acc);
       mfma_store<R>(acc, NFP, s_o, lane);
     }
     __syncthreads();
@@ -710,7 +713,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
       const int tl = task / NFP, fp = task - tl * NFP;
       R g[NF];
 #pragma unroll
-      for (int k = 0; k < NF; k++) g[k] = s_o[fp * NC + tl * NF + k];
+      for (int k = 0; k < NF; k++) g[k] = s_o[fp * LD + tl * NF + k];
       if (a.gf_out) {
         const long fe = fp + NFP * (long)(blockIdx.x * EPB + tl);
 #pragma unroll
@@ -729,7 +732,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
       {
         V4 acc[M::MT_I][NT];
         mfma_zero<R>(acc);
-        mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_s + l * NU * NC, lane, acc);
+        mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_s + l * NU * LD, lane, acc);
         mfma_store<R>(acc, NFP, s_o, lane);
       }
       __syncthreads();
@@ -740,7 +743,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
         const int tl = task / NFP, fp = task - tl * NFP;
         const R nl = R(a.fgeo[l * NN * GEO_PER_ELE + geo_idx(fp, blockIdx.x * EPB + tl)]);
 #pragma unroll
-        for (int k = 1; k < NF; k++) fs[t][k] += nl * s_o[fp * NC + tl * NF + k];
+        for (int k = 1; k < NF; k++) fs[t][k] += nl * s_o[fp * LD + tl * NF + k];
       }
       __syncthreads();
     }
