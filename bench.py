@@ -123,10 +123,10 @@ def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
 MEASURED_KIB = {
     ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (280706.3, 122881.4), "rk": (410887.3, 297642.8)},
     ("tgv32", False, "mixed"): {"n_eles": 32768, "grad": (181605.0, 61440.4), "rk": (332173.5, 236202.7)},
-    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (156359.2, 122880.6), "rk": (274721.0, 297642.9)},
-    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (99827.2, 61440.1), "rk": (223965.1, 236202.7)},
-    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (48130.3, 32400.0), "rk": (97337.0, 66960.1)},
-    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (33531.0, 16200.0), "rk": (77868.5, 50760.0)},
+    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (156317.8, 122880.8), "rk": (275776.2, 297642.7)},
+    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (99759.1, 61440.0), "rk": (223859.8, 236202.7)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (48035.9, 32400.0), "rk": (97272.6, 66960.0)},
+    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (33434.4, 16200.0), "rk": (78449.9, 50760.0)},
 }
 
 
