@@ -313,7 +313,13 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     }
   }
   if constexpr (LES == 3 && MODE != MODE_TRACE_U) mlp::stage(a.mlpw, s_mlp, tid, BLOCK);
-  __syncthreads();
+  // EARLY: the LDG delta pass takes the own state trace from the trace array, not
+  // from u in LDS, so it needs no barrier after the state load: the state load
+  // and the neighbour gathers are in flight together, and one barrier covers both
+  // (1-4 % faster).  Not in the FP64 advance kernels up to P = 3, which would
+  // spill under their register budget.
+  constexpr bool EARLY = VISC && MODE != MODE_TRACE_U && LES != 3 && !(MODE == MODE_ADVANCE && !MIX && P <= 3);
+  if constexpr (!EARLY) __syncthreads();
 
   if constexpr (MODE == MODE_TRACE_U) {
     // ---- traces of u only (start of the first stage) ----
@@ -344,7 +350,12 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
         nb &= 0x7fffffff;
         R uo[NF];
-        interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
+        if constexpr (EARLY) {
+#pragma unroll
+          for (int k = 0; k < NF; k++) uo[k] = a.uf_in[(fp + NFP * (long)te) * NF + k];
+        } else {
+          interp_line<n, NF, NU>(s_u + tl * NF * NU, side ? ops.lR : ops.lL, base, T::stride(dir), uo);
+        }
         if (nb & BDY_FLAG) {
           // wall: Dirichlet common solution u_c = (u_l + u_r)/2 (src/inters.cpp:664-668)
           R ur[NF];
