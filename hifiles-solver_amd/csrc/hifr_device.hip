@@ -402,7 +402,8 @@ hifr_status upload_dense(hifr_ctx* c, const double* opp_0, const double* opp_3, 
   switch (c->cfg.order) {
     case 1: tet_tiles<1>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
     case 2: tet_tiles<2>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
-    default: tet_tiles<3>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+    case 3: tet_tiles<3>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+    default: return HIFR_ERR_UNSUPPORTED;  // hifr_create admits tets of order 1-3 only
   }
   struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->tI, &tI}, {&c->tG, &tG}, {&c->tN, &tN},
                                                           {&c->tD, &tD}};
