@@ -58,6 +58,20 @@ def test_residual_matches_reference_and_oracle(require_gpu, name, metrics):
             assert cases.field_rel_err(gs[..., d], go[..., d]).max() < TOL
 
 
+@pytest.mark.parametrize("name", cases.golden_names("residual"))
+def test_forward_euler_step_matches_reference(require_gpu, name):
+    """The residual goldens come from one reference forward-Euler step with a huge
+    dt (u1 = u0 - dt R); one device step of the same deck (adv_type 0, the
+    advance kernels' forward-Euler branch) recovers R = (u0 - u1) / dt."""
+    g = cases.load_golden(name)
+    s, _ = _pair(g)
+    print("case", name, "dt", float(g["dt"]), flush=True)
+    s.set_state(g["u0"])
+    s.run(1)
+    R = (g["u0"] - s.get("disu_upts")) / float(g["dt"])
+    assert cases.field_rel_err(R, g["R"]).max() < TOL
+
+
 @pytest.mark.parametrize("name", cases.golden_names("rk45"))
 def test_rk45_matches_reference(require_gpu, name):
     g = cases.load_golden(name)
