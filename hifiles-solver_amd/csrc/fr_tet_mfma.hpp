@@ -76,11 +76,43 @@ struct TetMfma {
   static constexpr int ROWS = KU + NFP + KF;  // LDS rows: u, q (delta / fn / jump / div), flux
 };
 
-// acc[mt][nt] += A (tiles mt, k-steps ks0 .. ks0 + KS - 1 of an operator with KSA
+// Combined operators of the tet products (host tile order in hifr_set_operators).
+enum TetOp { TET_OP_I = 0, TET_OP_G = 1, TET_OP_N = 2, TET_OP_D = 3 };
+
+// Whether tile (mt, ks) of operator OP can hold a nonzero at order P.  The
+// reference tet's face normals give [L o tnorm_c] (in G) and [tnorm_c o I] (N) zero
+// blocks for the faces other than 0 and c + 1; I and [D | L] are dense.
+template <int P, int OP>
+__host__ __device__ constexpr bool tet_tile_nz(int mt, int ks) {
+  constexpr int NU = TetP<P>::NU, NFPF = TetP<P>::NFPF, NFP = TetP<P>::NFP, KU = (NU + 3) / 4 * 4;
+  if (OP != TET_OP_G && OP != TET_OP_N) return true;
+  for (int r = 16 * mt; r < 16 * mt + 16; r++)
+    for (int col = 4 * ks; col < 4 * ks + 4; col++) {
+      if (OP == TET_OP_G) {
+        if (r >= 3 * NU) continue;
+        if (col < KU) {
+          if (col < NU) return true;
+          continue;
+        }
+        const int face = (col - KU) / NFPF, c = r / NU;
+        if (face == 0 || face - 1 == c) return true;
+      } else {
+        if (r >= NFP || col >= 3 * NU) continue;
+        const int face = r / NFPF, c = col / NU;
+        if (face == 0 || face - 1 == c) return true;
+      }
+    }
+  return false;
+}
+
+// acc[mt][nt] += A (tiles mt, k-steps KS0 .. KS0 + KS - 1 of operator OP with KSA
 // k-steps) * B (KS * 4 rows of an LDS matrix with NC = 16 NT columns, pitch NC + 1).
-template <class R, int MT, int NT, int KSA, int KS>
-__device__ __forceinline__ void mfma_seg(const R* __restrict__ A, int ks0, const R* B, int lane,
+// Tiles that tet_tile_nz rules out are skipped at compile time (the loops unroll
+// to straight-line code without them).
+template <class R, int P, int OP, int MT, int NT, int KSA, int KS0, int KS>
+__device__ __forceinline__ void mfma_seg(const R* __restrict__ A, const R* B, int lane,
                                          typename Mfma<R>::v4 (&acc)[MT][NT]) {
+  constexpr int ks0 = KS0;
   constexpr int LD = 16 * NT + 1;
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
@@ -89,6 +121,7 @@ __device__ __forceinline__ void mfma_seg(const R* __restrict__ A, int ks0, const
     for (int nt = 0; nt < NT; nt++) b[nt] = B[(ks * 4 + (lane >> 4)) * LD + nt * 16 + (lane & 15)];
 #pragma unroll
     for (int mt = 0; mt < MT; mt++) {
+      if (!tet_tile_nz<P, OP>(mt, ks0 + ks)) continue;
       const R av = A[(mt * KSA + ks0 + ks) * 64 + lane];
 #pragma unroll
       for (int nt = 0; nt < NT; nt++) acc[mt][nt] = Mfma<R>::op(av, b[nt], acc[mt][nt]);
@@ -160,7 +193,7 @@ fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>
   {
     V4 acc[M::MT_I][NT];
     mfma_zero<R>(acc);
-    mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_u, tid, acc);
+    mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_u, tid, acc);
     mfma_store<R>(acc, NFP, s_q, tid);
   }
   __syncthreads();
@@ -241,7 +274,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     {
       V4 acc[M::MT_I][NT];
       mfma_zero<R>(acc);
-      mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_u, lane, acc);
+      mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_u, lane, acc);
       mfma_store<R>(acc, NFP, s_q, lane);
     }
     __syncthreads();
@@ -276,8 +309,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     {
       V4 acc[M::MT_G][NT];
       mfma_zero<R>(acc);
-      mfma_seg<R, M::MT_G, NT, M::KS_G, KU / 4>(a.tG, 0, s_u, lane, acc);
-      mfma_seg<R, M::MT_G, NT, M::KS_G, NFP / 4>(a.tG, KU / 4, s_q, lane, acc);
+      mfma_seg<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, 0, KU / 4>(a.tG, s_u, lane, acc);
+      mfma_seg<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4>(a.tG, s_q, lane, acc);
       mfma_store<R>(acc, 3 * NU, s_f, lane);
     }
     __syncthreads();
@@ -368,7 +401,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   {
     V4 acc[M::MT_N][NT];
     mfma_zero<R>(acc);
-    mfma_seg<R, M::MT_N, NT, M::KS_N, M::KS_N>(a.tN, 0, s_f, lane, acc);
+    mfma_seg<R, P, TET_OP_N, M::MT_N, NT, M::KS_N, 0, M::KS_N>(a.tN, s_f, lane, acc);
     mfma_store<R>(acc, NFP, s_q, lane);
   }
   __syncthreads();
@@ -448,8 +481,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   {
     V4 acc[M::MT_D][NT];
     mfma_zero<R>(acc);
-    mfma_seg<R, M::MT_D, NT, M::KS_D, KF / 4>(a.tD, 0, s_f, lane, acc);
-    mfma_seg<R, M::MT_D, NT, M::KS_D, NFP / 4>(a.tD, KF / 4, s_q, lane, acc);
+    mfma_seg<R, P, TET_OP_D, M::MT_D, NT, M::KS_D, 0, KF / 4>(a.tD, s_f, lane, acc);
+    mfma_seg<R, P, TET_OP_D, M::MT_D, NT, M::KS_D, KF / 4, NFP / 4>(a.tD, s_q, lane, acc);
     __syncthreads();
     mfma_store<R>(acc, NU, s_q, lane);
   }
@@ -494,7 +527,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     {
       V4 acc[M::MT_I][NT];
       mfma_zero<R>(acc);
-      mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_u, lane, acc);
+      mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_u, lane, acc);
       mfma_store<R>(acc, NFP, s_q, lane);
     }
     __syncthreads();
@@ -582,7 +615,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   {
     V4 acc[M::MT_I][NT];
     mfma_zero<R>(acc);
-    mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_u, lane, acc);
+    mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_u, lane, acc);
     mfma_store<R>(acc, NFP, s_q, lane);
   }
   __syncthreads();
@@ -616,8 +649,8 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   {
     V4 acc[M::MT_G][NT];
     mfma_zero<R>(acc);
-    mfma_seg<R, M::MT_G, NT, M::KS_G, KU / 4>(a.tG, 0, s_u, lane, acc);
-    mfma_seg<R, M::MT_G, NT, M::KS_G, NFP / 4>(a.tG, KU / 4, s_q, lane, acc);
+    mfma_seg<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, 0, KU / 4>(a.tG, s_u, lane, acc);
+    mfma_seg<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4>(a.tG, s_q, lane, acc);
     mfma_store<R>(acc, 3 * NU, s_f, lane);
   }
   __syncthreads();
@@ -702,7 +735,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
     {
       V4 acc[M::MT_I][NT];
       mfma_zero<R>(acc);
-      mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_f + m * NU * LD, lane, acc);
+      mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_f + m * NU * LD, lane, acc);
       mfma_store<R>(acc, NFP, s_o, lane);
     }
     __syncthreads();
@@ -732,7 +765,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
       {
         V4 acc[M::MT_I][NT];
         mfma_zero<R>(acc);
-        mfma_seg<R, M::MT_I, NT, M::KS_I, M::KS_I>(a.tI, 0, s_s + l * NU * LD, lane, acc);
+        mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_s + l * NU * LD, lane, acc);
         mfma_store<R>(acc, NFP, s_o, lane);
       }
       __syncthreads();

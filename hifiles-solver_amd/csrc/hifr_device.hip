@@ -367,7 +367,7 @@ void mfma_tiles(int M, int K, F a, std::vector<R>& out) {
 // reference's opp_0 (I), opp_4 (D) and opp_3 (L) and the reference tet's face
 // normals (face 0: (1,1,1)/sqrt(3); faces 1-3: -e_r, -e_s, -e_t).
 template <int P, class R>
-void tet_tiles(const double* opp_0, const double* opp_3, const double* opp_4, std::vector<R>& tI,
+bool tet_tiles(const double* opp_0, const double* opp_3, const double* opp_4, std::vector<R>& tI,
                std::vector<R>& tG, std::vector<R>& tN, std::vector<R>& tD) {
   using M = hifr::TetMfma<P>;
   constexpr int NU = M::NU, NFP = M::NFP, KU = M::KU, KF = M::KF, NFPF = NFP / 4;
@@ -389,6 +389,16 @@ void tet_tiles(const double* opp_0, const double* opp_3, const double* opp_4, st
     if (col < 3 * NU) return Dv(col / NU, p, col % NU);
     return col < KF ? 0.0 : Lv(p, col - KF);
   }, tD);
+  // the kernels skip the tiles that the face normals zero (hifr::tet_tile_nz)
+  auto skipped_are_zero = [](const std::vector<R>& t, int KS, auto nz) {
+    for (size_t i = 0; i < t.size(); i++) {
+      const int tile = (int)(i / 64);
+      if (!nz(tile / KS, tile % KS) && t[i] != R(0)) return false;
+    }
+    return true;
+  };
+  return skipped_are_zero(tG, M::KS_G, [](int mt, int ks) { return hifr::tet_tile_nz<P, hifr::TET_OP_G>(mt, ks); }) &&
+         skipped_are_zero(tN, M::KS_N, [](int mt, int ks) { return hifr::tet_tile_nz<P, hifr::TET_OP_N>(mt, ks); });
 }
 
 template <class R>
@@ -399,12 +409,14 @@ hifr_status upload_dense(hifr_ctx* c, const double* opp_0, const double* opp_3, 
   for (size_t i = 0; i < nd * nu * nu; i++)
     if (!std::isfinite(opp_4[i])) return HIFR_ERR_ARG;
   std::vector<R> tI, tG, tN, tD;
+  bool ok = false;
   switch (c->cfg.order) {
-    case 1: tet_tiles<1>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
-    case 2: tet_tiles<2>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
-    case 3: tet_tiles<3>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+    case 1: ok = tet_tiles<1>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+    case 2: ok = tet_tiles<2>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+    case 3: ok = tet_tiles<3>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
     default: return HIFR_ERR_UNSUPPORTED;  // hifr_create admits tets of order 1-3 only
   }
+  if (!ok) return HIFR_ERR_ARG;
   struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->tI, &tI}, {&c->tG, &tG}, {&c->tN, &tN},
                                                           {&c->tD, &tD}};
   for (auto& b : bufs) {
