@@ -111,7 +111,9 @@ hifr_status hifr_set_stream(hifr_ctx* ctx, void* hip_stream);
  * (src/eles.cpp:3700, 3771, 3956, 4018).  Replaces the opp_* cp_cpu_gpu uploads and
  * every cublasDgemm/bespoke_SPMV of src/eles.cpp.  For quads and hexes the
  * matrices must have the tensor-product structure of Lagrange bases on a 1-D
- * point set; the context keeps only their 1-D factors. */
+ * point set; the context keeps only their 1-D factors.  For tets it keeps the
+ * dense matrices combined with the reference tet's face normals, as MFMA operand
+ * tiles; HIFR_ERR_ARG if an entry is not finite. */
 hifr_status hifr_set_operators(hifr_ctx* ctx, const double* opp_0, const double* opp_3,
                                const double* opp_4);
 
