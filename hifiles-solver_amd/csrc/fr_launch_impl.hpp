@@ -91,15 +91,12 @@ hipError_t tet_les(const Args<MIX>& a, hipStream_t s) {
   using T = TetP<P>;
   const int grid = (a.N + T::EPB - 1) / T::EPB;
   if constexpr (MODE == MODE_RESID || MODE == MODE_ADVANCE) {
-    // flux pass: operator products on MFMA (one wavefront of EPB elements per block)
-    static_assert(T::BLOCK == 64, "one wavefront per block");
-    hipLaunchKernelGGL((fr_tet_mfma_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(64), 0, s, a);
+    // flux pass: operator products on MFMA (EPB elements per block)
+    hipLaunchKernelGGL((fr_tet_mfma_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
   } else if constexpr (MODE == MODE_TRACE_GRAD) {
-    static_assert(T::BLOCK == 64, "one wavefront per block");
-    hipLaunchKernelGGL((fr_tet_mfma_grad_kernel<P, VIS, AFF, MIX, LES>), dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((fr_tet_mfma_grad_kernel<P, VIS, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
   } else {
-    static_assert(T::BLOCK == 64, "one wavefront per block");
-    hipLaunchKernelGGL((fr_tet_mfma_trace_kernel<P, MIX>), dim3(grid), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((fr_tet_mfma_trace_kernel<P, MIX>), dim3(grid), dim3(T::BLOCK), 0, s, a);
   }
   return hipGetLastError();
 }

@@ -23,15 +23,19 @@
 
 namespace hifr {
 
-// Tet of order P: Nu solution points, 4 faces of Nfpf flux points; one wavefront
-// per workgroup holding EPB = 64 / Nu elements (16, 6, 3 at P = 1, 2, 3).
+// Tet of order P: Nu solution points, 4 faces of Nfpf flux points.  A workgroup
+// of WAVES wavefronts holds EPB = 64 WAVES / Nu elements (16, 6, 3 at P = 1, 2, 3
+// with one).  Two wavefronts, splitting the row tiles of every product so that an
+// operator tile feeds twice the elements, measured 12-22 % slower at P = 3 (FP64):
+// twice the LDS per block, barriers across waves, 3 row tiles split 2 + 1.
 template <int P>
 struct TetP {
   static constexpr int NU = (P + 1) * (P + 2) * (P + 3) / 6;
   static constexpr int NFPF = (P + 1) * (P + 2) / 2;
   static constexpr int NFP = 4 * NFPF;
-  static constexpr int EPB = 64 / NU > 0 ? 64 / NU : 1;
-  static constexpr int BLOCK = ((EPB * NU + 63) / 64) * 64;
+  static constexpr int WAVES = 1;
+  static constexpr int BLOCK = 64 * WAVES;
+  static constexpr int EPB = BLOCK / NU;
 };
 
 // 16x16x4 MFMA per precision.  Lane l supplies A[l % 16][l / 16] and
@@ -105,14 +109,13 @@ __host__ __device__ constexpr bool tet_tile_nz(int mt, int ks) {
   return false;
 }
 
-// acc[mt][nt] += A (tiles mt, k-steps KS0 .. KS0 + KS - 1 of operator OP with KSA
-// k-steps) * B (KS * 4 rows of an LDS matrix with NC = 16 NT columns, pitch NC + 1).
-// Tiles that tet_tile_nz rules out are skipped at compile time (the loops unroll
-// to straight-line code without them).
-template <class R, int P, int OP, int MT, int NT, int KSA, int KS0, int KS>
+// acc[j][nt] += A (row tiles MT0 + j, j < CNT; k-steps KS0 .. KS0 + KS - 1 of
+// operator OP with KSA k-steps) * B (KS * 4 rows of an LDS matrix with NC = 16 NT
+// columns, pitch NC + 1).  Tiles that tet_tile_nz rules out are skipped at compile
+// time (the loops unroll to straight-line code without them).
+template <class R, int P, int OP, int MT0, int CNT, int H, int NT, int KSA, int KS0, int KS>
 __device__ __forceinline__ void mfma_seg(const R* __restrict__ A, const R* B, int lane,
-                                         typename Mfma<R>::v4 (&acc)[MT][NT]) {
-  constexpr int ks0 = KS0;
+                                         typename Mfma<R>::v4 (&acc)[H][NT]) {
   constexpr int LD = 16 * NT + 1;
 #pragma unroll
   for (int ks = 0; ks < KS; ks++) {
@@ -120,37 +123,57 @@ __device__ __forceinline__ void mfma_seg(const R* __restrict__ A, const R* B, in
 #pragma unroll
     for (int nt = 0; nt < NT; nt++) b[nt] = B[(ks * 4 + (lane >> 4)) * LD + nt * 16 + (lane & 15)];
 #pragma unroll
-    for (int mt = 0; mt < MT; mt++) {
-      if (!tet_tile_nz<P, OP>(mt, ks0 + ks)) continue;
-      const R av = A[(mt * KSA + ks0 + ks) * 64 + lane];
+    for (int j = 0; j < CNT; j++) {
+      if (!tet_tile_nz<P, OP>(MT0 + j, KS0 + ks)) continue;
+      const R av = A[((MT0 + j) * KSA + KS0 + ks) * 64 + lane];
 #pragma unroll
-      for (int nt = 0; nt < NT; nt++) acc[mt][nt] = Mfma<R>::op(av, b[nt], acc[mt][nt]);
+      for (int nt = 0; nt < NT; nt++) acc[j][nt] = Mfma<R>::op(av, b[nt], acc[j][nt]);
     }
   }
 }
 
-template <class R, int MT, int NT>
-__device__ __forceinline__ void mfma_zero(typename Mfma<R>::v4 (&acc)[MT][NT]) {
-#pragma unroll
-  for (int mt = 0; mt < MT; mt++)
-#pragma unroll
-    for (int nt = 0; nt < NT; nt++) acc[mt][nt] = typename Mfma<R>::v4{0, 0, 0, 0};
-}
-
-// C rows < M into an LDS matrix with NC = 16 NT columns (pitch NC + 1)
-template <class R, int MT, int NT>
-__device__ __forceinline__ void mfma_store(const typename Mfma<R>::v4 (&acc)[MT][NT], int M, R* C, int lane) {
+// rows < M of row tiles MT0 + j (j < CNT) into an LDS matrix (pitch 16 NT + 1)
+template <class R, int MT0, int CNT, int H, int NT>
+__device__ __forceinline__ void mfma_store(const typename Mfma<R>::v4 (&acc)[H][NT], int M, R* C, int lane) {
   constexpr int LD = 16 * NT + 1;
 #pragma unroll
-  for (int mt = 0; mt < MT; mt++)
+  for (int j = 0; j < CNT; j++)
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      const int r = mt * 16 + Mfma<R>::row(lane, i);
+      const int r = (MT0 + j) * 16 + Mfma<R>::row(lane, i);
       if (r < M) {
 #pragma unroll
-        for (int nt = 0; nt < NT; nt++) C[r * LD + nt * 16 + (lane & 15)] = acc[mt][nt][i];
+        for (int nt = 0; nt < NT; nt++) C[r * LD + nt * 16 + (lane & 15)] = acc[j][nt][i];
       }
     }
+}
+
+// One product C (rows < M) = A * [B1 (4 KS1 rows); B2 (4 KS2 rows)] of the block.
+// With two wavefronts, wave 0 computes row tiles [0, H) and wave 1 [H, MT),
+// H = ceil(MT / 2): one wave-uniform branch around straight-line code.  SYNC: a
+// barrier between the products and the stores (C overwrites B).
+template <class R, int P, int OP, int MT, int NT, int KSA, int KS1, int KS2, bool SYNC = false>
+__device__ __forceinline__ void tet_product(const R* __restrict__ A, const R* B1, const R* B2, int M, R* C,
+                                            int tid) {
+  using V4 = typename Mfma<R>::v4;
+  constexpr int W = TetP<P>::WAVES, H = (MT + W - 1) / W;
+  const int lane = tid & 63;
+  const bool first = W == 1 || tid < 64;
+  V4 acc[H][NT];
+#pragma unroll
+  for (int j = 0; j < H; j++)
+#pragma unroll
+    for (int nt = 0; nt < NT; nt++) acc[j][nt] = V4{0, 0, 0, 0};
+  if (first) {
+    mfma_seg<R, P, OP, 0, H, H, NT, KSA, 0, KS1>(A, B1, lane, acc);
+    if constexpr (KS2 > 0) mfma_seg<R, P, OP, 0, H, H, NT, KSA, KS1, KS2>(A, B2, lane, acc);
+  } else {
+    mfma_seg<R, P, OP, H, MT - H, H, NT, KSA, 0, KS1>(A, B1, lane, acc);
+    if constexpr (KS2 > 0) mfma_seg<R, P, OP, H, MT - H, H, NT, KSA, KS1, KS2>(A, B2, lane, acc);
+  }
+  if constexpr (SYNC) __syncthreads();
+  if (first) mfma_store<R, 0, H, H, NT>(acc, M, C, lane);
+  else mfma_store<R, H, MT - H, H, NT>(acc, M, C, lane);
 }
 
 // Traces staged in LDS as [f][5 el + k] written to the trace array: the block's
@@ -161,7 +184,7 @@ __device__ __forceinline__ void tet_write_traces(const R* s_q, R* uf_out, int N,
   constexpr int NFP = M::NFP, EPB = M::EPB, LD = M::LD, NF = 5;
   const int nel = (N - (int)blockIdx.x * EPB) < EPB ? (N - (int)blockIdx.x * EPB) : EPB;
   R* out = uf_out + (long)blockIdx.x * EPB * NFP * NF;
-  for (int i = tid; i < nel * NFP * NF; i += 64) {
+  for (int i = tid; i < nel * NFP * NF; i += TetP<P>::BLOCK) {
     const int tl = i / (NFP * NF), r = i - tl * NFP * NF, f = r / NF, k = r - f * NF;
     out[i] = s_q[f * LD + tl * NF + k];
   }
@@ -169,10 +192,9 @@ __device__ __forceinline__ void tet_write_traces(const R* s_q, R* uf_out, int N,
 
 // First-stage traces (TRACE_U): uf = I u.
 template <int P, bool MIX>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(TetP<P>::BLOCK)
 fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
-  using V4 = typename Mfma<R>::v4;
   using M = TetMfma<P>;
   constexpr int NU = M::NU, NFP = M::NFP, EPB = M::EPB, KU = M::KU, NT = M::NT, LD = M::LD, NF = 5;
   __shared__ R smem[(KU + NFP) * LD];
@@ -180,7 +202,7 @@ fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>
   R* s_q = smem + KU * LD;
   const int tid = threadIdx.x, el = tid / NU, p = tid - el * NU, e = blockIdx.x * EPB + el;
   const long NN = a.N;
-  for (int i = tid; i < (KU + NFP) * LD; i += 64) smem[i] = R(0);
+  for (int i = tid; i < (KU + NFP) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
   __syncthreads();
   if (tid < EPB * NU && e < a.N) {
 #pragma unroll
@@ -190,21 +212,16 @@ fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>
     }
   }
   __syncthreads();
-  {
-    V4 acc[M::MT_I][NT];
-    mfma_zero<R>(acc);
-    mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_u, tid, acc);
-    mfma_store<R>(acc, NFP, s_q, tid);
-  }
+  tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
+      a.tI, s_u, nullptr, NFP, s_q, tid);
   __syncthreads();
   tet_write_traces<P, R>(s_q, a.uf_out, a.N, tid);
 }
 
 template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(TetP<P>::BLOCK)
 fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
-  using V4 = typename Mfma<R>::v4;
   constexpr int ND = 3, NF = 5, NFACE = 4;
   constexpr bool VISC = VIS != 0;
   constexpr bool SUTH = VIS == 2;
@@ -246,7 +263,6 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   const int N = a.N;
   const long NN = (long)N;
   const int tid = threadIdx.x;
-  const int lane = tid;
   const int el = tid / NU;
   const int p = tid - el * NU;
   const int e = blockIdx.x * EPB + el;
@@ -257,7 +273,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
   // ---- zero the LDS (padding rows and columns stay zero), then u ----
-  for (int i = tid; i < M::ROWS * LD; i += 64) smem[i] = R(0);
+  for (int i = tid; i < M::ROWS * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
   __syncthreads();
   if (active) {
 #pragma unroll
@@ -271,15 +287,11 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   if constexpr (VISC) {
     // ---- own traces uo = I u, then the LDG delta = u_c - uo at each flux point
     //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346) ----
-    {
-      V4 acc[M::MT_I][NT];
-      mfma_zero<R>(acc);
-      mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_u, lane, acc);
-      mfma_store<R>(acc, NFP, s_q, lane);
-    }
+    tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
+        a.tI, s_u, nullptr, NFP, s_q, tid);
     __syncthreads();
 #pragma unroll 1
-    for (int task = tid; task < EPB * NFP; task += 64) {
+    for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
       const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
       if (te >= N) continue;
       int nb = a.nbr[fp + NFP * te];
@@ -306,13 +318,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     }
     __syncthreads();
     // ---- corrected reference gradient: D_c u + (L o tnorm_c) delta (src/eles.cpp:1667-1870) ----
-    {
-      V4 acc[M::MT_G][NT];
-      mfma_zero<R>(acc);
-      mfma_seg<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, 0, KU / 4>(a.tG, s_u, lane, acc);
-      mfma_seg<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4>(a.tG, s_q, lane, acc);
-      mfma_store<R>(acc, 3 * NU, s_f, lane);
-    }
+    tet_product<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4>(
+        a.tG, s_u, s_q, 3 * NU, s_f, tid);
     __syncthreads();
   }
 
@@ -398,18 +405,14 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   __syncthreads();
 
   // ---- normal transformed flux at the flux points: fn = (tnorm o I) Ft ----
-  {
-    V4 acc[M::MT_N][NT];
-    mfma_zero<R>(acc);
-    mfma_seg<R, P, TET_OP_N, M::MT_N, NT, M::KS_N, 0, M::KS_N>(a.tN, s_f, lane, acc);
-    mfma_store<R>(acc, NFP, s_q, lane);
-  }
+  tet_product<R, P, TET_OP_N, M::MT_N, NT, M::KS_N, M::KS_N, 0>(
+      a.tN, s_f, nullptr, NFP, s_q, tid);
   __syncthreads();
 
   // ---- interface fluxes and jumps at the flux points
   //      (common_invFlux / common_viscFlux, extrapolate_totalFlux, daxpy) ----
 #pragma unroll 1
-  for (int task = tid; task < EPB * NFP; task += 64) {
+  for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
     const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
     if (te >= N) continue;
     const long fe = fp + NFP * (long)te;
@@ -478,14 +481,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   __syncthreads();
 
   // ---- divergence plus correction lift: [D_0 D_1 D_2 | L] [Ft; jump] ----
-  {
-    V4 acc[M::MT_D][NT];
-    mfma_zero<R>(acc);
-    mfma_seg<R, P, TET_OP_D, M::MT_D, NT, M::KS_D, 0, KF / 4>(a.tD, s_f, lane, acc);
-    mfma_seg<R, P, TET_OP_D, M::MT_D, NT, M::KS_D, KF / 4, NFP / 4>(a.tD, s_q, lane, acc);
-    __syncthreads();
-    mfma_store<R>(acc, NU, s_q, lane);
-  }
+  tet_product<R, P, TET_OP_D, M::MT_D, NT, M::KS_D, KF / 4, NFP / 4, true>(
+      a.tD, s_f, s_q, NU, s_q, tid);
   __syncthreads();
 
   // ---- output: div, and the RK update with the new state into LDS ----
@@ -524,12 +521,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     __syncthreads();
     // ---- traces of the new state for the next stage: I u_new, staged in LDS and
     //      written element by element (each element's traces are contiguous) ----
-    {
-      V4 acc[M::MT_I][NT];
-      mfma_zero<R>(acc);
-      mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_u, lane, acc);
-      mfma_store<R>(acc, NFP, s_q, lane);
-    }
+    tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
+        a.tI, s_u, nullptr, NFP, s_q, tid);
     __syncthreads();
     tet_write_traces<P, R>(s_q, a.uf_out, N, tid);
   }
@@ -543,10 +536,9 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 // [Nfp][LD] LDS buffer, and each flux-point task folds that column into its
 // velocity / energy gradients (registers) before the next product.
 template <int P, int VIS, bool AFF, bool MIX, int LES>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(TetP<P>::BLOCK)
 fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
-  using V4 = typename Mfma<R>::v4;
   constexpr int ND = 3, NF = 5, NFACE = 4;
   constexpr bool SUTH = VIS == 2;
   static_assert(VIS != 0, "viscous only");
@@ -554,7 +546,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   constexpr int NU = M::NU, NFP = M::NFP, NFPF = TetP<P>::NFPF, EPB = M::EPB;
   constexpr int KU = M::KU, KF = M::KF, NT = M::NT, LD = M::LD;
   constexpr int KA = KU + NFP;                       // rows of u + delta
-  constexpr int MT = (EPB * NFP + 63) / 64;          // flux-point tasks per thread
+  constexpr int MT = (EPB * NFP + TetP<P>::BLOCK - 1) / TetP<P>::BLOCK;  // flux-point tasks per thread
   static_assert(2 * NU + KU <= KF && 2 * NU + KU <= KA, "gradient / SGS blocks fit their regions");
   __shared__ R smem[(KA + KF + NFP) * LD];
   R* s_u = smem;              // [KU][LD]  u (deviations on MIX)
@@ -591,7 +583,6 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   const int N = a.N;
   const long NN = (long)N;
   const int tid = threadIdx.x;
-  const int lane = tid;
   const int el = tid / NU;
   const int p = tid - el * NU;
   const int e = blockIdx.x * EPB + el;
@@ -600,7 +591,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
                                                              : (long)fpt + NFP * (long)ee; };
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
-  for (int i = tid; i < (KA + KF + NFP) * LD; i += 64) smem[i] = R(0);
+  for (int i = tid; i < (KA + KF + NFP) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
   __syncthreads();
   if (active) {
 #pragma unroll
@@ -612,15 +603,11 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   __syncthreads();
 
   // ---- own traces, LDG delta (src/int_inters.cpp:325-346) ----
-  {
-    V4 acc[M::MT_I][NT];
-    mfma_zero<R>(acc);
-    mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_u, lane, acc);
-    mfma_store<R>(acc, NFP, s_q, lane);
-  }
+  tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
+      a.tI, s_u, nullptr, NFP, s_q, tid);
   __syncthreads();
 #pragma unroll 1
-  for (int task = tid; task < EPB * NFP; task += 64) {
+  for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
     const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
     if (te >= N) continue;
     int nb = a.nbr[fp + NFP * te];
@@ -646,13 +633,8 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   }
   __syncthreads();
   // ---- corrected reference gradient (src/eles.cpp:1667-1870) ----
-  {
-    V4 acc[M::MT_G][NT];
-    mfma_zero<R>(acc);
-    mfma_seg<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, 0, KU / 4>(a.tG, s_u, lane, acc);
-    mfma_seg<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4>(a.tG, s_q, lane, acc);
-    mfma_store<R>(acc, 3 * NU, s_f, lane);
-  }
+  tet_product<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4>(
+      a.tG, s_u, s_q, 3 * NU, s_f, tid);
   __syncthreads();
 
   // ---- per solution point: physical gradient over the reference one (same point
@@ -732,16 +714,12 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   auto valid = [&](int task) { return task < EPB * NFP && blockIdx.x * EPB + task / NFP < N; };
 #pragma unroll
   for (int m = 0; m < ND; m++) {
-    {
-      V4 acc[M::MT_I][NT];
-      mfma_zero<R>(acc);
-      mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_f + m * NU * LD, lane, acc);
-      mfma_store<R>(acc, NFP, s_o, lane);
-    }
+    tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
+        a.tI, s_f + m * NU * LD, nullptr, NFP, s_o, tid);
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < MT; t++) {
-      const int task = tid + 64 * t;
+      const int task = tid + TetP<P>::BLOCK * t;
       if (!valid(task)) continue;
       const int tl = task / NFP, fp = task - tl * NFP;
       R g[NF];
@@ -762,16 +740,12 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
     // extrapolated transformed SGS flux dotted with the physical normal
 #pragma unroll
     for (int l = 0; l < ND; l++) {
-      {
-        V4 acc[M::MT_I][NT];
-        mfma_zero<R>(acc);
-        mfma_seg<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, 0, M::KS_I>(a.tI, s_s + l * NU * LD, lane, acc);
-        mfma_store<R>(acc, NFP, s_o, lane);
-      }
+      tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
+          a.tI, s_s + l * NU * LD, nullptr, NFP, s_o, tid);
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < MT; t++) {
-        const int task = tid + 64 * t;
+        const int task = tid + TetP<P>::BLOCK * t;
         if (!valid(task)) continue;
         const int tl = task / NFP, fp = task - tl * NFP;
         const R nl = R(a.fgeo[l * NN * GEO_PER_ELE + geo_idx(fp, blockIdx.x * EPB + tl)]);
@@ -784,7 +758,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   // ---- viscous normal flux traces ----
 #pragma unroll
   for (int t = 0; t < MT; t++) {
-    const int task = tid + 64 * t;
+    const int task = tid + TetP<P>::BLOCK * t;
     if (!valid(task)) continue;
     const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
     const long fe = fp + NFP * (long)te;
