@@ -791,6 +791,19 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         else s_u[(el * NF + k) * NU + p] = v;
       };
       if (active) {
+        // the RK register and detJ are requested before the divergence, so that
+        // their latency overlaps it (the compiler cannot move these loads above
+        // the div store by itself)
+        [[maybe_unused]] double resv[NF];
+        [[maybe_unused]] double detjv = 0.0;
+        if constexpr (MODE == MODE_ADVANCE) {
+          const long pe0 = p + NU * (long)e;
+          detjv = a.detj[vol_idx(pe0, e)];
+          if (a.adv_type == 3) {
+#pragma unroll
+            for (int k = 0; k < NF; k++) resv[k] = a.res[pe0 + NU * NN * k];
+          }
+        }
         R div[NF];
 #pragma unroll
         for (int k = 0; k < NF; k++) div[k] = 0;
@@ -813,14 +826,14 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = double(div[k]);
         }
         if constexpr (MODE == MODE_ADVANCE) {
-          const double idetj = rcp(a.detj[vol_idx(pe, e)]);
+          const double idetj = rcp(detjv);
           if (a.adv_type == 3) {
             // low-storage RK45 stage (src/eles.cpp:1155-1178)
 #pragma unroll
             for (int k = 0; k < NF; k++) {
               const long idx = pe + NU * NN * k;
               const double rhs = -double(div[k]) * idetj + a.ph.const_src;
-              const double r = a.rk_a * a.res[idx] + a.dt * rhs;
+              const double r = a.rk_a * resv[k] + a.dt * rhs;
               a.res[idx] = r;
               const double unew = old_u(k, idx) + a.rk_b * r;
               a.u[idx] = unew;
