@@ -480,6 +480,26 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   }
   __syncthreads();
 
+  // the RK register and detJ are requested before the divergence product, so that
+  // their latency overlaps it
+  // (and, on the mixed path, the FP64 state)
+  [[maybe_unused]] double resv[NF], uoldv[NF];
+  [[maybe_unused]] double detjv = 0.0;
+  if constexpr (MODE == MODE_ADVANCE) {
+    if (active) {
+      const long pe0 = p + NU * (long)e;
+      detjv = a.detj[vol_idx(pe0, e)];
+      if (a.adv_type == 3) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) resv[k] = a.res[pe0 + NU * NN * k];
+      }
+      if constexpr (MIX) {
+#pragma unroll
+        for (int k = 0; k < NF; k++) uoldv[k] = a.u[pe0 + NU * NN * k];
+      }
+    }
+  }
+
   // ---- divergence plus correction lift: [D_0 D_1 D_2 | L] [Ft; jump] ----
   tet_product<R, P, TET_OP_D, M::MT_D, NT, M::KS_D, KF / 4, NFP / 4, true>(
       a.tD, s_f, s_q, NU, s_q, tid);
@@ -496,16 +516,16 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = double(div[k]);
     }
     if constexpr (MODE == MODE_ADVANCE) {
-      const double idetj = rcp(a.detj[vol_idx(pe, e)]);
+      const double idetj = rcp(detjv);
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         const long idx = pe + NU * NN * k;
-        const double uold = MIX ? a.u[idx] : double(s_u[p * LD + el * NF + k]);
+        const double uold = MIX ? uoldv[k] : double(s_u[p * LD + el * NF + k]);
         double unew;
         if (a.adv_type == 3) {
           // low-storage RK45 stage (src/eles.cpp:1155-1178)
           const double rhs = -double(div[k]) * idetj + a.ph.const_src;
-          const double r = a.rk_a * a.res[idx] + a.dt * rhs;
+          const double r = a.rk_a * resv[k] + a.dt * rhs;
           a.res[idx] = r;
           unew = uold + a.rk_b * r;
         } else {
