@@ -819,7 +819,9 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       if (active) {
         // the RK register and detJ are requested before the divergence, so that
         // their latency overlaps it (the compiler cannot move these loads above
-        // the div store by itself)
+        // the div store by itself).  The RK register is next read two kernels
+        // later: its loads and stores are non-temporal, which leaves the caches
+        // to the traces and the state that the next kernel reads
         [[maybe_unused]] double resv[NF];
         [[maybe_unused]] double detjv = 0.0;
         if constexpr (MODE == MODE_ADVANCE) {
@@ -827,7 +829,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           detjv = detj_at(vol_idx(pe0, e));
           if (a.adv_type == 3) {
 #pragma unroll
-            for (int k = 0; k < NF; k++) resv[k] = a.res[pe0 + NU * NN * k];
+            for (int k = 0; k < NF; k++) resv[k] = __builtin_nontemporal_load(&a.res[pe0 + NU * NN * k]);
           }
         }
         R div[NF];
@@ -860,7 +862,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
               const long idx = pe + NU * NN * k;
               const double rhs = -double(div[k]) * idetj + a.ph.const_src;
               const double r = a.rk_a * resv[k] + a.dt * rhs;
-              a.res[idx] = r;
+              __builtin_nontemporal_store(r, &a.res[idx]);
               const double unew = old_u(k, idx) + a.rk_b * r;
               a.u[idx] = unew;
               put_u(k, unew);
