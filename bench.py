@@ -124,10 +124,10 @@ def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
 MEASURED_KIB = {
     ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (280706.3, 122881.4), "rk": (410887.3, 297642.8)},
     ("tgv32", False, "mixed"): {"n_eles": 32768, "grad": (181605.0, 61440.4), "rk": (332173.5, 236202.7)},
-    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (153975.5, 122880.7), "rk": (287261.1, 297643.2)},
-    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (108284.9, 61440.4), "rk": (238355.2, 236203.0)},
-    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (48001.3, 32400.0), "rk": (99375.4, 66960.0)},
-    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (33389.5, 16200.0), "rk": (79455.3, 50760.0)},
+    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (154141.9, 122880.9), "rk": (278271.2, 297642.8)},
+    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (108851.5, 61440.3), "rk": (231665.1, 236203.1)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (48003.6, 32400.0), "rk": (99380.2, 66960.0)},
+    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (33392.2, 16200.0), "rk": (79386.7, 50760.0)},
 }
 
 
