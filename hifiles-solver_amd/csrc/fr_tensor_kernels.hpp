@@ -340,7 +340,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       s_u[(el * NF + k) * NU + p] = MIX ? R(v - ref_of(k)) : R(v);
     }
   }
-  if constexpr (LES == 3 && MODE != MODE_TRACE_U) mlp::stage(a.mlpw, s_mlp, tid, BLOCK);
+  // the network's weights are read only when one is attached (a.mlp_on is the
+  // same for every thread of the launch)
+  if constexpr (LES == 3 && MODE != MODE_TRACE_U)
+    if (a.mlp_on) mlp::stage(a.mlpw, s_mlp, tid, BLOCK);
   // EARLY: the LDG delta pass takes the own state trace from the trace array, not
   // from u in LDS, so it needs no barrier after the state load: the state load
   // and the neighbour gathers are in flight together, and one barrier covers both
@@ -479,7 +482,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         for (int f = 0; f < mlp::NIN; f++) s_x[tid * mlp::NIN + f] = 0.0f;
       }
       __syncthreads();
-      {
+      if (a.mlp_on) {
         const int lane = tid & 63, wv = tid >> 6;
 #pragma unroll 1
         for (int t4 = 0; t4 < 4; t4++) {

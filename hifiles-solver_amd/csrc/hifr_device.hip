@@ -11,6 +11,7 @@
 #include "../../include/hifr.h"
 #include "../../include/hifr_sgs.h"
 #include "fr_launch.hpp"
+#include "fr_launch_check.hpp"
 #include "hifr_ctx_view.hpp"
 #include "sgs_mlp_state.hpp"
 
@@ -33,7 +34,10 @@ struct hifr_ctx {
   void *uf[2], *fv, *gf, *jg;  // element type double, or float when mixed
   void *tI, *tG, *tN, *tD;     // tets: combined dense operators as MFMA A-operand tiles
                                // (double, or float when mixed)
-  // learned SGS closure (sgs_model HIFR_SGS_LEARNED): attached network, if any
+  // learned SGS closure (sgs_model HIFR_SGS_LEARNED): the context's own copy of the
+  // attached network's weights (mlp_buf, zero while none is attached), which the
+  // LES kernels read through mlpw
+  void* mlp_buf;
   hifr::mlp::Weights mlpw;
   hifr::mlp::Norm mlpn;
   int mlp_on;
@@ -65,6 +69,20 @@ hifr_status dev_err(hipError_t e, const char* what) {
     hifr_status _s = dev_err((x), #x);                 \
     if (_s != HIFR_OK) return _s;                      \
   } while (0)
+
+// The context's copy of a network's weights, one buffer in the layouts of
+// hifr::mlp::Weights: BF16 W1 [64][16], W2 and W3 [64][64], FP32 w4 [64], then the
+// FP32 parameter vector (the kernels read the biases from it).
+constexpr size_t kMlpW1 = 0, kMlpW2 = kMlpW1 + 2 * hifr::mlp::H * hifr::mlp::NPAD,
+                 kMlpW3 = kMlpW2 + 2 * hifr::mlp::H * hifr::mlp::H, kMlpW4 = kMlpW3 + 2 * hifr::mlp::H * hifr::mlp::H,
+                 kMlpP = kMlpW4 + 4 * hifr::mlp::H, kMlpBufBytes = kMlpP + 4 * hifr::mlp::NPARAM;
+
+hifr::mlp::Weights mlp_weights_in(void* buf) {
+  char* b = (char*)buf;
+  return hifr::mlp::Weights{(const unsigned short*)(b + kMlpW1), (const unsigned short*)(b + kMlpW2),
+                            (const unsigned short*)(b + kMlpW3), (const float*)(b + kMlpW4),
+                            (const float*)(b + kMlpP)};
+}
 
 bool order_supported(int ele_type, int P) {
   if (ele_type == HIFR_HEX) return P >= 1 && P <= 5;
@@ -132,18 +150,7 @@ struct StageIO {
 };
 
 template <class R>
-hipError_t launch_io(hifr_ctx* c, int mode, const StageIO& io) {
-  StageArgsT<R> a = base_args<R>(c);
-  a.uf_in = io.uf_in >= 0 ? (const R*)c->uf[io.uf_in] : nullptr;
-  a.uf_out = io.uf_out >= 0 ? (R*)c->uf[io.uf_out] : nullptr;
-  a.fv_in = io.fv_in ? (const R*)c->fv : nullptr;
-  a.fv_out = io.fv_out ? (R*)c->fv : nullptr;
-  a.gf_out = io.gf_out ? (R*)c->gf : nullptr;
-  a.div = io.div ? c->div : nullptr;
-  a.adv_type = io.adv_type;
-  a.rk_a = io.rk_a;
-  a.rk_b = io.rk_b;
-  a.dt = io.dt;
+hipError_t launch_args(hifr_ctx* c, int mode, const StageArgsT<R>& a) {
   if (c->cfg.ele_type == HIFR_TET) {
     if constexpr (std::is_same_v<R, float>)
       return hifr::launch_tet_mixed(c->cfg.order, mode, vis_kind(c), c->affine, les_kind(c), a, c->stream);
@@ -156,7 +163,33 @@ hipError_t launch_io(hifr_ctx* c, int mode, const StageIO& io) {
     return hifr::launch_fp64(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, les_kind(c), a, c->stream);
 }
 
-hipError_t launch(hifr_ctx* c, int mode, const StageIO& io) {
+// Fills the arguments of one stage launch, checks that every array the variant
+// uses is set (hifr::missing_input), and launches it.
+template <class R>
+hifr_status launch_io(hifr_ctx* c, int mode, const StageIO& io) {
+  StageArgsT<R> a = base_args<R>(c);
+  a.uf_in = io.uf_in >= 0 ? (const R*)c->uf[io.uf_in] : nullptr;
+  a.uf_out = io.uf_out >= 0 ? (R*)c->uf[io.uf_out] : nullptr;
+  a.fv_in = io.fv_in ? (const R*)c->fv : nullptr;
+  a.fv_out = io.fv_out ? (R*)c->fv : nullptr;
+  a.gf_out = io.gf_out ? (R*)c->gf : nullptr;
+  a.div = io.div ? c->div : nullptr;
+  a.adv_type = io.adv_type;
+  a.rk_a = io.rk_a;
+  a.rk_b = io.rk_b;
+  a.dt = io.dt;
+  const hifr::LaunchPtrs ptrs{a.u,     a.res,      a.div,      a.uf_in,    a.uf_out,    a.fv_in, a.fv_out,
+                              a.detj,  a.jg,       a.nbr,      a.fgeo,     a.tI,        a.tG,    a.tN,
+                              a.tD,    a.mlpw.w1,  a.mlpw.w2,  a.mlpw.w3,  a.mlpw.w4,   a.mlpw.params};
+  const hifr::LaunchKind kind{mode, c->cfg.viscous != 0, les_kind(c), c->cfg.ele_type == HIFR_TET};
+  if (const char* miss = hifr::missing_input(ptrs, kind)) {
+    std::fprintf(stderr, "hifr: stage kernel (mode %d) not launched: %s is not set\n", mode, miss);
+    return HIFR_ERR_STATE;
+  }
+  return dev_err(launch_args<R>(c, mode, a), "stage kernel launch");
+}
+
+hifr_status launch(hifr_ctx* c, int mode, const StageIO& io) {
   return c->mixed ? launch_io<float>(c, mode, io) : launch_io<double>(c, mode, io);
 }
 
@@ -166,7 +199,8 @@ hifr_status timed_launch(hifr_ctx* c, int mode, int kind, const StageIO& io) {
     HIP_TRY(hipEventRecord(c->ev[c->ev_used], c->stream));
     c->ev_kind[c->ev_used / 2] = kind;
   }
-  HIP_TRY(launch(c, mode, io));
+  const hifr_status ls = launch(c, mode, io);
+  if (ls != HIFR_OK) return ls;
   if (c->timing) {
     HIP_TRY(hipEventRecord(c->ev[c->ev_used + 1], c->stream));
     c->ev_used += 2;
@@ -298,6 +332,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   c->detj = c->fgeo = nullptr;  // sized by hifr_set_metrics / hifr_set_interfaces
   c->jg = nullptr;
   c->tI = c->tG = c->tN = c->tD = nullptr;
+  c->mlp_buf = nullptr;
   std::memset(&c->mlpw, 0, sizeof(c->mlpw));
   std::memset(&c->mlpn, 0, sizeof(c->mlpn));
   c->mlp_on = 0;
@@ -307,6 +342,14 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
       hifr_destroy(c);
       return HIFR_ERR_DEVICE;
     }
+  }
+  if (cfg->les && cfg->sgs_model == HIFR_SGS_LEARNED) {
+    // the weights the LES kernels read: zero until a network is attached
+    if (hipMalloc(&c->mlp_buf, kMlpBufBytes) != hipSuccess || hipMemset(c->mlp_buf, 0, kMlpBufBytes) != hipSuccess) {
+      hifr_destroy(c);
+      return HIFR_ERR_DEVICE;
+    }
+    c->mlpw = mlp_weights_in(c->mlp_buf);
   }
   if (hipMemset(c->res, 0, sizeof(double) * nu * N * nf) != hipSuccess) {
     hifr_destroy(c);
@@ -318,8 +361,8 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
 
 hifr_status hifr_destroy(hifr_ctx* c) {
   if (!c) return HIFR_OK;
-  void* ps[] = {c->u, c->res, c->div, c->uf[0], c->uf[1], c->fv, c->gf, c->detj, c->jg, c->fgeo, c->nbr, c->red,
-                c->tI, c->tG, c->tN, c->tD};
+  void* ps[] = {c->u,    c->res, c->div, c->uf[0], c->uf[1], c->fv, c->gf,     c->detj, c->jg,
+                c->fgeo, c->nbr, c->red, c->tI,    c->tG,    c->tN, c->tD,     c->mlp_buf};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -628,13 +671,8 @@ hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, con
     seen[L] = seen[R] = 1;
     double nl[3] = {0, 0, 0};
     for (size_t m = 0; m < nd; m++) nl[m] = c->h_norm[L + tot * m];
-    // LDG switch direction of the left normal (src/inters.cpp:650-660)
-    const double sw = nd == 3 ? (nl[0] + nl[1] + std::sqrt(2.) * nl[2]) : (nl[0] + nl[1]);
-    const bool neg_l = sw < 0.;
-    nbr[L] = (int)R | (neg_l ? (int)0x80000000u : 0);
-    nbr[R] = (int)L | (!neg_l ? (int)0x80000000u : 0);
     // one normal (the left one) serves both sides of the point, as in the
-    // reference; per-face storage takes the face's first visited point
+    // reference; per-face storage keeps the value of the face's last point
     const size_t gl = gidx(L), gr = gidx(R);
     for (size_t m = 0; m < nd; m++) {
       geo[m * gtot + gl] = nl[m];
@@ -649,6 +687,17 @@ hifr_status hifr_set_interfaces(hifr_ctx* c, int n_pts, const int32_t* left, con
                    i / nfp);
       return HIFR_ERR_UNSUPPORTED;
     }
+  // LDG switch direction of the left normal (src/inters.cpp:650-660), from the
+  // normal the kernels read (per point: the reference's own; per face: the face's)
+  for (int q = 0; q < n_pts; q++) {
+    const long L = left[q], R = right[q];
+    const size_t gl = gidx(L);
+    const double n0 = geo[gl], n1 = geo[gtot + gl], n2 = nd == 3 ? geo[2 * gtot + gl] : 0.0;
+    const double sw = nd == 3 ? (n0 + n1 + std::sqrt(2.) * n2) : (n0 + n1);
+    const bool neg_l = sw < 0.;
+    nbr[L] = (int)R | (neg_l ? (int)0x80000000u : 0);
+    nbr[R] = (int)L | (!neg_l ? (int)0x80000000u : 0);
+  }
   HIP_TRY(hipMemcpy(c->nbr, nbr.data(), sizeof(int) * tot, hipMemcpyHostToDevice));
   hifr_status st = realloc_dev(&c->fgeo, sizeof(double) * geo.size());
   if (st != HIFR_OK) return st;
@@ -685,6 +734,49 @@ hifr_status hifr_upload_state(hifr_ctx* c, const double* u) {
   return HIFR_OK;
 }
 
+// The metrics as the kernels read them, expanded to the per-point layouts of
+// hifr_set_metrics (HIFR_DETJAC_UPTS .. HIFR_NORM_FPTS).
+static hifr_status download_metrics(hifr_ctx* c, int what, double* host) {
+  if (!c->have_metrics) return HIFR_ERR_STATE;
+  if ((what == HIFR_TDA_FPTS || what == HIFR_NORM_FPTS) && !c->have_faces) return HIFR_ERR_STATE;
+  const size_t N = c->N, nu = c->nu, nfp = c->nfp, nd = c->nd, nfpf = c->nfpf;
+  const size_t nv = c->affine ? N : nu * N;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (what == HIFR_DETJAC_UPTS) {
+    std::vector<double> dj(nv);
+    HIP_TRY(hipMemcpy(dj.data(), c->detj, sizeof(double) * nv, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nu * N; i++) host[i] = dj[c->affine ? i / nu : i];
+    return HIFR_OK;
+  }
+  if (what == HIFR_JGINV_UPTS) {
+    std::vector<double> jg(nd * nd * nv);
+    if (c->mixed) {
+      std::vector<float> jf(jg.size());
+      HIP_TRY(hipMemcpy(jf.data(), c->jg, sizeof(float) * jf.size(), hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < jf.size(); i++) jg[i] = jf[i];
+    } else {
+      HIP_TRY(hipMemcpy(jg.data(), c->jg, sizeof(double) * jg.size(), hipMemcpyDeviceToHost));
+    }
+    // component-major jg[(l*nd+m)][v] -> JGinv(l,m,p,e)
+    for (size_t pe = 0; pe < nu * N; pe++) {
+      const size_t v = c->affine ? pe / nu : pe;
+      for (size_t l = 0; l < nd; l++)
+        for (size_t m = 0; m < nd; m++) host[l + nd * m + nd * nd * pe] = jg[(l * nd + m) * nv + v];
+    }
+    return HIFR_OK;
+  }
+  const size_t gtot = c->affine ? (size_t)c->nfaces * N : nfp * N;
+  std::vector<double> geo((nd + 1) * gtot);
+  HIP_TRY(hipMemcpy(geo.data(), c->fgeo, sizeof(double) * geo.size(), hipMemcpyDeviceToHost));
+  for (size_t fe = 0; fe < nfp * N; fe++) {
+    const size_t g = c->affine ? fe / nfpf : fe;
+    if (what == HIFR_TDA_FPTS) host[fe] = geo[nd * gtot + g];
+    else
+      for (size_t m = 0; m < nd; m++) host[fe + nfp * N * m] = geo[m * gtot + g];
+  }
+  return HIFR_OK;
+}
+
 hifr_status hifr_download(hifr_ctx* c, int what, double* host) {
   if (!c || !host) return HIFR_ERR_ARG;
   const size_t N = c->N, nu = c->nu, nfp = c->nfp, nf = c->nf, nd = c->nd;
@@ -698,6 +790,10 @@ hifr_status hifr_download(hifr_ctx* c, int what, double* host) {
     case HIFR_GRAD_DISU_FPTS:
       if (!c->cfg.viscous) return HIFR_ERR_ARG;
       src = c->gf; n = nfp * N * nf * nd; esz = c->tsz; break;
+    case HIFR_DETJAC_UPTS:
+    case HIFR_JGINV_UPTS:
+    case HIFR_TDA_FPTS:
+    case HIFR_NORM_FPTS: return download_metrics(c, what, host);
     default: return HIFR_ERR_ARG;
   }
   if (esz == sizeof(double) && what != HIFR_DISU_FPTS) {
@@ -960,13 +1056,25 @@ extern "C" hifr_status hifr_calibrate_copy(long n, int reps, double* ms) {
 extern "C" hifr_status hifr_set_sgs_network(hifr_ctx* c, const hifr_mlp* m) {
   if (!hifr::mlp::kLearnedEnabled) return HIFR_ERR_UNSUPPORTED;
   if (!c) return HIFR_ERR_ARG;
-  if (!(c->cfg.les && c->cfg.sgs_model == HIFR_SGS_LEARNED)) return HIFR_ERR_STATE;
+  if (!(c->cfg.les && c->cfg.sgs_model == HIFR_SGS_LEARNED) || !c->mlp_buf) return HIFR_ERR_STATE;
+  using namespace hifr::mlp;
+  char* b = (char*)c->mlp_buf;
   if (!m) {
     c->mlp_on = 0;
-    std::memset(&c->mlpw, 0, sizeof(c->mlpw));
+    HIP_TRY(hipMemsetAsync(c->mlp_buf, 0, kMlpBufBytes, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return HIFR_OK;
   }
-  c->mlpw = hifr::mlp::Weights{m->w1, m->w2, m->w3, m->w4, m->params};
+  // the network's pending work (training on its own stream) ends before the copy
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  const struct { size_t off; const void* src; size_t bytes; } parts[] = {
+      {kMlpW1, m->w1, 2 * H * NPAD}, {kMlpW2, m->w2, 2 * H * H}, {kMlpW3, m->w3, 2 * H * H},
+      {kMlpW4, m->w4, 4 * H},        {kMlpP, m->params, 4 * NPARAM}};
+  for (const auto& pt : parts) {
+    if (!pt.src) return HIFR_ERR_STATE;
+    HIP_TRY(hipMemcpyAsync(b + pt.off, pt.src, pt.bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
   c->mlpn = m->norm;
   c->mlp_on = 1;
   return HIFR_OK;

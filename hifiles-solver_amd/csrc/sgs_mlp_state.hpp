@@ -7,11 +7,10 @@
 namespace hifr {
 namespace mlp {
 
-// The learned closure's device entry points (network training and inference,
-// training samples, sgs_model HIFR_SGS_LEARNED) are switched off: their first
-// device run coincided with a device fault whose cause is not established.  While
-// false, they return HIFR_ERR_UNSUPPORTED before touching the device.
-constexpr bool kLearnedEnabled = false;
+// Switch of the learned closure's device entry points (network training and
+// inference, training samples, sgs_model HIFR_SGS_LEARNED).  While false, they
+// return HIFR_ERR_UNSUPPORTED before touching the device.
+constexpr bool kLearnedEnabled = true;
 
 // Input standardisation, training range and output scale of the closure.
 struct Norm {

@@ -56,6 +56,7 @@ def lib():
         L.hifr_run_timed.argtypes = [vp, C.c_int, C.c_int, C.c_double, dp, dp, ip]
         L.hifr_synchronize.argtypes = [vp]
         L.hifr_run.argtypes = [vp, C.c_int, C.c_int, C.c_double]
+        L.hifr_download.argtypes = [vp, C.c_int, dp]
         L.hifr_status_string.argtypes = [C.c_int]
         L.hifr_status_string.restype = C.c_char_p
         fp = C.POINTER(C.c_float)
@@ -218,6 +219,20 @@ class Solver:
             raise RuntimeError("hifr_metrics_per_element failed (%d)" % rc)
         return bool(flag.value)
 
+    def device_metrics(self):
+        """The metrics the kernels read (hifr_download HIFR_DETJAC_UPTS ..
+        HIFR_NORM_FPTS), expanded to the per-point layouts: dict of detjac_upts,
+        JGinv_upts, tdA_fpts, norm_fpts."""
+        out = {}
+        for what, name in ((5, "detjac_upts"), (6, "JGinv_upts"), (7, "tdA_fpts"), (8, "norm_fpts")):
+            shp = self.shape_of(name)
+            a = np.zeros(int(np.prod(shp)), dtype=np.float64)
+            rc = lib().hifr_download(C.c_void_p(self.ctx()), what, _dp(a))
+            if rc != 0:
+                raise RuntimeError("hifr_download(%s): %s" % (name, lib().hifr_status_string(rc).decode()))
+            out[name] = a.reshape(shp, order="F")
+        return out
+
     def dt(self):
         return lib().hifr_solver_dt(self._h)
 
@@ -243,7 +258,8 @@ class Solver:
         rc = lib().hifr_set_sgs_network(self.ctx(), mlp.handle if mlp is not None else None)
         if rc != 0:
             raise RuntimeError("hifr_set_sgs_network: " + lib().hifr_status_string(rc).decode())
-        self._mlp = mlp  # the context reads the network's weights in place
+        # the context copies the weights: the network may be trained further or
+        # closed afterwards without affecting it
 
     def sgs_samples(self, order_lo, filter_ratio=1.0, want_state=False):
         """Training samples of the learned closure from the current state, filtered

@@ -50,8 +50,17 @@ enum {
   HIFR_RK_REGISTER = 1,     /* disu_upts(1)     (Nu,N,nf)      */
   HIFR_DIV_TCONF_UPTS = 2,  /* div_tconf_upts(0)(Nu,N,nf)     after hifr_residual */
   HIFR_DISU_FPTS = 3,       /* disu_fpts        (Nfp,N,nf)     current traces */
-  HIFR_GRAD_DISU_FPTS = 4   /* grad_disu_fpts   (Nfp,N,nf,d)   after hifr_residual (viscous; not
+  HIFR_GRAD_DISU_FPTS = 4,  /* grad_disu_fpts   (Nfp,N,nf,d)   after hifr_residual (viscous; not
                                                                  refreshed by hifr_advance/hifr_run) */
+  /* The metrics as the kernels read them (no counterpart in the reference), in
+   * the layouts of hifr_set_metrics.  With per-element storage every point of an
+   * element gets its element's value and every flux point its face's value; the
+   * normal of a face is the outward normal its element uses in the interface
+   * flux (on an interface, the left element's normal and its negation). */
+  HIFR_DETJAC_UPTS = 5,     /* detjac_upts (Nu,N)     */
+  HIFR_JGINV_UPTS = 6,      /* JGinv_upts  (d,d,Nu,N) */
+  HIFR_TDA_FPTS = 7,        /* tdA_fpts    (Nfp,N)    */
+  HIFR_NORM_FPTS = 8        /* norm_fpts   (Nfp,N,d)  */
 };
 
 /* Physical and numerical parameters after non-dimensionalisation

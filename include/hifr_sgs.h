@@ -83,10 +83,12 @@ hifr_status hifr_mlp_predict(hifr_mlp* m, long n, const float* x, float* y);
 hifr_status hifr_sgs_samples(hifr_ctx* c, int order_lo, const double* filter_1d, const double* deriv_1d,
                              double filter_ratio, float* x, float* c_exact, float* c_wale, double* u_lo);
 
-/* Attaches a network to a context created with sgs_model HIFR_SGS_LEARNED: its
- * weights (read in place by the residual kernels, so the network must outlive
- * the context's use of it) and a copy of its normalisation (attach again after
- * changing it).  m = NULL detaches: the WALE fallback everywhere. */
+/* Attaches a network to a context created with sgs_model HIFR_SGS_LEARNED: the
+ * context copies its current weights and normalisation into buffers of its own
+ * (after the network's stream has finished its work), so the network may be
+ * trained further or destroyed afterwards; attach again to take new weights.
+ * m = NULL detaches (the context's copy is zeroed): the WALE fallback everywhere,
+ * as before any network is attached. */
 hifr_status hifr_set_sgs_network(hifr_ctx* c, const hifr_mlp* m);
 
 #ifdef __cplusplus

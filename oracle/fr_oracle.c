@@ -1521,6 +1521,18 @@ static void calc_sgsf_nu(const or_solver* s, const double* u, const double* g, d
   }
 }
 
+/* Replaces the metrics set_transforms computed (src/eles.cpp:4557-4847) with the
+ * caller's, in the same layouts: detjac_upts (Nu,N), JGinv_upts (d,d,Nu,N),
+ * tdA_fpts (Nfp,N), norm_fpts (Nfp,N,d).  The residual reads them as they are. */
+void or_set_metrics(or_solver* s, const double* detjac, const double* JGinv, const double* tdA,
+                    const double* norm) {
+  const size_t nuN = (size_t)s->nu * s->n_eles, nfpN = (size_t)s->nfp * s->n_eles, nd = s->nd;
+  memcpy(s->detjac_upts, detjac, sizeof(double) * nuN);
+  memcpy(s->JGinv_upts, JGinv, sizeof(double) * nuN * nd * nd);
+  memcpy(s->tdA_fpts, tdA, sizeof(double) * nfpN);
+  memcpy(s->norm_fpts, norm, sizeof(double) * nfpN * nd);
+}
+
 void or_set_sgs_nu(or_solver* s, const double* nu_t) {
   if (!s->sgs_nu) return;
   memcpy(s->sgs_nu, nu_t, sizeof(double) * (size_t)s->nu * s->n_eles);

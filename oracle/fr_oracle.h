@@ -112,6 +112,9 @@ void or_run_steps(or_solver* s, int n_steps);
 void or_norm_residual(const or_solver* s, int norm_type, double* out5);
 /* SGS_model 10: kinematic eddy viscosity at every solution point (Nu,N). */
 void or_set_sgs_nu(or_solver* s, const double* nu_t);
+/* Replaces the computed metrics (layouts of or_get 6..9); the residual reads them as given. */
+void or_set_metrics(or_solver* s, const double* detjac_upts, const double* JGinv_upts, const double* tdA_fpts,
+                    const double* norm_fpts);
 /* Number of OpenMP threads used by the dense loops (1 = serial). */
 void or_set_threads(int n);
 

@@ -63,6 +63,7 @@ def lib():
         L.or_get.restype = C.c_long
         L.or_set_state.argtypes = [vp, dp]
         L.or_set_sgs_nu.argtypes = [vp, dp]
+        L.or_set_metrics.argtypes = [vp, dp, dp, dp, dp]
         L.or_zero_rk_register.argtypes = [vp]
         L.or_set_ics_tgv.argtypes = [vp, C.c_double]
         L.or_get_face_pairs.argtypes = [vp, ip, ip]
@@ -185,6 +186,17 @@ class Oracle:
 
     def advance(self, stage, adv_type):
         lib().or_advance(self._h, int(stage), int(adv_type))
+
+    def set_metrics(self, detjac_upts, JGinv_upts, tdA_fpts, norm_fpts):
+        """Replaces the oracle's own metrics (reference layouts, Fortran order)."""
+        arrs = []
+        for name, a in (("detjac_upts", detjac_upts), ("JGinv_upts", JGinv_upts), ("tdA_fpts", tdA_fpts),
+                        ("norm_fpts", norm_fpts)):
+            a = np.asarray(a, dtype=np.float64)
+            if a.shape != self.shape_of(name):
+                raise ValueError("%s: shape %s, expected %s" % (name, a.shape, self.shape_of(name)))
+            arrs.append(np.ascontiguousarray(a.reshape(-1, order="F")))
+        lib().or_set_metrics(self._h, *[_dp(a) for a in arrs])
 
     def set_sgs_nu(self, nu_t):
         """SGS_model 10: kinematic eddy viscosity at the solution points, (Nu, N)."""

@@ -57,7 +57,7 @@ def deck_text(c, restart_iter=0):
         "upts_type_hexa 0", "vcjh_scheme_hexa %d" % c["vcjh"], "eta_hexa 0.0", "sparse_hexa 0", "sparse_pri 0",
         "upts_type_tet 0", "fpts_type_tet 0", "vcjh_scheme_tet %d" % c["vcjh"], "c_tet 0.0", "sparse_tet 0",
         "gamma 1.4", "prandtl 0.72", "S_gas 120.", "T_gas 291.15", "R_gas 286.9", "mu_gas 1.827E-05",
-        "fix_vis 1", "Mach_free_stream %.17g" % c.get("mach", 0.1), "nx_free_stream 1.",
+        "fix_vis %d" % c.get("fix_vis", 1), "Mach_free_stream %.17g" % c.get("mach", 0.1), "nx_free_stream 1.",
         "ny_free_stream 0.", "nz_free_stream 0.", "Re_free_stream %.17g" % c.get("Re", 1600.0),
         "L_free_stream 1.", "T_free_stream 300", "T_wall %.17g" % c.get("T_wall", 300.0),
         "Mach_wall 0.", "nx_wall 0.", "ny_wall 0.", "nz_wall 0.",
@@ -77,7 +77,7 @@ def oracle_deck(c):
         Re_free_stream=c.get("Re", 1600.0), L_free_stream=1.0, T_free_stream=300.0, T_wall=c.get("T_wall", 300.0),
         vcjh_scheme=c["vcjh"], eta=0.0, dx_cyclic=c["cyc"][0], dy_cyclic=c["cyc"][1],
         dz_cyclic=c["cyc"][2], LES=int(c.get("les", 0)), SGS_model=c.get("sgs_model", 0),
-        filter_ratio=c.get("filter_ratio", 1.0))
+        filter_ratio=c.get("filter_ratio", 1.0), fix_vis=float(c.get("fix_vis", 1)))
 
 
 def write_restart(path, ele_type, loc1d, order, u0, loc_upts=None):
@@ -195,6 +195,17 @@ CASES = [
     dict(name="hex_p3_les_wale_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=1, viscous=1, tau=1.0,
          pen_fact=0.5, state="tgv", noise=1e-2, seed=24, perturb=0.0, kind="rk45", steps=3, dt=1e-3, les=1,
          sgs_model=1, filter_ratio=2.0),
+    # Sutherland's law (fix_vis 0, src/flux.cpp:326-329): the viscosity follows the
+    # local temperature instead of the free-stream value
+    dict(name="hex_p3_ns_sutherland_perturbed_2x2x2", ele_type=4, n=(2, 2, 2), order=3, vcjh=1, viscous=1,
+         tau=1.0, pen_fact=0.5, state="tgv", noise=5e-2, seed=51, perturb=0.1, kind="residual", fix_vis=0),
+    dict(name="quad_p3_ns_sutherland_perturbed_4x3", ele_type=1, n=(4, 3), order=3, vcjh=2, viscous=1, tau=1.0,
+         pen_fact=0.5, state="vortex", noise=5e-2, seed=52, perturb=0.1, kind="residual", fix_vis=0),
+    dict(name="tet_p2_ns_sutherland_2x2x1", ele_type=2, n=(2, 2, 1), order=2, vcjh=1, viscous=1, tau=0.5,
+         pen_fact=0.5, state="tgv", noise=5e-2, seed=53, perturb=0.1, kind="residual", fix_vis=0),
+    dict(name="hex_p2_channel_walls_sutherland_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=2, vcjh=1, viscous=1,
+         tau=1.0, pen_fact=0.5, state="channel", noise=5e-2, seed=54, perturb=0.0, kind="rk45", steps=3, dt=1e-3,
+         channel=True, mach=0.2, Re=2800.0, T_wall=320.0, fix_vis=0),
 ]
 
 
@@ -280,7 +291,7 @@ def main():
                    u0=u0, kind=c["kind"], les=int(c.get("les", 0)), sgs_model=int(c.get("sgs_model", 0)),
                    filter_ratio=float(c.get("filter_ratio", 1.0)),
                    bfaces=np.array(bfaces, dtype=np.int32).reshape(-1, 3), mach=c.get("mach", 0.1),
-                   Re=c.get("Re", 1600.0), T_wall=c.get("T_wall", 300.0))
+                   Re=c.get("Re", 1600.0), T_wall=c.get("T_wall", 300.0), fix_vis=int(c.get("fix_vis", 1)))
         if c["kind"] == "residual":
             R = (u0 - u1) / c["dt"]
             rec["R"] = R

@@ -17,7 +17,7 @@ import hifr
 from oracle import sgs_closure as sc
 from oracle import sgs_mlp as om
 
-pytestmark = [pytest.mark.gpu, pytest.mark.skip(reason="the learned-closure device entry points are switched off (hifr::mlp::kLearnedEnabled): their first device run coincided with a device fault whose cause is not established")]
+pytestmark = pytest.mark.gpu
 
 TWO_PI = cases.TWO_PI
 
@@ -117,6 +117,33 @@ def test_network_residual_matches_oracle(require_gpu, order):
     print("P", order, "network: field errors", err, "(network vs WALE residual differ by", dw, ")", flush=True)
     assert np.all(dw[1:] > 1e-6)  # the network's eddy viscosity is not WALE's (no SGS density flux)
     assert np.all(err < 1e-4), err
+
+
+def test_network_weights_are_copied_on_attach(require_gpu):
+    """The context keeps its own copy of the attached weights: closing the network
+    (freeing its device buffers) or changing its parameters afterwards leaves the
+    residual unchanged; detaching gives the WALE fallback again."""
+    g = _case(2)
+    s = cases.solver_of(g, hifr_learned_sgs=1)
+    u0 = _state(s.get("pos_upts"))
+    s.set_state(u0)
+    R_wale = s.residual().copy()
+    m, p, norm = _network()
+    s.set_sgs_network(m)
+    R_net = s.residual().copy()
+    q = p.copy()
+    q[om.N_PARAMS - 1] = 1.0
+    m.set_params(q)
+    R_changed = s.residual().copy()
+    m.close()
+    R_closed = s.residual().copy()
+    s.set_sgs_network(None)
+    R_detached = s.residual().copy()
+    d = cases.field_rel_err(R_net, R_wale)
+    print("network vs WALE", d, flush=True)
+    assert np.all(d[1:] > 1e-6)
+    assert np.array_equal(R_changed, R_net) and np.array_equal(R_closed, R_net)
+    assert np.array_equal(R_detached, R_wale)
 
 
 def test_mixed_learned_residual_close_to_oracle(require_gpu):

@@ -88,17 +88,24 @@ def test_tgv_32cubed_free_stream_and_conservation(require_gpu):
 
 
 # Residual tolerance per conserved variable, relative to max|R| of that variable.
-# Per-point metrics are the reference's own arrays: 1e-12.  Per-element metrics
-# (the box is affine) replace the reference's per-point JGinv, whose values differ
-# between the points of one element by rounding (~1e-16 relative); at Mach 0.1 the
-# momentum flux carries the mean pressure p0 = 71.4, so that rounding enters the
-# reference's momentum divergence as ~p0 * |D| * 1e-16 ~ 5e-14 absolute, i.e.
-# ~1e-11 of the momentum residual (max ~5e-3).  Density and energy stay at 1e-12.
-RESID_TOL = {1: [1e-12] * 5, 0: [1e-12, 3e-11, 3e-11, 3e-11, 1e-12]}
+#  * per_point: the context keeps the reference's per-point metrics; the oracle
+#    uses the same arrays: 1e-12.
+#  * per_element: the context keeps one JGinv / detJ per element and one normal /
+#    tdA per face (the box is affine); the oracle is given exactly those metrics
+#    (downloaded from the context, expanded to the per-point layouts): 1e-12.
+#  * per_element_vs_reference_metrics: the same context against the oracle's own
+#    per-point metrics, which are different inputs: the reference's per-point
+#    JGinv values differ between the points of one element by rounding (~1e-16
+#    relative); at Mach 0.1 the momentum flux carries the mean pressure
+#    p0 = 71.4, so that rounding enters the momentum divergence as
+#    ~p0 * |D| * 1e-16 ~ 5e-14 absolute, ~1e-11 of the momentum residual.
+RESID_TOL = {"per_point": [1e-12] * 5, "per_element": [1e-12] * 5,
+             "per_element_vs_reference_metrics": [1e-12, 3e-11, 3e-11, 3e-11, 1e-12]}
 
 
-@pytest.mark.parametrize("metrics", [1, 0], ids=["per_point", "per_element"])
-def test_tgv_32cubed_against_oracle(require_gpu, metrics):
+@pytest.mark.parametrize("case", ["per_point", "per_element", "per_element_vs_reference_metrics"])
+def test_tgv_32cubed_against_oracle(require_gpu, case):
+    metrics = 1 if case == "per_point" else 0
     s, _ = _tgv_solver(metrics=metrics)
     assert s.metrics_per_element() == (metrics == 0)
     xv = s.get("xv").reshape(-1, 3)
@@ -108,13 +115,20 @@ def test_tgv_32cubed_against_oracle(require_gpu, metrics):
                          Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, vcjh_scheme=1,
                          dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI)
     o = po.Oracle(od, 4, xv, c2v, threads=min(16, os.cpu_count() or 1))
+    if case == "per_element":
+        dm = s.device_metrics()
+        # the context's per-element values are the reference's at each element's
+        # first point (to rounding)
+        assert cases.field_rel_err(dm["JGinv_upts"].reshape(9, -1).T[:, None, :],
+                                   o.get("JGinv_upts").reshape(9, -1).T[:, None, :]).max() < 1e-13
+        o.set_metrics(dm["detjac_upts"], dm["JGinv_upts"], dm["tdA_fpts"], dm["norm_fpts"])
     u = _tgv_state(o.get("pos_upts"))
     s.set_state(u)
     o.set_state(u)
     o.residual()
     err = cases.field_rel_err(s.residual(), o.get("div_tconf_upts"))
-    print("metrics", metrics, "residual field errors", err)
-    assert np.all(err < RESID_TOL[metrics]), err
+    print(case, "residual field errors", err)
+    assert np.all(err < RESID_TOL[case]), err
     s.set_state(u)
     o.set_state(u)
     o.zero_rk_register()
@@ -123,5 +137,5 @@ def test_tgv_32cubed_against_oracle(require_gpu, metrics):
     # the z-momentum starts at zero (w = 0), so after one step its scale is
     # dt * |R|: the state carries the residual's tolerance
     err = cases.field_rel_err(s.get("disu_upts"), o.get("disu_upts"))
-    print("metrics", metrics, "state field errors", err)
-    assert np.all(err < RESID_TOL[metrics]), err
+    print(case, "state field errors after one RK45 step", err)
+    assert np.all(err < RESID_TOL[case]), err

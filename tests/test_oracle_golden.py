@@ -66,6 +66,36 @@ def test_oracle_rk45_matches_reference(name):
     assert np.array_equal(printed, g["u_final"])
 
 
+def _sutherland_goldens():
+    return [n for n in cases.golden_names() if int(cases.load_golden(n).get("fix_vis", 1)) == 0]
+
+
+def test_sutherland_goldens_exist():
+    assert len(_sutherland_goldens()) >= 4
+
+
+@pytest.mark.parametrize("name", _sutherland_goldens())
+def test_sutherland_goldens_pin_the_law(name):
+    """The fix_vis 0 fixtures depend on Sutherland's law: the oracle with the law
+    switched off (fix_vis 1, constant viscosity mu_inf) misses them by far more
+    than their 15-digit precision, so a wrong exponent or constant in the
+    oracle's law (or in the non-dimensionalisation that sets c_sth, rt_inf and
+    mu_inf) fails the reference comparisons above."""
+    g = cases.load_golden(name)
+    o = _oracle(g, fix_vis=1.0)
+    o.set_state(g["u0"])
+    if str(g["kind"]) == "residual":
+        o.residual()
+        R = o.get("div_tconf_upts") / o.get("detjac_upts")[:, :, None]
+        rel = cases.field_rel_err(R, g["R"])
+        assert rel[1:].max() > 1e-6, rel
+    else:
+        o.zero_rk_register()
+        o.run_steps(int(g["steps"]))
+        rel = cases.field_rel_err(o.get("disu_upts"), g["u_final"])
+        assert rel[1:].max() > 1e-9, rel
+
+
 @pytest.mark.slow
 def test_oracle_tgv_regression_norms():
     with open(os.path.join(cases.GOLDEN, "tgv_regression.json")) as fh:

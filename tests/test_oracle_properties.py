@@ -438,3 +438,24 @@ def test_tet_free_stream_and_conservation(order):
     total = np.einsum("j,jek->k", W, R)
     scale = np.einsum("j,jek->k", W, np.abs(R))
     assert np.all(np.abs(total) < 1e-12 * scale), (total, scale)
+
+
+def test_set_metrics_replaces_the_computed_metrics():
+    """or_set_metrics: given its own metrics back, the oracle's residual is
+    unchanged bit for bit; given per-element metrics of an affine mesh (each
+    element's first-point values everywhere), it changes only by rounding."""
+    import cases as _cases
+    g = _cases.load_golden("hex_p3_ns_tgv_2x2x2")
+    o = _cases.oracle_of(g)
+    o.set_state(g["u0"])
+    R0 = o.residual().copy()
+    m = {k: o.get(k) for k in ("detjac_upts", "JGinv_upts", "tdA_fpts", "norm_fpts")}
+    o.set_metrics(m["detjac_upts"], m["JGinv_upts"], m["tdA_fpts"], m["norm_fpts"])
+    assert np.array_equal(o.residual(), R0)
+    nu = m["detjac_upts"].shape[0]
+    dj = np.repeat(m["detjac_upts"][:1], nu, axis=0)
+    jg = np.repeat(m["JGinv_upts"][:, :, :1], nu, axis=2)
+    o.set_metrics(dj, jg, m["tdA_fpts"], m["norm_fpts"])
+    R1 = o.residual()
+    err = _cases.field_rel_err(R1, R0)
+    assert err.max() < 1e-11 and not np.array_equal(R1, R0), err

@@ -138,3 +138,75 @@ def test_samples_match_brute_force_projection():
     assert np.allclose(x[:, 9], np.log(delta ** 2 * G * rb.ravel() / mu), atol=1e-11)
     assert np.allclose(c_ex, nu / (delta ** 2 * G), rtol=1e-9, atol=1e-12)
     assert np.all(c_w >= 0)
+
+
+# ---- the C oracle's supplied eddy viscosity (SGS_model 10, calc_sgsf_nu) ----
+
+def _nu_oracle(L, les=1, order=3, n=4):
+    import frmesh
+    from oracle import pyoracle as po
+    xv, conn = frmesh.box_hex(n, n, n, L, L, L)
+    c2v = frmesh.gambit_to_internal(4, conn)
+    deck = po.default_deck(equation=0, viscous=1, order=order, adv_type=0, dt=1e-3, tau=1.0, pen_fact=0.5,
+                           vcjh_scheme=1, eta=0.0, gamma=1.4, prandtl=0.72, Mach_free_stream=0.1,
+                           nx_free_stream=1.0, Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0,
+                           dx_cyclic=L, dy_cyclic=L, dz_cyclic=L, LES=les, SGS_model=10 if les else 0,
+                           filter_ratio=1.0, fix_vis=1.0)
+    return po.Oracle(deck, 4, xv, c2v)
+
+
+def test_supplied_nu_zero_is_laminar():
+    """nu_t = 0 adds nothing: the residual equals the laminar one bit for bit."""
+    import frmesh
+    L = 2 * np.pi
+    lam, les = _nu_oracle(L, les=0), _nu_oracle(L, les=1)
+    pos = lam.get("pos_upts")
+    u0 = frmesh.tgv_state(pos, p0=1.0 / (1.4 * 0.1 ** 2))
+    u0 = u0 * (1.0 + frmesh.seeded_perturbation(u0.shape, 1e-2, 3))
+    lam.set_state(u0)
+    les.set_state(u0)
+    les.set_sgs_nu(np.zeros(pos.shape[:2]))
+    assert np.array_equal(lam.residual(), les.residual())
+    assert np.all(les.get("sgsf_upts") == 0.0)
+
+
+def test_supplied_nu_closed_form_on_linear_fields():
+    """rho, the velocity and the pressure linear in x: at P = 3 the corrected
+    gradient of rho, rho v and E (degree <= 3) is exact on elements away from the
+    periodic boundary, so the SGS flux there has its closed form
+        F(1+i, j) = -2 rho nu_t Sd_ij,   Sd = sym(A) - tr(A)/3 I,
+        F(4, j)   = -gamma rho nu_t / 0.5 * de/dx_j,  e = p / ((gamma-1) rho),
+    with a different nu_t at every point; sgsf_upts holds JGinv . F."""
+    L, gam = 4.0, 1.4
+    o = _nu_oracle(L, les=1)
+    pos = o.get("pos_upts")                       # (Nu, N, 3)
+    X = pos - L / 2
+    r = np.array([0.05, -0.03, 0.02])             # d rho / dx
+    A = np.array([[0.3, -0.2, 0.1], [0.4, -0.1, 0.25], [-0.15, 0.2, 0.05]])   # dv_i / dx_j
+    b = np.array([1.0, -0.5, 0.25])
+    c = np.array([0.7, -0.4, 0.9])                # dp / dx
+    rho = 1.0 + X @ r
+    v = b + np.einsum("ij,...j->...i", A, X)
+    p = 20.0 + X @ c
+    u0 = np.concatenate([rho[..., None], rho[..., None] * v,
+                         (p / (gam - 1) + 0.5 * rho * (v ** 2).sum(-1))[..., None]], axis=-1)
+    nu = 1e-3 * (1.0 + np.random.default_rng(9).random(pos.shape[:2]))
+    o.set_state(u0)
+    o.set_sgs_nu(nu)
+    o.residual()
+    sg = o.get("sgsf_upts")                       # (Nu, N, 5, 3), transformed
+    JG = o.get("JGinv_upts")                      # (3, 3, Nu, N)
+    Sd = 0.5 * (A + A.T) - np.trace(A) / 3 * np.eye(3)
+    de = (c[None, None, :] * rho[..., None] - p[..., None] * r[None, None, :]) / (rho[..., None] ** 2 * (gam - 1))
+    F = np.zeros(pos.shape[:2] + (5, 3))
+    F[..., 1:4, :] = -2.0 * (rho * nu)[..., None, None] * Sd
+    F[..., 4, :] = -gam * (rho * nu)[..., None] / 0.5 * de
+    Ft = np.einsum("lmpe,pekm->pekl", JG, F)
+    # elements with no face on the periodic boundary (element centres strictly inside)
+    cen = pos.mean(axis=0)
+    inner = np.all((cen > L / 4) & (cen < 3 * L / 4), axis=1)
+    assert inner.sum() == 8
+    err = np.abs(sg[:, inner] - Ft[:, inner]).max() / np.abs(Ft[:, inner]).max()
+    assert err < 1e-11, err
+    # a transposed strain or a dropped density-gradient term would miss by O(1)
+    assert np.abs(Ft[:, inner]).max() > 1e-4
