@@ -23,4 +23,10 @@ hipError_t launch_tet_fp64(int P, int mode, int vis, bool aff, int les, const St
 hipError_t launch_tet_mixed(int P, int mode, int vis, bool aff, int les, const StageArgsT<float>& a,
                             hipStream_t s);
 
+// Roe interface flux (riemann_solve_type 2): FP64, no LES, flux modes (RESID,
+// ADVANCE) only; quads/hexes and tets
+hipError_t launch_roe_fp64(int nd, int P, int mode, int vis, bool aff, const StageArgsT<double>& a,
+                           hipStream_t s);
+hipError_t launch_tet_roe_fp64(int P, int mode, int vis, bool aff, const StageArgsT<double>& a, hipStream_t s);
+
 }  // namespace hifr

@@ -14,12 +14,13 @@ struct Cfg {
 template <bool MIX>
 using Args = StageArgsT<std::conditional_t<MIX, float, double>>;
 
-template <int P, int ND, int VIS, int MODE, bool AFF, bool MIX, int LES>
+template <int P, int ND, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE = 0>
 hipError_t launch_les(const Args<MIX>& a, hipStream_t s) {
   constexpr int EPB = Cfg<P, ND>::EPB;
   using T = TP<P, ND, EPB>;
   const int grid = (a.N + EPB - 1) / EPB;
-  hipLaunchKernelGGL((fr_tensor_kernel<P, ND, EPB, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
+  hipLaunchKernelGGL((fr_tensor_kernel<P, ND, EPB, VIS, MODE, AFF, MIX, LES, RIE>), dim3(grid), dim3(T::BLOCK), 0,
+                     s, a);
   return hipGetLastError();
 }
 
@@ -85,14 +86,54 @@ hipError_t launch(int nd, int P, int mode, int vis, bool aff, int les, const Arg
   }
 }
 
+// ---- Roe interface flux (FP64, no LES): the flux kernels only; the trace kernels
+// do not evaluate the inviscid flux ----
+template <int P, int ND, int VIS>
+hipError_t roe_vis(int mode, bool aff, const Args<false>& a, hipStream_t s) {
+  if (mode == MODE_RESID)
+    return aff ? launch_les<P, ND, VIS, MODE_RESID, true, false, 0, 2>(a, s)
+               : launch_les<P, ND, VIS, MODE_RESID, false, false, 0, 2>(a, s);
+  return aff ? launch_les<P, ND, VIS, MODE_ADVANCE, true, false, 0, 2>(a, s)
+             : launch_les<P, ND, VIS, MODE_ADVANCE, false, false, 0, 2>(a, s);
+}
+
+template <int P, int ND>
+hipError_t roe_order(int mode, int vis, bool aff, const Args<false>& a, hipStream_t s) {
+  return vis == 2 ? roe_vis<P, ND, 2>(mode, aff, a, s)
+       : vis == 1 ? roe_vis<P, ND, 1>(mode, aff, a, s)
+                  : roe_vis<P, ND, 0>(mode, aff, a, s);
+}
+
+// hexes P = 1..5, quads P = 1..6; mode RESID or ADVANCE
+inline hipError_t launch_roe(int nd, int P, int mode, int vis, bool aff, const Args<false>& a, hipStream_t s) {
+  if (nd == 3) {
+    switch (P) {
+      case 1: return roe_order<1, 3>(mode, vis, aff, a, s);
+      case 2: return roe_order<2, 3>(mode, vis, aff, a, s);
+      case 3: return roe_order<3, 3>(mode, vis, aff, a, s);
+      case 4: return roe_order<4, 3>(mode, vis, aff, a, s);
+      default: return roe_order<5, 3>(mode, vis, aff, a, s);
+    }
+  }
+  switch (P) {
+    case 1: return roe_order<1, 2>(mode, vis, aff, a, s);
+    case 2: return roe_order<2, 2>(mode, vis, aff, a, s);
+    case 3: return roe_order<3, 2>(mode, vis, aff, a, s);
+    case 4: return roe_order<4, 2>(mode, vis, aff, a, s);
+    case 5: return roe_order<5, 2>(mode, vis, aff, a, s);
+    default: return roe_order<6, 2>(mode, vis, aff, a, s);
+  }
+}
+
 // ---- tetrahedra ----
-template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES>
+template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE = 0>
 hipError_t tet_les(const Args<MIX>& a, hipStream_t s) {
   using T = TetP<P>;
   const int grid = (a.N + T::EPB - 1) / T::EPB;
   if constexpr (MODE == MODE_RESID || MODE == MODE_ADVANCE) {
     // flux pass: operator products on MFMA (EPB elements per block)
-    hipLaunchKernelGGL((fr_tet_mfma_kernel<P, VIS, MODE, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
+    hipLaunchKernelGGL((fr_tet_mfma_kernel<P, VIS, MODE, AFF, MIX, LES, RIE>), dim3(grid), dim3(T::BLOCK), 0, s,
+                       a);
   } else if constexpr (MODE == MODE_TRACE_GRAD) {
     hipLaunchKernelGGL((fr_tet_mfma_grad_kernel<P, VIS, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);
   } else {
@@ -141,6 +182,31 @@ hipError_t launch_tet(int P, int mode, int vis, bool aff, int les, const Args<MI
     case 1: return tet_mode<1, MIX>(mode, vis, a, s, aff, les);
     case 2: return tet_mode<2, MIX>(mode, vis, a, s, aff, les);
     default: return tet_mode<3, MIX>(mode, vis, a, s, aff, les);
+  }
+}
+
+template <int P, int VIS>
+hipError_t tet_roe_vis(int mode, bool aff, const Args<false>& a, hipStream_t s) {
+  if (mode == MODE_RESID)
+    return aff ? tet_les<P, VIS, MODE_RESID, true, false, 0, 2>(a, s)
+               : tet_les<P, VIS, MODE_RESID, false, false, 0, 2>(a, s);
+  return aff ? tet_les<P, VIS, MODE_ADVANCE, true, false, 0, 2>(a, s)
+             : tet_les<P, VIS, MODE_ADVANCE, false, false, 0, 2>(a, s);
+}
+
+template <int P>
+hipError_t tet_roe_order(int mode, int vis, bool aff, const Args<false>& a, hipStream_t s) {
+  return vis == 2 ? tet_roe_vis<P, 2>(mode, aff, a, s)
+       : vis == 1 ? tet_roe_vis<P, 1>(mode, aff, a, s)
+                  : tet_roe_vis<P, 0>(mode, aff, a, s);
+}
+
+// tets P = 1..3; mode RESID or ADVANCE
+inline hipError_t launch_tet_roe(int P, int mode, int vis, bool aff, const Args<false>& a, hipStream_t s) {
+  switch (P) {
+    case 1: return tet_roe_order<1>(mode, vis, aff, a, s);
+    case 2: return tet_roe_order<2>(mode, vis, aff, a, s);
+    default: return tet_roe_order<3>(mode, vis, aff, a, s);
   }
 }
 

@@ -472,6 +472,97 @@ __device__ __forceinline__ void rusanov(const T* uo, const T* un, const T* n, T 
   for (int k = 0; k < NF; k++) fn[k] = T(0.5) * ((fo[k] + fb[k]) - eig * (un[k] - uo[k]));
 }
 
+// Roe common normal flux with Harten's entropy fix for the left state ul, right
+// state ur and the left element's outward normal n (src/inters.cpp:390-520, static
+// mesh).  The reference evaluates the central part and the pressures in 2-D only;
+// in 3-D the same expressions take the third velocity component.  Unlike the
+// Rusanov flux, its Roe averages are not symmetric in rounding under a swap of the
+// sides, so both elements of an interface evaluate it in the left element's
+// orientation (the right one negates the result).
+template <int ND, class T>
+__device__ __forceinline__ void roe(const T* ul, const T* ur, const T* n, T gamma, T* fn) {
+  constexpr int NF = ND + 2;
+  const T g1 = gamma - T(1);
+  const T irl = T(1) / ul[0], irr = T(1) / ur[0];
+  T vl[ND], vr[ND], kl = 0, kr = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    vl[i] = ul[1 + i] * irl;
+    vr[i] = ur[1 + i] * irr;
+    kl += vl[i] * vl[i];
+    kr += vr[i] * vr[i];
+  }
+  const T pl = g1 * (ul[ND + 1] - T(0.5) * ul[0] * kl);
+  const T pr = g1 * (ur[ND + 1] - T(0.5) * ur[0] * kr);
+  const T hl = (ul[ND + 1] + pl) * irl, hr = (ur[ND + 1] + pr) * irr;
+  // Roe averages
+  const T sq = sqrt(ur[0] * irl);
+  const T w = T(1) / (sq + T(1));
+  T um[ND], usq = 0, unm = 0, mnl = 0, mnr = 0;
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    um[i] = w * (vl[i] + sq * vr[i]);
+    usq += T(0.5) * um[i] * um[i];
+    unm += um[i] * n[i];
+    mnl += ul[1 + i] * n[i];
+    mnr += ur[1 + i] * n[i];
+  }
+  const T hm = w * (hl + sq * hr);
+  const T am2 = g1 * (hm - usq);
+  const T am = sqrt(am2);
+  // central part (twice the average of the two normal fluxes)
+  fn[0] = mnl + mnr;
+#pragma unroll
+  for (int i = 0; i < ND; i++) fn[1 + i] = mnl * vl[i] + mnr * vr[i] + (pl + pr) * n[i];
+  fn[ND + 1] = mnl * hl + mnr * hr;
+  // wave speeds with the entropy fix
+  T l0 = fabs(unm), lp = fabs(unm + am), lm = fabs(unm - am);
+  const T eps = T(0.5) * (fabs(mnl * irl - mnr * irr) + fabs(sqrt(gamma * pl * irl) - sqrt(gamma * pr * irr)));
+  const T e2 = T(2) * eps;
+  if (l0 < e2) l0 = T(0.25) * l0 * l0 / eps + eps;
+  if (lp < e2) lp = T(0.25) * lp * lp / eps + eps;
+  if (lm < e2) lm = T(0.25) * lm * lm / eps + eps;
+  const T a2 = T(0.5) * (lp + lm) - l0;
+  const T a3 = T(0.5) * (lp - lm) / am;
+  const T a1 = a2 * g1 / am2;
+  const T a4 = a3 * g1;
+  T du[NF];
+#pragma unroll
+  for (int k = 0; k < NF; k++) du[k] = ur[k] - ul[k];
+  T a5 = usq * du[0], a6 = unm * du[0];
+#pragma unroll
+  for (int i = 0; i < ND; i++) {
+    a5 -= um[i] * du[1 + i];
+    a6 -= n[i] * du[1 + i];
+  }
+  a5 += du[ND + 1];
+  const T aL = a1 * a5 - a3 * a6;
+  const T bL = a4 * a5 - a2 * a6;
+  fn[0] = T(0.5) * (fn[0] - (l0 * du[0] + aL));
+#pragma unroll
+  for (int i = 0; i < ND; i++) fn[1 + i] = T(0.5) * (fn[1 + i] - (l0 * du[1 + i] + aL * um[i] + bL * n[i]));
+  fn[ND + 1] = T(0.5) * (fn[ND + 1] - (l0 * du[ND + 1] + aL * hm + bL * unm));
+}
+
+// Roe flux at an interface point seen from one side: own trace index fe, state
+// uo and outward normal n; the neighbour's trace index nb and state un.  Both sides
+// evaluate roe() with the state of the smaller trace index first and that side's
+// normal (the other side's normal is its exact negation), so the two get the same
+// value with opposite signs and the interface flux is exactly conservative.
+template <int ND, class T>
+__device__ __forceinline__ void roe_pair(long fe, long nb, const T* uo, const T* un, const T* n, T gamma, T* fn) {
+  if (fe < nb) {
+    roe<ND, T>(uo, un, n, gamma, fn);
+  } else {
+    T nl[ND];
+#pragma unroll
+    for (int l = 0; l < ND; l++) nl[l] = -n[l];
+    roe<ND, T>(un, uo, nl, gamma, fn);
+#pragma unroll
+    for (int k = 0; k < ND + 2; k++) fn[k] = -fn[k];
+  }
+}
+
 // Rusanov flux of the mixed path on deviations: (F_c - F(u_ref)) . n.
 template <int ND, class T>
 __device__ __forceinline__ void rusanov_dev(const T* duo, const T* dun, const T* n, T gamma, const RefState<T>& r,

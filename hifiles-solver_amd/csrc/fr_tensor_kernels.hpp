@@ -209,19 +209,23 @@ __device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base
 // number of waves that their LDS allows, so that registers do not bind first (at
 // P = 3: 128 VGPRs and 7 blocks per CU instead of 151 and 6); every other variant
 // would spill under that budget and keeps the compiler's choice.
-template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES>
+template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE>
 constexpr int kernel_waves() {
   using S = SmemLayout<P, ND, EPB, MODE, LES, MIX ? 4 : 8>;
-  return (MODE == MODE_ADVANCE && ND == 3 && P <= 4 && AFF && !MIX && LES == 0 && VIS != 2) ? S::WAVES : 1;
+  return (MODE == MODE_ADVANCE && ND == 3 && P <= 4 && AFF && !MIX && LES == 0 && VIS != 2 && RIE == 0) ? S::WAVES
+                                                                                                          : 1;
 }
 
-template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES>
-__global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK), (kernel_waves<P, ND, EPB, VIS, MODE, AFF, MIX, LES>()))
+// RIE: inviscid interface flux, 0 Rusanov, 2 Roe (riemann_solve_type)
+template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE = 0>
+__global__ void __launch_bounds__((TP<P, ND, EPB>::BLOCK),
+                                  (kernel_waves<P, ND, EPB, VIS, MODE, AFF, MIX, LES, RIE>()))
 fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
   constexpr bool VISC = VIS != 0;
   constexpr bool SUTH = VIS == 2;
   static_assert(LES == 0 || VISC, "LES needs the viscous kernels");
+  static_assert(RIE == 0 || (RIE == 2 && !MIX), "Roe flux: FP64 kernels only");
   using T = TP<P, ND, EPB>;
   using S = SmemLayout<P, ND, EPB, MODE, LES, (int)sizeof(R)>;
   constexpr int n = T::n, NU = T::NU, NFP = T::NFP, NF = T::NF, BLOCK = T::BLOCK;
@@ -748,23 +752,29 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         }
         const bool wall = (nb & BDY_FLAG) != 0;
         if (wall) {
-          // isothermal wall: central flux between the trace and the ghost state
-          // (convective_flux_boundary, src/inters.cpp:368-387)
           ghost(uo, un);
-          R fo[NF], fg[NF];
-          if constexpr (MIX) {
-            inv_flux_dot_dev<ND, R>(uo, nrm, gam, ref, fo);
-            inv_flux_dot_dev<ND, R>(un, nrm, gam, ref, fg);
+          if constexpr (RIE == 2) {
+            // Roe flux against the ghost state (src/bdy_inters.cpp:330)
+            roe<ND, R>(uo, un, nrm, gam, fn);
           } else {
-            inv_flux_dot<ND, R>(uo, nrm, gam, fo);
-            inv_flux_dot<ND, R>(un, nrm, gam, fg);
-          }
+            // isothermal wall: central flux between the trace and the ghost state
+            // (convective_flux_boundary, src/inters.cpp:368-387)
+            R fo[NF], fg[NF];
+            if constexpr (MIX) {
+              inv_flux_dot_dev<ND, R>(uo, nrm, gam, ref, fo);
+              inv_flux_dot_dev<ND, R>(un, nrm, gam, ref, fg);
+            } else {
+              inv_flux_dot<ND, R>(uo, nrm, gam, fo);
+              inv_flux_dot<ND, R>(un, nrm, gam, fg);
+            }
 #pragma unroll
-          for (int k = 0; k < NF; k++) fn[k] = R(0.5) * (fo[k] + fg[k]);
+            for (int k = 0; k < NF; k++) fn[k] = R(0.5) * (fo[k] + fg[k]);
+          }
         } else {
 #pragma unroll
           for (int k = 0; k < NF; k++) un[k] = a.uf_in[(long)nb * NF + k];
-          if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
+          if constexpr (RIE == 2) roe_pair<ND, R>(fe, nb, uo, un, nrm, gam, fn);
+          else if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
           else rusanov<ND, R>(uo, un, nrm, gam, fn);
         }
         R jump[NF];

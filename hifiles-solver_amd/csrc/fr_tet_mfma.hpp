@@ -218,10 +218,12 @@ fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>
   tet_write_traces<P, R>(s_q, a.uf_out, a.N, tid);
 }
 
-template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES>
+// RIE: inviscid interface flux, 0 Rusanov, 2 Roe (riemann_solve_type)
+template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE = 0>
 __global__ void __launch_bounds__(TetP<P>::BLOCK)
 fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
+  static_assert(RIE == 0 || (RIE == 2 && !MIX), "Roe flux: FP64 kernels only");
   constexpr int ND = 3, NF = 5, NFACE = 4;
   constexpr bool VISC = VIS != 0;
   constexpr bool SUTH = VIS == 2;
@@ -430,23 +432,29 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     for (int k = 0; k < NF; k++) uo[k] = a.uf_in[fe * NF + k];
     const bool wall = (nb & BDY_FLAG) != 0;
     if (wall) {
-      // isothermal wall: central flux between the trace and the ghost state
-      // (convective_flux_boundary, src/inters.cpp:368-387)
       ghost(uo, un);
-      R fo[NF], fg[NF];
-      if constexpr (MIX) {
-        inv_flux_dot_dev<ND, R>(uo, nrm, gam, ref, fo);
-        inv_flux_dot_dev<ND, R>(un, nrm, gam, ref, fg);
+      if constexpr (RIE == 2) {
+        // Roe flux against the ghost state (src/bdy_inters.cpp:330)
+        roe<ND, R>(uo, un, nrm, gam, fn);
       } else {
-        inv_flux_dot<ND, R>(uo, nrm, gam, fo);
-        inv_flux_dot<ND, R>(un, nrm, gam, fg);
-      }
+        // isothermal wall: central flux between the trace and the ghost state
+        // (convective_flux_boundary, src/inters.cpp:368-387)
+        R fo[NF], fg[NF];
+        if constexpr (MIX) {
+          inv_flux_dot_dev<ND, R>(uo, nrm, gam, ref, fo);
+          inv_flux_dot_dev<ND, R>(un, nrm, gam, ref, fg);
+        } else {
+          inv_flux_dot<ND, R>(uo, nrm, gam, fo);
+          inv_flux_dot<ND, R>(un, nrm, gam, fg);
+        }
 #pragma unroll
-      for (int k = 0; k < NF; k++) fn[k] = R(0.5) * (fo[k] + fg[k]);
+        for (int k = 0; k < NF; k++) fn[k] = R(0.5) * (fo[k] + fg[k]);
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < NF; k++) un[k] = a.uf_in[(long)nb * NF + k];
-      if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
+      if constexpr (RIE == 2) roe_pair<ND, R>(fe, nb, uo, un, nrm, gam, fn);
+      else if constexpr (MIX) rusanov_dev<ND, R>(uo, un, nrm, gam, ref, fn);
       else rusanov<ND, R>(uo, un, nrm, gam, fn);
     }
     R jump[NF];

@@ -151,6 +151,14 @@ struct StageIO {
 
 template <class R>
 hipError_t launch_args(hifr_ctx* c, int mode, const StageArgsT<R>& a) {
+  if constexpr (std::is_same_v<R, double>) {
+    // the Roe flux changes only the kernels that evaluate the inviscid interface flux
+    if (c->cfg.riemann_solve_type == 2 && (mode == hifr::MODE_RESID || mode == hifr::MODE_ADVANCE)) {
+      if (c->cfg.ele_type == HIFR_TET)
+        return hifr::launch_tet_roe_fp64(c->cfg.order, mode, vis_kind(c), c->affine, a, c->stream);
+      return hifr::launch_roe_fp64(c->nd, c->cfg.order, mode, vis_kind(c), c->affine, a, c->stream);
+    }
+  }
   if (c->cfg.ele_type == HIFR_TET) {
     if constexpr (std::is_same_v<R, float>)
       return hifr::launch_tet_mixed(c->cfg.order, mode, vis_kind(c), c->affine, les_kind(c), a, c->stream);
@@ -282,7 +290,11 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
     return HIFR_ERR_UNSUPPORTED;
   if (!order_supported(cfg->ele_type, cfg->order)) return HIFR_ERR_UNSUPPORTED;
   if (cfg->n_eles <= 0) return HIFR_ERR_ARG;
-  if (cfg->riemann_solve_type != 0 || cfg->vis_riemann_solve_type != 0) return HIFR_ERR_UNSUPPORTED;
+  // inviscid flux: Rusanov, or Roe in FP64 without LES; viscous flux: LDG
+  if (cfg->riemann_solve_type != 0 &&
+      !(cfg->riemann_solve_type == 2 && cfg->precision == HIFR_FP64 && !cfg->les))
+    return HIFR_ERR_UNSUPPORTED;
+  if (cfg->vis_riemann_solve_type != 0) return HIFR_ERR_UNSUPPORTED;
   if (cfg->precision != HIFR_FP64 && cfg->precision != HIFR_MIXED) return HIFR_ERR_UNSUPPORTED;
   if (cfg->les && (!cfg->viscous || cfg->fix_vis != 1.0 ||
                    !(cfg->sgs_model == HIFR_SGS_SMAGORINSKY || cfg->sgs_model == HIFR_SGS_WALE ||

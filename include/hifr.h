@@ -70,7 +70,7 @@ typedef struct {
   int order;                 /* polynomial order P */
   int n_eles;
   int viscous;               /* 0 Euler, 1 Navier-Stokes (LDG) */
-  int riemann_solve_type;    /* 0 Rusanov (only inviscid flux built) */
+  int riemann_solve_type;    /* 0 Rusanov; 2 Roe (FP64 without LES) */
   int vis_riemann_solve_type;/* 0 LDG */
   int precision;             /* HIFR_FP64 or HIFR_MIXED */
   double gamma, prandtl, mu_inf, rt_inf, c_sth, fix_vis;

@@ -1224,6 +1224,77 @@ static void rusanov_flux(int nd, double gamma, const double* ul, const double* u
   for (int k = 0; k < nf; k++) fn[k] = 0.5 * ((fnl[k] + fnr[k]) - eig * (ur[k] - ul[k]));
 }
 
+/* Roe flux with Harten's entropy fix, src/inters.cpp:390-520, static mesh (grid
+ * velocity zero).  The reference evaluates the first (central) part and the
+ * pressure in 2-D only and stops in 3-D; its dissipation part is written for both.
+ * Here the 3-D first part is the same expression with the third velocity
+ * component: rho u.n, rho u.n v_i + p n_i, rho u.n h. */
+static void roe_flux(int nd, double gamma, const double* ul, const double* ur, const double* n, double* fn) {
+  int nf = nd + 2;
+  double v_l[3] = {0, 0, 0}, v_r[3] = {0, 0, 0}, um[3] = {0, 0, 0}, du[5];
+  for (int i = 0; i < nd; i++) { v_l[i] = ul[i + 1] / ul[0]; v_r[i] = ur[i + 1] / ur[0]; }
+  double p_l, p_r;
+  if (nd == 2) {
+    p_l = (gamma - 1.0) * (ul[3] - (0.5 * ul[0] * ((v_l[0] * v_l[0]) + (v_l[1] * v_l[1]))));
+    p_r = (gamma - 1.0) * (ur[3] - (0.5 * ur[0] * ((v_r[0] * v_r[0]) + (v_r[1] * v_r[1]))));
+  } else {
+    p_l = (gamma - 1.0) * (ul[4] - (0.5 * ul[0] * ((v_l[0] * v_l[0]) + (v_l[1] * v_l[1]) + (v_l[2] * v_l[2]))));
+    p_r = (gamma - 1.0) * (ur[4] - (0.5 * ur[0] * ((v_r[0] * v_r[0]) + (v_r[1] * v_r[1]) + (v_r[2] * v_r[2]))));
+  }
+  double h_l = (ul[nd + 1] + p_l) / ul[0], h_r = (ur[nd + 1] + p_r) / ur[0];
+  double sq_rho = sqrt(ur[0] / ul[0]);
+  double rrho = 1. / (sq_rho + 1.);
+  for (int i = 0; i < nd; i++) um[i] = rrho * (v_l[i] + sq_rho * v_r[i]);
+  double hm = rrho * (h_l + sq_rho * h_r);
+  double usq = 0.;
+  for (int i = 0; i < nd; i++) usq += 0.5 * um[i] * um[i];
+  double am_sq = (gamma - 1.) * (hm - usq);
+  double am = sqrt(am_sq);
+  double unm = 0., vgn = 0.;
+  for (int i = 0; i < nd; i++) { unm += um[i] * n[i]; vgn += 0.0 * n[i]; }
+  double rhoun_l = 0., rhoun_r = 0.;
+  for (int i = 0; i < nd; i++) { rhoun_l += ul[i + 1] * n[i]; rhoun_r += ur[i + 1] * n[i]; }
+  fn[0] = rhoun_l + rhoun_r;
+  for (int i = 0; i < nd; i++) fn[i + 1] = rhoun_l * v_l[i] + rhoun_r * v_r[i] + (p_l + p_r) * n[i];
+  fn[nd + 1] = rhoun_l * h_l + rhoun_r * h_r;
+  for (int i = 0; i < nf; i++) du[i] = ur[i] - ul[i];
+  double lambda0 = fabs(unm - vgn), lambdaP = fabs(unm - vgn + am), lambdaM = fabs(unm - vgn - am);
+  double eps = 0.5 * (fabs(rhoun_l / ul[0] - rhoun_r / ur[0]) + fabs(sqrt(gamma * p_l / ul[0]) - sqrt(gamma * p_r / ur[0])));
+  if (lambda0 < 2. * eps) lambda0 = 0.25 * lambda0 * lambda0 / eps + eps;
+  if (lambdaP < 2. * eps) lambdaP = 0.25 * lambdaP * lambdaP / eps + eps;
+  if (lambdaM < 2. * eps) lambdaM = 0.25 * lambdaM * lambdaM / eps + eps;
+  double a2 = 0.5 * (lambdaP + lambdaM) - lambda0;
+  double a3 = 0.5 * (lambdaP - lambdaM) / am;
+  double a1 = a2 * (gamma - 1.) / am_sq;
+  double a4 = a3 * (gamma - 1.);
+  double a5, a6;
+  if (nd == 2) {
+    a5 = usq * du[0] - um[0] * du[1] - um[1] * du[2] + du[3];
+    a6 = unm * du[0] - n[0] * du[1] - n[1] * du[2];
+  } else {
+    a5 = usq * du[0] - um[0] * du[1] - um[1] * du[2] - um[2] * du[3] + du[4];
+    a6 = unm * du[0] - n[0] * du[1] - n[1] * du[2] - n[2] * du[3];
+  }
+  double aL1 = a1 * a5 - a3 * a6;
+  double bL1 = a4 * a5 - a2 * a6;
+  fn[0] = fn[0] - (lambda0 * du[0] + aL1);
+  for (int i = 0; i < nd; i++) fn[i + 1] = fn[i + 1] - (lambda0 * du[i + 1] + aL1 * um[i] + bL1 * n[i]);
+  fn[nd + 1] = fn[nd + 1] - (lambda0 * du[nd + 1] + aL1 * hm + bL1 * unm);
+  for (int i = 0; i < nf; i++) fn[i] = 0.5 * fn[i] - 0.5 * vgn * (ur[i] + ul[i]);
+}
+
+void or_common_flux(int nd, int riemann, double gamma, const double* ul, const double* ur, const double* n,
+                    double* fn) {
+  if (riemann == 2) {
+    roe_flux(nd, gamma, ul, ur, n, fn);
+  } else {
+    double fl[15], fr[15];
+    calc_invf(nd, gamma, ul, fl);
+    calc_invf(nd, gamma, ur, fr);
+    rusanov_flux(nd, gamma, ul, ur, fl, fr, n, fn);
+  }
+}
+
 /* LDG switch direction, src/inters.cpp:556-566 and 650-660 */
 static double ldg_pen(int nd, double pen, const double* n) {
   if (nd == 2) { if ((n[0] + n[1]) < 0.) pen = -pen; }
@@ -1330,9 +1401,13 @@ static void common_invFlux(or_solver* s) {
     double ul[5], ur[5], fl[15], fr[15], n[3] = {0, 0, 0}, fn[5];
     for (int k = 0; k < nf; k++) { ul[k] = s->disu_fpts[a + nfpN * k]; ur[k] = s->disu_fpts[b + nfpN * k]; }
     for (int m = 0; m < nd; m++) n[m] = s->norm_fpts[a + nfpN * m];
-    calc_invf(nd, s->ph.gamma, ul, fl);
-    calc_invf(nd, s->ph.gamma, ur, fr);
-    rusanov_flux(nd, s->ph.gamma, ul, ur, fl, fr, n, fn);
+    if (s->deck.riemann_solve_type == 2) {
+      roe_flux(nd, s->ph.gamma, ul, ur, n, fn);
+    } else {
+      calc_invf(nd, s->ph.gamma, ul, fl);
+      calc_invf(nd, s->ph.gamma, ur, fr);
+      rusanov_flux(nd, s->ph.gamma, ul, ur, fl, fr, n, fn);
+    }
     double tdl = s->tdA_fpts[a], tdr = s->tdA_fpts[b];
     for (int k = 0; k < nf; k++) {
       s->norm_tconf_fpts[a + nfpN * k] = fn[k] * tdl;
@@ -1351,7 +1426,8 @@ static void common_invFlux(or_solver* s) {
 
 /* bdy_inters::evaluate_boundaryConditions_invFlux, src/bdy_inters.cpp:245-389:
  * central flux (convective_flux_boundary, src/inters.cpp:368-387) between the
- * trace and the ghost state; LDG Dirichlet common solution u_c = (u_l + u_r)/2. */
+ * trace and the ghost state, or the Roe flux with riemann_solve_type 2
+ * (src/bdy_inters.cpp:330); LDG Dirichlet common solution u_c = (u_l + u_r)/2. */
 static void bdy_invFlux(or_solver* s) {
   int nf = s->nf, nd = s->nd;
   long nfpN = (long)s->nfp * s->n_eles;
@@ -1369,6 +1445,7 @@ static void bdy_invFlux(or_solver* s) {
       for (int l = 0; l < nd; l++) { fnl[k] += fl[k + nf * l] * n[l]; fnr[k] += fr[k + nf * l] * n[l]; }
     }
     for (int k = 0; k < nf; k++) fn[k] = 0.5 * (fnl[k] + fnr[k]);
+    if (s->deck.riemann_solve_type == 2) roe_flux(nd, s->ph.gamma, ul, ur, n, fn);
     for (int k = 0; k < nf; k++) s->norm_tconf_fpts[a + nfpN * k] = fn[k] * s->tdA_fpts[a];
     if (s->deck.viscous)
       for (int k = 0; k < nf; k++) {

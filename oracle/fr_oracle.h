@@ -117,6 +117,10 @@ void or_set_metrics(or_solver* s, const double* detjac_upts, const double* JGinv
                     const double* norm_fpts);
 /* Number of OpenMP threads used by the dense loops (1 = serial). */
 void or_set_threads(int n);
+/* Common normal flux at one point (static mesh): riemann 0 Rusanov, 2 Roe;
+ * ul, ur, fn: nd + 2 conserved values; n: unit normal (nd). */
+void or_common_flux(int nd, int riemann, double gamma, const double* ul, const double* ur, const double* n,
+                    double* fn);
 
 #ifdef __cplusplus
 }

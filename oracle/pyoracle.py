@@ -73,6 +73,7 @@ def lib():
         L.or_run_steps.argtypes = [vp, C.c_int]
         L.or_norm_residual.argtypes = [vp, C.c_int, dp]
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_common_flux.argtypes = [C.c_int, C.c_int, C.c_double, dp, dp, dp, dp]
         _LIB = L
     return _LIB
 
@@ -85,6 +86,14 @@ def default_deck(**kw):
             raise KeyError(k)
         setattr(d, k, v)
     return d
+
+
+def common_flux(riemann, ul, ur, n, gamma=1.4):
+    """Common normal flux of one point (riemann 0 Rusanov, 2 Roe), static mesh."""
+    ul, ur, n = (np.ascontiguousarray(v, dtype=np.float64) for v in (ul, ur, n))
+    fn = np.zeros(len(ul))
+    lib().or_common_flux(len(n), riemann, gamma, _dp(ul), _dp(ur), _dp(n), _dp(fn))
+    return fn
 
 
 def setup_params(deck):

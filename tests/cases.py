@@ -67,7 +67,8 @@ def product_deck(g, host_only=False, **over):
     kw = dict(viscous=int(g["viscous"]), order=int(g["order"]), tau=float(g["tau"]),
               pen_fact=float(g["pen_fact"]), dt=float(g["dt"]),
               adv_type=3 if str(g["kind"]) == "rk45" else 0,
-              dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2], fix_vis=int(g.get("fix_vis", 1)))
+              dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2], fix_vis=int(g.get("fix_vis", 1)),
+              riemann_solve_type=int(g.get("riemann", 0)))
     kw.update(phys_of(g))
     if ele_type == 4:
         kw["vcjh_scheme_hexa"] = int(g["vcjh"])
@@ -92,7 +93,7 @@ def oracle_deck(g, **over):
               pen_fact=float(g["pen_fact"]), vcjh_scheme=int(g["vcjh"]), eta=0.0,
               dx_cyclic=cyc[0], dy_cyclic=cyc[1], dz_cyclic=cyc[2], LES=int(g.get("les", 0)),
               SGS_model=int(g.get("sgs_model", 0)), filter_ratio=float(g.get("filter_ratio", 1.0)),
-              fix_vis=float(g.get("fix_vis", 1)))
+              fix_vis=float(g.get("fix_vis", 1)), riemann_solve_type=int(g.get("riemann", 0)))
     kw.update(phys_of(g))
     kw.update(over)
     return po.default_deck(**kw)

@@ -42,7 +42,7 @@ TWO_PI = 6.2831853071795862
 
 def deck_text(c, restart_iter=0):
     lines = [
-        "equation 0", "viscous %d" % c["viscous"], "riemann_solve_type 0", "vis_riemann_solve_type 0",
+        "equation 0", "viscous %d" % c["viscous"], "riemann_solve_type %d" % c.get("riemann", 0), "vis_riemann_solve_type 0",
         "ic_form %d" % c.get("ic_form", 1), "test_case 0", "order %d" % c["order"], "dt_type 0",
         "dt %.17g" % c["dt"], "n_steps %d" % c["n_steps"], "adv_type %d" % c["adv_type"],
         "tau %.17g" % c.get("tau", 0.0), "pen_fact %.17g" % c.get("pen_fact", 0.5),
@@ -77,7 +77,8 @@ def oracle_deck(c):
         Re_free_stream=c.get("Re", 1600.0), L_free_stream=1.0, T_free_stream=300.0, T_wall=c.get("T_wall", 300.0),
         vcjh_scheme=c["vcjh"], eta=0.0, dx_cyclic=c["cyc"][0], dy_cyclic=c["cyc"][1],
         dz_cyclic=c["cyc"][2], LES=int(c.get("les", 0)), SGS_model=c.get("sgs_model", 0),
-        filter_ratio=c.get("filter_ratio", 1.0), fix_vis=float(c.get("fix_vis", 1)))
+        filter_ratio=c.get("filter_ratio", 1.0), fix_vis=float(c.get("fix_vis", 1)),
+        riemann_solve_type=c.get("riemann", 0))
 
 
 def write_restart(path, ele_type, loc1d, order, u0, loc_upts=None):
@@ -115,6 +116,24 @@ def read_restart(path, nu, n, nf):
     return u
 
 
+def write_restart_hifr(path, c, xv, conn_g, u0):
+    """The initial restart file written by the library's own writer
+    (hifr_solver_write_restart, host-only solver holding u0)."""
+    sys.path.insert(0, os.path.join(ROOT, "hifiles-solver_amd"))
+    import hifr
+    c2v = frmesh.gambit_to_internal(c["ele_type"], conn_g)
+    kw = dict(viscous=c["viscous"], order=c["order"], tau=c.get("tau", 0.0), pen_fact=c.get("pen_fact", 0.5),
+              dt=c["dt"], adv_type=c["adv_type"], dx_cyclic=min(c["cyc"][0], 1e30),
+              dy_cyclic=min(c["cyc"][1], 1e30), dz_cyclic=min(c["cyc"][2], 1e30), hifr_host_only=1,
+              gamma=1.4, prandtl=0.72, Mach_free_stream=c.get("mach", 0.1), nx_free_stream=1.0,
+              Re_free_stream=c.get("Re", 1600.0), L_free_stream=1.0, T_free_stream=300.0)
+    kw[{4: "vcjh_scheme_hexa", 2: "vcjh_scheme_tet", 1: "vcjh_scheme_quad"}[c["ele_type"]]] = c["vcjh"]
+    s = hifr.Solver.from_arrays(hifr.deck_text(**kw), c["ele_type"], xv, c2v)
+    s.set_state(u0)
+    s.write_restart(path)
+    s.close()
+
+
 def run_reference(c, xv, conn_g, u0, loc1d, loc_upts=None):
     wd = tempfile.mkdtemp(prefix="golden_")
     try:
@@ -129,12 +148,18 @@ def run_reference(c, xv, conn_g, u0, loc1d, loc_upts=None):
         frmesh.write_gambit(os.path.join(wd, "mesh.neu"), xv, conn_g, c["ele_type"], bc_groups=groups)
         with open(os.path.join(wd, "input"), "w") as fh:
             fh.write(deck_text(c))
-        write_restart(os.path.join(wd, "Rest_000000000_p0000.dat"), c["ele_type"], loc1d, c["order"], u0, loc_upts)
+        if c.get("restart_writer") == "hifr":
+            write_restart_hifr(os.path.join(wd, "Rest_000000000_p0000.dat"), c, xv, conn_g, u0)
+        else:
+            write_restart(os.path.join(wd, "Rest_000000000_p0000.dat"), c["ele_type"], loc1d, c["order"], u0,
+                          loc_upts)
         env = dict(os.environ, HIFILES_HOME=REF_HOME)
         r = subprocess.run([REF_BIN, "input"], cwd=wd, env=env, capture_output=True, text=True, timeout=3600)
         if r.returncode != 0:
             raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
         out = os.path.join(wd, "Rest_%09d_p0000.dat" % c["n_steps"])
+        with open(out) as fh:
+            c["rest_text"] = fh.read()
         return read_restart(out, *u0.shape), r.stdout
     finally:
         shutil.rmtree(wd, ignore_errors=True)
@@ -206,6 +231,26 @@ CASES = [
     dict(name="hex_p2_channel_walls_sutherland_rk45_2x2x2", ele_type=4, n=(2, 2, 2), order=2, vcjh=1, viscous=1,
          tau=1.0, pen_fact=0.5, state="channel", noise=5e-2, seed=54, perturb=0.0, kind="rk45", steps=3, dt=1e-3,
          channel=True, mach=0.2, Re=2800.0, T_wall=320.0, fix_vis=0),
+    # restart round trip: the initial state reaches the reference through the
+    # library's restart writer; the reference's final restart file is kept as text
+    dict(name="hex_p2_ns_restart_2x2x1", ele_type=4, n=(2, 2, 1), order=2, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=81, perturb=0.0, kind="rk45", steps=2, dt=1e-3,
+         restart_writer="hifr"),
+    dict(name="tet_p2_ns_restart_2x1x1", ele_type=2, n=(2, 1, 1), order=2, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=82, perturb=0.05, kind="rk45", steps=2, dt=1e-3,
+         restart_writer="hifr"),
+    dict(name="quad_p3_euler_restart_3x3", ele_type=1, n=(3, 3), order=3, vcjh=1, viscous=0,
+         state="vortex", noise=1e-2, seed=83, perturb=0.05, kind="rk45", steps=2, dt=1e-3, restart_writer="hifr"),
+    # Roe flux with entropy fix (riemann_solve_type 2, src/inters.cpp:390-520); the
+    # reference evaluates it in 2-D only, so these are quads
+    dict(name="quad_p3_euler_roe_perturbed_4x3", ele_type=1, n=(4, 3), order=3, vcjh=1, viscous=0,
+         state="vortex", noise=1e-2, seed=61, perturb=0.1, kind="residual", riemann=2),
+    dict(name="quad_p3_ns_roe_channel_walls_4x3", ele_type=1, n=(4, 3), order=3, vcjh=2, viscous=1, tau=1.0,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=62, perturb=0.1, kind="residual", channel=True,
+         mach=0.2, Re=2800.0, T_wall=305.0, riemann=2),
+    dict(name="quad_p2_ns_roe_rk45_3x3", ele_type=1, n=(3, 3), order=2, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="vortex", noise=1e-2, seed=63, perturb=0.05, kind="rk45", steps=3, dt=1e-3,
+         riemann=2),
 ]
 
 
@@ -282,7 +327,9 @@ def main():
             u0 = frmesh.shu_vortex_state(pos)
         if c["noise"] > 0:
             u0 = u0 * (1.0 + frmesh.seeded_perturbation(u0.shape, c["noise"], c["seed"]))
-        u0 = np.array([float("%.17g" % v) for v in u0.ravel()]).reshape(u0.shape)
+        # 17 digits reach the reference exactly; the library's writer prints 15
+        fmt = "%.15g" if c.get("restart_writer") == "hifr" else "%.17g"
+        u0 = np.array([float(fmt % v) for v in u0.ravel()]).reshape(u0.shape)
         loc1d = o.get("loc_upts")[0, : c["order"] + 1]
         u1, log = run_reference(c, xv, conn_g, u0, loc1d, o.get("loc_upts"))
         rec = dict(xv=xv, conn_gambit=conn_g, ele_type=c["ele_type"], order=c["order"],
@@ -291,7 +338,10 @@ def main():
                    u0=u0, kind=c["kind"], les=int(c.get("les", 0)), sgs_model=int(c.get("sgs_model", 0)),
                    filter_ratio=float(c.get("filter_ratio", 1.0)),
                    bfaces=np.array(bfaces, dtype=np.int32).reshape(-1, 3), mach=c.get("mach", 0.1),
-                   Re=c.get("Re", 1600.0), T_wall=c.get("T_wall", 300.0), fix_vis=int(c.get("fix_vis", 1)))
+                   Re=c.get("Re", 1600.0), T_wall=c.get("T_wall", 300.0), fix_vis=int(c.get("fix_vis", 1)),
+                   riemann=int(c.get("riemann", 0)))
+        if c.get("restart_writer"):
+            rec["rest_text"] = np.array(c["rest_text"])
         if c["kind"] == "residual":
             R = (u0 - u1) / c["dt"]
             rec["R"] = R

@@ -459,3 +459,113 @@ def test_set_metrics_replaces_the_computed_metrics():
     R1 = o.residual()
     err = _cases.field_rel_err(R1, R0)
     assert err.max() < 1e-11 and not np.array_equal(R1, R0), err
+
+
+# ---- Roe flux (riemann_solve_type 2) ----
+# The 2-D Roe flux is pinned by the reference-binary fixtures (quad_*_roe_*: the
+# residual of Euler and Navier-Stokes decks with walls, and an RK45 trajectory).
+# The reference has no 3-D Roe; the oracle's 3-D form is pinned below by what the
+# mathematics fixes: it reduces to the 2-D flux when the third velocity and normal
+# components vanish, it commutes with rotations of the frame, and it is consistent.
+
+
+def _roe_states(rng, nd, n=64):
+    rho = rng.uniform(0.5, 2.0, (n, 2))
+    v = rng.uniform(-0.8, 0.8, (n, 2, nd))
+    p = rng.uniform(0.5, 3.0, (n, 2))
+    E = p / 0.4 + 0.5 * rho * (v ** 2).sum(-1)
+    u = np.concatenate([rho[..., None], rho[..., None] * v, E[..., None]], axis=-1)
+    nrm = rng.normal(size=(n, nd))
+    nrm /= np.linalg.norm(nrm, axis=1)[:, None]
+    return u, nrm
+
+
+def _euler_flux_n(u, n, gamma=1.4):
+    nd = len(n)
+    v = u[1:1 + nd] / u[0]
+    p = (gamma - 1.0) * (u[nd + 1] - 0.5 * u[0] * (v @ v))
+    vn = v @ n
+    return np.concatenate([[u[0] * vn], u[1:1 + nd] * vn + p * n, [(u[nd + 1] + p) * vn]])
+
+
+def test_roe_3d_reduces_to_2d():
+    rng = np.random.default_rng(71)
+    u2, n2 = _roe_states(rng, 2)
+    for (ul, ur), n in zip(u2, n2):
+        f2 = po.common_flux(2, ul, ur, n)
+        lift = lambda u: np.array([u[0], u[1], u[2], 0.0, u[3]])
+        f3 = po.common_flux(2, lift(ul), lift(ur), np.array([n[0], n[1], 0.0]))
+        assert np.array_equal(f3[[0, 1, 2, 4]], f2) and f3[3] == 0.0
+
+
+def test_roe_3d_rotation_invariant():
+    """F(Q u_L, Q u_R, Q n) = Q F(u_L, u_R, n) for a rotation Q acting on the momentum:
+    a swapped or dropped velocity component, or a wrong normal index, breaks it."""
+    rng = np.random.default_rng(72)
+    u3, n3 = _roe_states(rng, 3)
+    for (ul, ur), n in zip(u3, n3):
+        Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        rot = lambda u: np.concatenate([[u[0]], Q @ u[1:4], [u[4]]])
+        fa = po.common_flux(2, rot(ul), rot(ur), Q @ n)
+        fb = rot(po.common_flux(2, ul, ur, n))
+        assert np.allclose(fa, fb, rtol=0, atol=1e-12 * np.abs(fb).max())
+
+
+def test_roe_3d_from_planar_2d_in_every_plane():
+    """A 2-D state embedded in the (y, z) or (x, z) plane gives the 2-D flux in that
+    plane's components (the 2-D flux itself is pinned by the reference)."""
+    rng = np.random.default_rng(73)
+    u2, n2 = _roe_states(rng, 2)
+    for plane in ((1, 2), (0, 2), (2, 0)):
+        for (ul, ur), n in zip(u2, n2):
+            def emb(u):
+                out = np.zeros(5)
+                out[0], out[4] = u[0], u[3]
+                out[1 + plane[0]], out[1 + plane[1]] = u[1], u[2]
+                return out
+            n3 = np.zeros(3)
+            n3[plane[0]], n3[plane[1]] = n
+            f3 = po.common_flux(2, emb(ul), emb(ur), n3)
+            f2 = po.common_flux(2, ul, ur, n)
+            want = emb(f2)
+            assert np.allclose(f3, want, rtol=0, atol=1e-13 * np.abs(f2).max())
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_roe_consistent_and_upwind(nd):
+    """F(u, u, n) = f(u).n; for a supersonic state moving along n the Roe flux is
+    the left state's flux (all waves run to the right)."""
+    rng = np.random.default_rng(74 + nd)
+    u, nrm = _roe_states(rng, nd)
+    for (ul, _), n in zip(u, nrm):
+        f = po.common_flux(2, ul, ul, n)
+        assert np.allclose(f, _euler_flux_n(ul, n), rtol=0, atol=1e-13 * np.abs(f).max())
+    n = np.zeros(nd)
+    n[0] = 1.0
+    for (ul, ur), _ in zip(u, nrm):
+        for s in (ul, ur):
+            s[1:1 + nd] = s[0] * np.r_[4.0, np.zeros(nd - 1)] + 0.01 * s[1:1 + nd]
+            p = 1.0
+            s[nd + 1] = p / 0.4 + 0.5 * (s[1:1 + nd] @ s[1:1 + nd]) / s[0]
+        f = po.common_flux(2, ul, ur, n)
+        assert np.allclose(f, _euler_flux_n(ul, n), rtol=1e-12, atol=1e-12)
+
+
+def test_roe_goldens_pin_the_flux():
+    """The Roe fixtures are far from what the Rusanov flux gives, so the
+    reference comparisons above pin the oracle's 2-D Roe flux."""
+    names = [n for n in cases.golden_names() if int(cases.load_golden(n).get("riemann", 0)) == 2]
+    assert len(names) >= 3
+    for name in names:
+        g = cases.load_golden(name)
+        ele_type, xv, c2v = cases.mesh_of(g)
+        o = po.Oracle(cases.oracle_deck(g, riemann_solve_type=0), ele_type, xv, c2v, bfaces=cases.bfaces_of(g))
+        o.set_state(g["u0"])
+        if str(g["kind"]) == "residual":
+            o.residual()
+            R = o.get("div_tconf_upts") / o.get("detjac_upts")[:, :, None]
+            assert cases.field_rel_err(R, g["R"]).max() > 1e-5
+        else:
+            o.zero_rk_register()
+            o.run_steps(int(g["steps"]))
+            assert cases.field_rel_err(o.get("disu_upts"), g["u_final"]).max() > 1e-9
