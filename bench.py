@@ -23,9 +23,13 @@ and RK update) with its error against the FP64 state after the same steps.
 `roofline` is for the dominant kernel (flux, divergence
 and RK update, one launch per stage): algorithmic bytes per launch / its average
 duration from HIP events recorded around every launch in the timed region.
+`stage.grad_kernel_roofline` is the same figure for the gradient-trace kernel.
+`per_point_fp64` repeats the FP64 measurement with per-point metrics (the storage a
+curved mesh needs; the default keeps one metric set per affine element).
 `cpu_baseline` is the oracle (the C restatement of the reference's dense-operator
-CPU residual) on the same mesh and state, whole RK45 steps up to about 10 s of CPU
-work, timed on this host.
+CPU residual) on the same mesh and state on this host: all physical cores (median
+of 5 RK45 steps) and one core (median of 5 RK stages), with the CPU model and the
+oracle's calibration against the reference binary.
 """
 import argparse
 import json
@@ -117,6 +121,12 @@ def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
     return grad, rk
 
 
+# Where the traffic constants below come from: they are not measured by this run.
+TRAFFIC_SOURCE = {
+    "tgv": "constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in profiles/fr_tensor_kernel_tgv32_p3.md",
+    "tet": "constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in profiles/fr_tet_mfma_kernel_tet_channel_p3.md",
+}
+
 # HBM traffic per element of each stage kernel from rocprofv3 --pmc FETCH_SIZE (x2.0)
 # and WRITE_SIZE (x1.0), KiB per dispatch, the factors read off a 1 GiB copy kernel;
 # keyed by (workload, per-element metrics, precision).  Sources: the summaries
@@ -148,37 +158,102 @@ def survey_model_bytes(nu, nfp, nd=3, les=False):
     return b + (8 * 2 * nfp * nf * nd if les else 0)
 
 
+def cpu_model():
+    """CPU model name and the physical cores this process may run on."""
+    model, cores = "unknown", set()
+    allowed = os.sched_getaffinity(0) if hasattr(os, "sched_getaffinity") else set(range(os.cpu_count() or 1))
+    try:
+        cpu, phys, core = None, "0", None
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "processor":
+                    cpu = int(val)
+                elif key == "model name":
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                elif key == "core id":
+                    core = val
+                elif not key and cpu is not None:
+                    if cpu in allowed:
+                        cores.add((phys, core if core is not None else str(cpu)))
+                    cpu, phys, core = None, "0", None
+        if cpu is not None and cpu in allowed:
+            cores.add((phys, core if core is not None else str(cpu)))
+    except OSError:
+        pass
+    return model, max(1, len(cores)) if cores else len(allowed)
+
+
+# The oracle against the reference binary (oracle/_ref/HiFiLES_cpu, serial,
+# no-BLAS dgemm, -O2), both on one core of the build host over the same 8^3 P=3
+# Navier-Stokes TGV workload: measured by oracle/calibrate_ref.py, recorded in
+# profiles/cpu_baseline_calibration.md.  The oracle is the faster of the two, so
+# the baseline below overstates the reference's CPU speed.
+CPU_CALIBRATION = {"host": "Intel(R) Xeon(R) Processor @ 2.70GHz", "workload": "8^3 hexes P=3 NS TGV, RK45, 1 core",
+                   "reference_s_per_step": 0.9686, "oracle_s_per_step": 0.3449, "oracle_over_reference_speed": 2.808,
+                   "source": "profiles/cpu_baseline_calibration.md (oracle/calibrate_ref.py)"}
+
+
 def cpu_baseline(s, w, order, g):
+    """The oracle (dense reference-order operators) on this host's cores, on the
+    same mesh and state: one core, the median of 5 RK stages after a warm-up stage;
+    all physical cores available (at most 16), the median of 5 RK45 steps after a
+    warm-up step.  Wall-clock time (perf_counter)."""
+    import statistics
+
     from oracle import pyoracle as po
-    threads = min(16, os.cpu_count() or 1)
-    if w["kind"] == "tet":
-        import cases
-        o = cases.oracle_of(g, threads=threads)
-        o.set_state(cases.channel_tet_state(g, o.get("pos_upts")))
-        what = "the same 20,736-tet P=%d channel workload" % order
-    else:
+    model, phys = cpu_model()
+    threads = max(1, min(16, phys))
+
+    def make(nthreads):
+        if w["kind"] == "tet":
+            import cases
+            o = cases.oracle_of(g, threads=nthreads)
+            o.set_state(cases.channel_tet_state(g, o.get("pos_upts")))
+            return o, "the same 20,736-tet P=%d channel workload" % order
         xv = s.get("xv").reshape(-1, 3)
         c2v = s.get("c2v").reshape(-1, 8).astype(np.int32)
         od = po.default_deck(equation=0, viscous=1, order=order, adv_type=3, dt=w["dt"], tau=1.0, pen_fact=0.5,
                              gamma=1.4, prandtl=0.71, Mach_free_stream=0.1, nx_free_stream=1.0,
                              Re_free_stream=1600.0, L_free_stream=1.0, T_free_stream=300.0, vcjh_scheme=1,
                              dx_cyclic=TWO_PI, dy_cyclic=TWO_PI, dz_cyclic=TWO_PI)
-        o = po.Oracle(od, 4, xv, c2v, threads=threads)
+        o = po.Oracle(od, 4, xv, c2v, threads=nthreads)
         o.set_state(tgv_state(o.get("pos_upts")))
-        what = "the same %d^3 P=%d Navier-Stokes workload" % (round(s.n_eles ** (1 / 3)), order)
+        return o, "the same %d^3 P=%d Navier-Stokes workload" % (round(s.n_eles ** (1 / 3)), order)
+
+    # all cores: whole RK45 steps
+    o, what = make(threads)
+    dof = o.nu * o.n_eles * o.nf
     o.zero_rk_register()
     o.run_steps(1)  # warm-up
-    # timed sample: whole RK45 steps until about 10 s of CPU work
-    steps, dt = 0, 0.0
-    while dt < 10.0 and steps < 200:
+    steps = []
+    for _ in range(5):
         t0 = time.perf_counter()
         o.run_steps(1)
-        dt += time.perf_counter() - t0
-        steps += 1
-    dof = o.nu * o.n_eles * o.nf
-    return {"value": round(dof * 5 * steps / dt / 1e9, 4), "unit": "GDOF-stage/s", "cores": threads, "kind": "port",
-            "sample": "%d RK45 steps (%d residual stages) of %s, dense reference-order operators, %d OpenMP "
-                      "threads, %.1f s" % (steps, 5 * steps, what, threads, dt)}
+        steps.append(time.perf_counter() - t0)
+    t_step = statistics.median(steps)
+    del o
+    # one core: single RK stages (residual + update), each a fifth of a step
+    o, _ = make(1)
+    o.zero_rk_register()
+    stages = []
+    for i in range(6):
+        t0 = time.perf_counter()
+        o.residual()
+        o.advance(i % 5, 3)
+        if i:  # the first is the warm-up
+            stages.append(time.perf_counter() - t0)
+    t_stage = statistics.median(stages)
+    return {"value": round(dof * 5 / t_step / 1e9, 5), "unit": "GDOF-stage/s", "cores": threads, "kind": "port",
+            "sample": "median of 5 RK45 steps (after 1 warm-up step) of %s, dense reference-order operators, "
+                      "%d OpenMP threads, %.2f s per step" % (what, threads, t_step),
+            "one_core": {"value": round(dof / t_stage / 1e9, 5), "unit": "GDOF-stage/s",
+                         "sample": "median of 5 RK stages (after 1 warm-up stage), 1 thread, %.2f s per stage"
+                                   % t_stage},
+            "cpu": model, "physical_cores_available": phys, "calibration": CPU_CALIBRATION}
 
 
 def main():
@@ -232,6 +307,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic["rk"] * s.n_eles) if traffic else None,
+                     "traffic_source": TRAFFIC_SOURCE[w["kind"]] if traffic else None,
                      "kernel": "%s ADVANCE (flux, divergence, RK update, next traces)" % kname,
                      "bytes_per_launch": r["rk_b"] * s.n_eles, "bytes_per_element": r["rk_b"],
                      "avg_launch_ms": round(r["avg_rk_ms"], 5)},
@@ -249,6 +325,18 @@ def main():
         "residual_L1": [float("%.6e" % v) for v in r["norms"]],
         "setup_s": round(r["setup"], 2),
     }
+    if not args.per_point_metrics:
+        pp = measure(args, w, n, order, "fp64", per_point=True)
+        l2p, _ = rel_errors(pp["solver"].get("disu_upts"), s.get("disu_upts"))
+        rec["per_point_fp64"] = {"value": round(pp["value"], 4), "unit": "GDOF-stage/s",
+                                 "ms_per_step": round(pp["wall"] * 1e3 / args.steps, 4),
+                                 "roofline": {"achieved": round(pp["achieved"], 1),
+                                              "frac": round(pp["achieved"] / HBM_PEAK_GBS, 4),
+                                              "bytes_per_element": pp["rk_b"],
+                                              "avg_launch_ms": round(pp["avg_rk_ms"], 5)},
+                                 "stage": stage_block(pp, pp["solver"], None, w),
+                                 "max_rel_l2_diff_vs_per_element": float("%.3e" % l2p.max())}
+        del pp
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(s, w, order, fp64["g"])
     else:
@@ -263,9 +351,10 @@ def rel_errors(a, b):
     return np.sqrt((d ** 2).sum(0) / (ref ** 2).sum(0)), np.abs(d).max(0) / np.abs(ref).max(0)
 
 
-def measure(args, w, n, order, precision):
+def measure(args, w, n, order, precision, per_point=None):
     t_setup = time.perf_counter()
-    s, g = make_solver(w, n, order, args.steps, args.per_point_metrics, precision)
+    per_point = args.per_point_metrics if per_point is None else per_point
+    s, g = make_solver(w, n, order, args.steps, per_point, precision)
     t_setup = time.perf_counter() - t_setup
     s.run(args.warmup)
     s.synchronize()
@@ -293,8 +382,11 @@ def stage_block(r, s, traffic, w):
     stage_ms = r["avg_rk_ms"] + r["avg_grad_ms"]
     stage_bytes = (r["grad_b"] + r["rk_b"]) * s.n_eles
     model = survey_model_bytes(s.nu, s.nfp, s.nd, les=w["kind"] == "tet")
+    grad_gbs = r["grad_b"] * s.n_eles / (r["avg_grad_ms"] * 1e-3) / 1e9 if r["avg_grad_ms"] else 0.0
     return {"avg_grad_kernel_ms": round(r["avg_grad_ms"], 5), "avg_rk_kernel_ms": round(r["avg_rk_ms"], 5),
             "grad_kernel_bytes_per_element": r["grad_b"],
+            "grad_kernel_roofline": {"achieved": round(grad_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": round(grad_gbs / HBM_PEAK_GBS, 4)},
             "grad_kernel_traffic": round(traffic["grad"] * s.n_eles) if traffic else None,
             "algorithmic_bytes": stage_bytes,
             "achieved_GBs": round(stage_bytes / (stage_ms * 1e-3) / 1e9, 1),

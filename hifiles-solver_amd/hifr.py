@@ -48,6 +48,9 @@ def lib():
         L.hifr_solver_run.argtypes = [vp, C.c_int]
         L.hifr_solver_norms.argtypes = [vp, C.c_int, dp]
         L.hifr_solver_write_restart.argtypes = [vp, cp]
+        L.hifr_solver_read_restart.argtypes = [vp, cp, cp, C.c_int]
+        L.hifr_solver_time.argtypes = [vp]
+        L.hifr_solver_time.restype = C.c_double
         L.hifr_solver_ctx.argtypes = [vp]
         L.hifr_solver_ctx.restype = vp
         L.hifr_solver_dt.argtypes = [vp]
@@ -210,6 +213,19 @@ class Solver:
 
     def ctx(self):
         return lib().hifr_solver_ctx(self._h)
+
+    def write_restart(self, path):
+        """The state in the reference's restart format (15 significant digits)."""
+        if lib().hifr_solver_write_restart(self._h, str(path).encode()) != 0:
+            raise RuntimeError("write_restart failed")
+
+    def read_restart(self, path):
+        """Reads a restart file of the reference's format into the state."""
+        err = C.create_string_buffer(1024)
+        self._check(lib().hifr_solver_read_restart(self._h, str(path).encode(), err, 1024), err)
+
+    def time(self):
+        return lib().hifr_solver_time(self._h)
 
     def metrics_per_element(self):
         """True when the device keeps one metric set per element (all elements affine)."""

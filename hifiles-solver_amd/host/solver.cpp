@@ -22,6 +22,7 @@ struct hifr_solver {
   std::unique_ptr<eles> el;
   int_inters ii;
   double time = 0.0;
+  std::vector<double> u_host;  // the state of a host-only solver (no device context)
 };
 
 namespace {
@@ -106,11 +107,9 @@ std::string set_ics(hifr_solver* s, std::vector<double>& u) {
   return "";
 }
 
-// Reads Rest_%09d_p0000.dat (src/solver.cpp:390-444, src/eles.cpp:623-716); the
-// file's solution points must be the solver's (same order and 1-D points).
-std::string read_restart(hifr_solver* s, std::vector<double>& u) {
-  char name[64];
-  std::snprintf(name, sizeof name, "Rest_%09d_p%04d.dat", s->in.restart_iter, 0);
+// Reads a restart file (src/solver.cpp:390-444, src/eles.cpp:623-716); the file's
+// solution points must be the solver's (same order and 1-D points).
+std::string read_restart_file(hifr_solver* s, const char* name, std::vector<double>& u) {
   std::ifstream f(name);
   if (!f) return std::string("Could not open restart file ") + name;
   eles& e = *s->el;
@@ -156,6 +155,13 @@ std::string read_restart(hifr_solver* s, std::vector<double>& u) {
   return "";
 }
 
+// Rest_<restart_iter>_p0000.dat in the working directory, as the reference reads it
+std::string read_restart(hifr_solver* s, std::vector<double>& u) {
+  char name[64];
+  std::snprintf(name, sizeof name, "Rest_%09d_p%04d.dat", s->in.restart_iter, 0);
+  return read_restart_file(s, name, u);
+}
+
 // GeoPreprocess + InitSolution on an already filled mesh
 std::string setup_solver(hifr_solver* s) {
   input& in = s->in;
@@ -192,7 +198,11 @@ std::string setup_solver(hifr_solver* s) {
   if (!err.empty()) return err;
   for (int t : s->ii.bdy_type)
     if (t != 11) return "boundary condition " + std::to_string(t) + " is not built (isothermal walls, bctype 11)";
-  if (in.host_only) return "";
+  if (in.host_only) {
+    // the initial state stays on the host
+    err = in.restart_flag ? read_restart(s, s->u_host) : set_ics(s, s->u_host);
+    return err;
+  }
   err = e.mv_all_cpu_gpu(in);
   if (!err.empty()) return err;
   if (!s->ii.bdy_fpt.empty()) {
@@ -345,15 +355,41 @@ long hifr_solver_get(hifr_solver* s, int what, double* out) {
     if (out) std::memcpy(out, v->data(), sizeof(double) * n);
     return n;
   }
-  if (!e.ctx) return -1;
+  if (!e.ctx) {
+    // host-only solver: the state it holds
+    if (dev != HIFR_DISU_UPTS || s->u_host.size() != (size_t)n) return -1;
+    if (out) std::memcpy(out, s->u_host.data(), sizeof(double) * n);
+    return n;
+  }
   if (out && n > 0 && hifr_download(e.ctx, dev, out) != HIFR_OK) return -1;
   return n;
 }
 
 int hifr_solver_set_state(hifr_solver* s, const double* u) {
-  if (!s->el->ctx) return 1;
+  if (!s->el->ctx) {
+    const eles& e = *s->el;
+    s->u_host.assign(u, u + (size_t)e.n_upts_per_ele * e.n_eles * e.n_fields);
+    return 0;
+  }
   return hifr_upload_state(s->el->ctx, u) == HIFR_OK ? 0 : 1;
 }
+
+int hifr_solver_read_restart(hifr_solver* s, const char* path, char* err, int errlen) {
+  std::vector<double> u;
+  const double t0 = s->time;
+  const std::string e = read_restart_file(s, path, u);
+  if (!e.empty()) {
+    s->time = t0;
+    return fail(e, err, errlen);
+  }
+  if (hifr_solver_set_state(s, u.data()) != 0) {
+    s->time = t0;
+    return fail("hifr_upload_state failed", err, errlen);
+  }
+  return 0;
+}
+
+double hifr_solver_time(const hifr_solver* s) { return s->time; }
 
 long hifr_solver_interfaces(const hifr_solver* s, int* left, int* right) {
   const long n = (long)s->ii.left.size();
@@ -388,8 +424,12 @@ int hifr_solver_write_restart(hifr_solver* s, const char* path) {
   eles& e = *s->el;
   const int nu = e.n_upts_per_ele, N = e.n_eles, nf = e.n_fields;
   std::vector<double> u((size_t)nu * N * nf);
-  if (!e.ctx) return 1;
-  if (hifr_download(e.ctx, HIFR_DISU_UPTS, u.data()) != HIFR_OK) return 1;
+  if (!e.ctx) {
+    if (s->u_host.size() != u.size()) return 1;
+    u = s->u_host;
+  } else if (hifr_download(e.ctx, HIFR_DISU_UPTS, u.data()) != HIFR_OK) {
+    return 1;
+  }
   std::ofstream f(path);
   if (!f) return 1;
   f.precision(15);
@@ -406,8 +446,9 @@ int hifr_solver_write_restart(hifr_solver* s, const char* path) {
     f << "Number of solution points per " << (e.ele_type == 4 ? "hexahedral" : "quadrilateral") << " element\n"
       << nu << "\nLocation of solution points in 1D\n";
     for (double x : e.loc_1d_upts) f << x << " ";
+    f << "\n";
   }
-  f << "\nn_eles\n" << N << "\nele2global_ele array\n";
+  f << "n_eles\n" << N << "\nele2global_ele array\n";
   for (int i = 0; i < N; i++) f << i << " ";
   f << "\ndata\n";
   for (int i = 0; i < N; i++) {

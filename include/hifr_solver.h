@@ -47,7 +47,8 @@ void hifr_solver_destroy(hifr_solver* s);
 /* n_upts, n_fpts per element, n_fields, n_dims, n_eles, n_inters */
 void hifr_solver_sizes(const hifr_solver* s, int* out6);
 
-/* Host arrays (reference layouts):
+/* Host arrays (reference layouts); a host-only solver (deck key hifr_host_only 1)
+ * has no device context and returns the state it holds for 0, and -1 for 1-3, 14:
  *  0 disu_upts (Nu,N,nf)  1 div_tconf_upts  2 disu_fpts (Nfp,N,nf)  3 grad_disu_fpts (Nfp,N,nf,d)
  *  4 detjac_upts (Nu,N)   5 pos_upts (Nu,N,d)  6 opp_0  7 opp_3  8 opp_4
  *  9 JGinv_upts (d,d,Nu,N) 10 tdA_fpts (Nfp,N) 11 norm_fpts (Nfp,N,d) 12 xv (n_verts,d) 13 c2v (as doubles)
@@ -62,8 +63,14 @@ int hifr_solver_residual(hifr_solver* s);
 int hifr_solver_advance(hifr_solver* s, int stage);
 int hifr_solver_run(hifr_solver* s, int n_steps);
 int hifr_solver_norms(hifr_solver* s, int norm_type, double* out);
-/* Writes Rest_<iter>_p0000.dat in the reference's restart format (src/output.cpp:1057-1107). */
+/* Writes the state in the reference's restart format (src/output.cpp:1057-1107;
+ * the reference names the file Rest_<iter>_p0000.dat), 15 significant digits. */
 int hifr_solver_write_restart(hifr_solver* s, const char* path);
+/* Reads a restart file in the reference's format (src/solver.cpp:390-444,
+ * src/eles.cpp:623-716) into the state; its solution points must be the solver's.
+ * Sets the solution time from the file. */
+int hifr_solver_read_restart(hifr_solver* s, const char* path, char* err, int errlen);
+double hifr_solver_time(const hifr_solver* s);
 hifr_ctx* hifr_solver_ctx(hifr_solver* s);
 double hifr_solver_dt(const hifr_solver* s);
 int hifr_solver_adv_type(const hifr_solver* s);

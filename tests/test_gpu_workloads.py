@@ -118,9 +118,10 @@ def test_tgv_32cubed_against_oracle(require_gpu, case):
     if case == "per_element":
         dm = s.device_metrics()
         # the context's per-element values are the reference's at each element's
-        # first point (to rounding)
-        assert cases.field_rel_err(dm["JGinv_upts"].reshape(9, -1).T[:, None, :],
-                                   o.get("JGinv_upts").reshape(9, -1).T[:, None, :]).max() < 1e-13
+        # first point, to rounding of the metric's scale (the box mesh's zero
+        # entries are ~1e-17 at some points of the reference's per-point values)
+        jd, jo = dm["JGinv_upts"], o.get("JGinv_upts")
+        assert np.abs(jd - jo).max() < 1e-13 * np.abs(jo).max()
         o.set_metrics(dm["detjac_upts"], dm["JGinv_upts"], dm["tdA_fpts"], dm["norm_fpts"])
     u = _tgv_state(o.get("pos_upts"))
     s.set_state(u)
