@@ -5,7 +5,10 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <cctype>
 #include <fstream>
+#include <memory>
 #include <sstream>
 #include <unordered_map>
 
@@ -114,6 +117,218 @@ std::string read_gambit(const std::string& path, mesh& m) {
   return "";
 }
 
+namespace {
+
+struct FaceKey {
+  std::array<int, 4> v;
+  bool operator==(const FaceKey& o) const { return v == o.v; }
+};
+struct FaceKeyHash {
+  size_t operator()(const FaceKey& k) const {
+    size_t h = 1469598103934665603ull;
+    for (int x : k.v) h = (h ^ (size_t)(unsigned)x) * 1099511628211ull;
+    return h;
+  }
+};
+
+}  // namespace
+
+namespace {
+
+// Gmsh node order of a linear element -> internal shape-node position
+// (src/geometry.cpp:1987-2050: quads and hexes swap the last two nodes of each
+// quadrilateral, tets keep theirs)
+const int kGmshQuadPos[4] = {0, 1, 3, 2};
+const int kGmshHexPos[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+// nodes of a linear Gmsh element type (2.x numbering), 0 if not read here
+int gmsh_nodes(int etype) {
+  switch (etype) {
+    case 1: return 2;  // line
+    case 2: return 3;  // triangle
+    case 3: return 4;  // quadrangle
+    case 4: return 4;  // tetrahedron
+    case 5: return 8;  // hexahedron
+    default: return 0;
+  }
+}
+
+// Physical name of a $PhysicalNames line as the reference reads it: the first
+// word after the dimension and tag, quotes removed
+// (src/geometry.cpp:1340-1346, 1377-1381: at most 14 characters)
+std::string gmsh_name(const std::string& word) {
+  std::string n = word;
+  if (!n.empty() && n[0] == '"') n.erase(0, 1);
+  n = n.substr(0, 14);
+  while (!n.empty() && (n.back() == '"' || std::isspace((unsigned char)n.back()))) n.pop_back();
+  return n;
+}
+
+}  // namespace
+
+std::string read_gmsh(const std::string& path, mesh& m) {
+  std::ifstream f(path);
+  if (!f) return "Unable to open mesh file " + path;
+  std::string line;
+  auto seek = [&](const char* tag) {
+    f.clear();
+    f.seekg(0);
+    while (std::getline(f, line))
+      if (line.find(tag) != std::string::npos) return true;
+    return false;
+  };
+  if (seek("$MeshFormat")) {
+    double version = 0;
+    int file_type = -1, dsize = 0;
+    f >> version >> file_type >> dsize;
+    if (!f || version >= 3.0 || file_type != 0) return "Gmsh: only the ASCII 2.x mesh format is read";
+  }
+  if (!seek("$PhysicalNames")) return "$PhysicalNames tag not found!";
+  int nphys = 0;
+  f >> nphys;
+  std::getline(f, line);
+  std::unordered_map<int, std::string> names;
+  std::unordered_map<int, bool> fluid;
+  int nd = 0;
+  for (int i = 0; i < nphys; i++) {
+    std::getline(f, line);
+    std::istringstream ss(line);
+    int dim = 0, tag = 0;
+    std::string word;
+    ss >> dim >> tag >> word;
+    if (!ss) return "error reading $PhysicalNames";
+    // the reference takes every name that contains FLUID as the domain
+    // (strstr, src/geometry.cpp:1918-1921) and its dimension as the mesh's
+    fluid[tag] = word.find("FLUID") != std::string::npos;
+    names[tag] = gmsh_name(word);
+    if (fluid[tag]) nd = dim;
+  }
+  if (nd != 2 && nd != 3) return "Invalid mesh dimensionality. Expected 2D or 3D.";
+  if (!seek("$Nodes")) return "$Nodes tag not found!";
+  int nn = 0;
+  f >> nn;
+  if (!f || nn <= 0) return "error reading $Nodes";
+  // vertex index = rank of the node tag (tag - 1 for tags 1..nn, as the reference
+  // indexes them), whatever order the lines list them in
+  std::vector<std::pair<long, std::array<double, 3>>> nodes((size_t)nn);
+  for (int i = 0; i < nn; i++) {
+    f >> nodes[i].first >> nodes[i].second[0] >> nodes[i].second[1] >> nodes[i].second[2];
+    if (!f) return "error reading $Nodes";
+  }
+  std::sort(nodes.begin(), nodes.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  std::unordered_map<long, int> node_index;
+  node_index.reserve(nn);
+  m.n_dims = nd;
+  m.n_verts = nn;
+  m.xv.assign((size_t)nn * nd, 0.0);
+  for (int i = 0; i < nn; i++) {
+    if (i > 0 && nodes[i].first == nodes[i - 1].first) return "Gmsh node tag listed twice";
+    node_index[nodes[i].first] = i;
+    for (int k = 0; k < nd; k++) m.xv[(size_t)i * nd + k] = nodes[i].second[k];
+  }
+  if (!seek("$Elements")) return "$Elements tag not found!";
+  int nent = 0;
+  f >> nent;
+  if (!f || nent <= 0) return "error reading $Elements";
+  m.ele_type = 0;
+  m.n_eles = 0;
+  m.c2v.clear();
+  struct BFace {
+    std::array<int, 4> v;
+    int type;
+  };
+  std::vector<BFace> bfaces;
+  for (int i = 0; i < nent; i++) {
+    long id;
+    int etype = 0, ntags = 0;
+    f >> id >> etype >> ntags;
+    if (!f || ntags < 1) return "error reading $Elements (each element needs its physical tag)";
+    int phys = 0, dummy;
+    f >> phys;
+    for (int t = 1; t < ntags; t++) f >> dummy;
+    const int nv = gmsh_nodes(etype);
+    if (nv == 0) return "Gmsh element type " + std::to_string(etype) + " is not read (linear elements only)";
+    int v[8];
+    for (int q = 0; q < nv; q++) {
+      long nid;
+      f >> nid;
+      auto it = node_index.find(nid);
+      if (!f || it == node_index.end()) return "Gmsh element " + std::to_string(id) + " names an unknown node";
+      v[q] = it->second;
+    }
+    if (!names.count(phys)) return "Gmsh element " + std::to_string(id) + " has no physical name";
+    if (fluid[phys]) {
+      const int et = etype == 3 ? 1 : (etype == 5 ? 4 : (etype == 4 ? 2 : -1));
+      if (et < 0) return "only linear quads, linear tets and 8-node hexes are supported";
+      if (m.ele_type == 0) {
+        m.ele_type = et;
+        m.n_spts = nv;
+      } else if (m.ele_type != et) {
+        return "mixed element types are not supported";
+      }
+      const int* pos = et == 4 ? kGmshHexPos : (et == 1 ? kGmshQuadPos : kTetPos);
+      const size_t base = m.c2v.size();
+      m.c2v.resize(base + nv);
+      for (int q = 0; q < nv; q++) m.c2v[base + pos[q]] = v[q];
+      m.n_eles++;
+    } else {
+      const int type = bc_number(names[phys]);
+      if (type < 0) return "boundary condition '" + names[phys] + "' not recognized";
+      if (type == 9) continue;  // cyclic: matched by translation
+      if (nv > 4) return "Gmsh boundary element type " + std::to_string(etype) + " not recognized";
+      BFace b;
+      b.v = {-1, -1, -1, -1};
+      for (int q = 0; q < nv; q++) b.v[q] = v[q];
+      std::sort(b.v.begin(), b.v.begin() + nv);
+      b.type = type;
+      bfaces.push_back(b);
+    }
+  }
+  if (m.n_eles == 0) return "the mesh has no FLUID elements";
+  // boundary faces -> (element, local face) by their vertex sets
+  // (read_boundary_gmsh, src/geometry.cpp:1480-1530)
+  std::unique_ptr<eles> shape;
+  if (m.ele_type == 4) shape.reset(new eles_hexas());
+  else if (m.ele_type == 1) shape.reset(new eles_quads());
+  else shape.reset(new eles_tets());
+  const int nfaces = m.ele_type == 4 ? 6 : (m.ele_type == 1 ? 4 : 4);
+  std::unordered_map<FaceKey, long, FaceKeyHash> where;
+  where.reserve((size_t)m.n_eles * nfaces);
+  for (int e = 0; e < m.n_eles; e++)
+    for (int lf = 0; lf < nfaces; lf++) {
+      int nv;
+      const int* fn = shape->face_nodes(lf, nv);
+      FaceKey k;
+      k.v = {-1, -1, -1, -1};
+      for (int q = 0; q < nv; q++) k.v[q] = m.c2v[(size_t)e * m.n_spts + fn[q]];
+      std::sort(k.v.begin(), k.v.begin() + nv);
+      where.emplace(k, (long)e * nfaces + lf);
+    }
+  m.bf_ele.clear();
+  m.bf_face.clear();
+  m.bf_type.clear();
+  for (const BFace& b : bfaces) {
+    FaceKey k;
+    k.v = b.v;
+    auto it = where.find(k);
+    if (it == where.end()) return "a Gmsh boundary face matches no element face";
+    m.bf_ele.push_back((int)(it->second / nfaces));
+    m.bf_face.push_back((int)(it->second % nfaces));
+    m.bf_type.push_back(b.type);
+  }
+  return "";
+}
+
+std::string read_mesh(const std::string& path, mesh& m) {
+  auto ends = [&](const char* ext) {
+    const size_t n = std::strlen(ext);
+    return path.size() >= n && path.compare(path.size() - n, n, ext) == 0;
+  };
+  if (ends("neu")) return read_gambit(path, m);
+  if (ends("msh")) return read_gmsh(path, m);
+  return "Mesh format not recognized";
+}
+
 void make_box(int ele_type, const int* nxyz, const double* l, const double* o, int digits, mesh& m) {
   const int nd = ele_type == 4 ? 3 : 2;
   m.n_dims = nd;
@@ -164,22 +379,6 @@ void make_box(int ele_type, const int* nxyz, const double* l, const double* o, i
         }
       }
 }
-
-namespace {
-
-struct FaceKey {
-  std::array<int, 4> v;
-  bool operator==(const FaceKey& o) const { return v == o.v; }
-};
-struct FaceKeyHash {
-  size_t operator()(const FaceKey& k) const {
-    size_t h = 1469598103934665603ull;
-    for (int x : k.v) h = (h ^ (size_t)(unsigned)x) * 1099511628211ull;
-    return h;
-  }
-};
-
-}  // namespace
 
 std::string build_interfaces(const eles& el, const mesh& m, const double* delta, int_inters& out) {
   const int N = el.n_eles, nfaces = el.n_inters_per_ele, nd = el.n_dims, nfp = el.n_fpts_per_ele,

@@ -27,6 +27,18 @@ int bc_number(std::string name);
 // groups other than "cyclic" (matched geometrically) become boundary faces.
 std::string read_gambit(const std::string& path, mesh& m);
 
+// Gmsh 2.x ASCII reader (src/geometry.cpp:1309-1530, 1594-1640, 1881-2088): the
+// elements of the physical group(s) whose name contains FLUID (one linear element
+// type: 4-node quads, 8-node hexes or 4-node tets; node order permuted as the
+// reference does); elements of the other physical groups are boundary faces, given
+// the bctype of their name and matched to element faces by their vertices, except
+// "cyclic" faces, which are matched by translation.
+std::string read_gmsh(const std::string& path, mesh& m);
+
+// The reader the mesh file's extension names: .neu Gambit, .msh Gmsh
+// (src/input.cpp:355-360).
+std::string read_mesh(const std::string& path, mesh& m);
+
 // Structured box of nx*ny(*nz) elements on [o, o+l], optionally with vertex
 // coordinates rounded to `digits` significant digits (0 = exact).
 void make_box(int ele_type, const int* nxyz, const double* l, const double* o, int digits, mesh& m);

@@ -244,7 +244,7 @@ int hifr_solver_from_deck(const char* path, hifr_solver** out, char* err, int er
   const std::string dp(path);
   const size_t slash = dp.find_last_of('/');
   if (!mpath.empty() && mpath[0] != '/' && slash != std::string::npos) mpath = dp.substr(0, slash + 1) + mpath;
-  e = read_gambit(mpath, s->msh);
+  e = read_mesh(mpath, s->msh);
   if (!e.empty()) return fail(e, err, errlen);
   e = setup_solver(s.get());
   if (!e.empty()) return fail(e, err, errlen);

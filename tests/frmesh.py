@@ -118,6 +118,49 @@ def write_gambit(path, xv, conn_gambit, ele_type, bc_groups=()):
         fh.write("\n".join(lines) + "\n")
 
 
+# internal shape-node position of each Gmsh node of a linear element (the reader's
+# permutation, src/geometry.cpp:1987-2050)
+GMSH_TO_INTERNAL = {4: [0, 1, 3, 2, 4, 5, 7, 6], 1: [0, 1, 3, 2], 2: [0, 1, 2, 3]}
+
+
+def write_gmsh(path, xv, c2v_internal, ele_type, bc_groups=(), node_order=None):
+    """Gmsh 2.2 ASCII file as read by src/geometry.cpp:1309-1530, 1594-1640,
+    1881-2088: a FLUID physical group of the elements and one physical group per
+    boundary condition.  bc_groups: iterable of (name, [(element, internal face)]).
+    node_order: optional permutation in which the $Nodes lines are listed (the ids
+    stay index + 1)."""
+    nd = xv.shape[1]
+    pos = GMSH_TO_INTERNAL[ele_type]
+    fn = face_nodes_of(ele_type)
+    cell = {4: 5, 1: 3, 2: 4}[ele_type]
+    face = {4: 3, 1: 1, 2: 2}[ele_type]
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", "%d" % (1 + len(bc_groups))]
+    lines.append('%d %d "FLUID"' % (nd, 1))
+    for g, (name, _) in enumerate(bc_groups):
+        lines.append('%d %d "%s"' % (nd - 1, g + 2, name))
+    lines += ["$EndPhysicalNames", "$Nodes", "%d" % xv.shape[0]]
+    order = range(xv.shape[0]) if node_order is None else node_order
+    for i in order:
+        p = list(xv[i]) + [0.0] * (3 - nd)
+        lines.append("%d %.17g %.17g %.17g" % (i + 1, p[0], p[1], p[2]))
+    lines += ["$EndNodes", "$Elements"]
+    recs = []
+    for g, (name, faces) in enumerate(bc_groups):
+        for e, f in faces:
+            verts = [int(c2v_internal[e][q]) for q in fn[f]]
+            if ele_type == 4:  # quadrilateral face: around its perimeter
+                verts = [verts[0], verts[1], verts[3], verts[2]]
+            recs.append((face, g + 2, verts))
+    for e in range(len(c2v_internal)):
+        recs.append((cell, 1, [int(c2v_internal[e][pos[q]]) for q in range(len(pos))]))
+    lines.append("%d" % len(recs))
+    for k, (t, phys, verts) in enumerate(recs):
+        lines.append("%d %d 2 %d %d %s" % (k + 1, t, phys, phys, " ".join("%d" % (v + 1) for v in verts)))
+    lines.append("$EndElements")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
 # Internal local faces as shape-node sets (reference coordinates r,s,t):
 # hex 0:t=-1 1:s=-1 2:r=+1 3:s=+1 4:r=-1 5:t=+1 (src/eles_hexas.cpp:266-324);
 # quad 0:s=-1 1:r=+1 2:s=+1 3:r=-1 (src/eles_quads.cpp:259-300).

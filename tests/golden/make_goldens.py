@@ -47,7 +47,7 @@ def deck_text(c, restart_iter=0):
         "dt %.17g" % c["dt"], "n_steps %d" % c["n_steps"], "adv_type %d" % c["adv_type"],
         "tau %.17g" % c.get("tau", 0.0), "pen_fact %.17g" % c.get("pen_fact", 0.5),
         "restart_flag 1", "restart_iter %d" % restart_iter, "n_restart_files 1",
-        "mesh_file mesh.neu",
+        "mesh_file mesh.%s" % c.get("mesh", "neu"),
         "dx_cyclic %.17g" % min(c["cyc"][0], 1e30), "dy_cyclic %.17g" % min(c["cyc"][1], 1e30),
         "dz_cyclic %.17g" % min(c["cyc"][2], 1e30),
         "p_res 2", "write_type 0", "inters_cub_order 3", "volume_cub_order 3",
@@ -145,7 +145,14 @@ def run_reference(c, xv, conn_g, u0, loc1d, loc_upts=None):
         if walls:
             groups.append(("isotherm_fix", [(e, frmesh.internal_face_to_gambit(c["ele_type"], f))
                                             for e, f in sorted(walls)]))
-        frmesh.write_gambit(os.path.join(wd, "mesh.neu"), xv, conn_g, c["ele_type"], bc_groups=groups)
+        if c.get("mesh") == "msh":
+            ggroups = [("CYCLIC", [(e, f) for e, f in frmesh.boundary_faces(c["ele_type"], c2v)
+                                   if (e, f) not in walls])]
+            if walls:
+                ggroups.append(("isotherm_fix", sorted(walls)))
+            frmesh.write_gmsh(os.path.join(wd, "mesh.msh"), xv, c2v, c["ele_type"], bc_groups=ggroups)
+        else:
+            frmesh.write_gambit(os.path.join(wd, "mesh.neu"), xv, conn_g, c["ele_type"], bc_groups=groups)
         with open(os.path.join(wd, "input"), "w") as fh:
             fh.write(deck_text(c))
         if c.get("restart_writer") == "hifr":
@@ -241,6 +248,16 @@ CASES = [
          restart_writer="hifr"),
     dict(name="quad_p3_euler_restart_3x3", ele_type=1, n=(3, 3), order=3, vcjh=1, viscous=0,
          state="vortex", noise=1e-2, seed=83, perturb=0.05, kind="rk45", steps=2, dt=1e-3, restart_writer="hifr"),
+    # Gmsh 2.2 meshes (mesh_file *.msh): the reference reads the same elements and
+    # boundary groups through its Gmsh reader
+    dict(name="hex_p3_ns_channel_gmsh_3x2x2", ele_type=4, n=(3, 2, 2), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=91, perturb=0.1, kind="residual", channel=True,
+         mach=0.2, Re=2800.0, mesh="msh"),
+    dict(name="tet_p2_ns_channel_gmsh_2x2x1", ele_type=2, n=(2, 2, 1), order=2, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="channel", noise=5e-2, seed=92, perturb=0.1, kind="residual", channel=True,
+         mach=0.2, Re=2800.0, mesh="msh"),
+    dict(name="quad_p3_ns_gmsh_perturbed_4x3", ele_type=1, n=(4, 3), order=3, vcjh=2, viscous=1, tau=1.0,
+         pen_fact=0.5, state="vortex", noise=1e-2, seed=93, perturb=0.1, kind="residual", mesh="msh"),
     # Roe flux with entropy fix (riemann_solve_type 2, src/inters.cpp:390-520); the
     # reference evaluates it in 2-D only, so these are quads
     dict(name="quad_p3_euler_roe_perturbed_4x3", ele_type=1, n=(4, 3), order=3, vcjh=1, viscous=0,
@@ -342,6 +359,8 @@ def main():
                    riemann=int(c.get("riemann", 0)))
         if c.get("restart_writer"):
             rec["rest_text"] = np.array(c["rest_text"])
+        if c.get("mesh"):
+            rec["mesh_format"] = np.array(c["mesh"])
         if c["kind"] == "residual":
             R = (u0 - u1) / c["dt"]
             rec["R"] = R

@@ -167,3 +167,24 @@ def test_entry_points_fail_loudly(require_gpu):
     cfg = (C.c_byte * 256)()
     del cfg
     assert L.hifr_residual(None) == 1  # HIFR_ERR_ARG on a null context
+
+
+@pytest.mark.parametrize("name", [n for n in cases.golden_names()
+                                  if str(cases.load_golden(n).get("mesh_format", "neu")) == "msh"])
+def test_gmsh_deck_matches_reference(require_gpu, tmp_path, name):
+    """A deck naming a Gmsh file (mesh_file mesh.msh, read by the library's Gmsh
+    reader) reproduces the residual the reference computed from the same file."""
+    g = cases.load_golden(name)
+    ele_type, xv, c2v = cases.mesh_of(g)
+    bf = cases.bfaces_of(g)
+    walls = [] if bf is None else [(int(e), int(f)) for e, f, _ in bf]
+    groups = [("CYCLIC", [(e, f) for e, f in frmesh.boundary_faces(ele_type, c2v) if (e, f) not in walls])]
+    if walls:
+        groups.append(("isotherm_fix", walls))
+    frmesh.write_gmsh(str(tmp_path / "mesh.msh"), xv, c2v, ele_type, bc_groups=groups)
+    (tmp_path / "input").write_text(cases.product_deck(g, mesh_file="mesh.msh"))
+    print("case", name, flush=True)
+    s = hifr.Solver.from_deck(str(tmp_path / "input"))
+    s.set_state(g["u0"])
+    R = s.residual() / s.get("detjac_upts")[:, :, None]
+    assert cases.field_rel_err(R, g["R"]).max() < TOL
