@@ -64,28 +64,30 @@ def stable_dt(n, order, mach=0.1):
     return 0.6 * h / ((order + 1) ** 2 * (1.0 + 1.0 / mach))
 
 
-def run(solver, u0, dt, t_end, every, on_sample=None, log=None):
-    """Steps to t_end; KE every `every` steps (and on_sample(solver, t) there)."""
+def run(solver, u0, dt, t_end, every, on_sample=None, log=None, sample_steps=()):
+    """Steps to t_end; KE every `every` steps and at each step of sample_steps
+    (and on_sample(solver, t, is_sample_step) at those stops)."""
     solver.set_state(u0)
     detj = solver.get("detjac_upts")
     order = int(round(solver.nu ** (1 / 3))) - 1
     steps = int(round(t_end / dt))
+    samples = {int(k) for k in sample_steps if 0 < int(k) <= steps}
+    stops = sorted(set(range(every, steps + 1, every)) | {steps} | samples)
     ts, kes = [0.0], [kinetic_energy(u0, detj, order)]
     if on_sample:
-        on_sample(solver, 0.0)
+        on_sample(solver, 0.0, False)
     t0 = time.perf_counter()
     done = 0
-    while done < steps:
-        k = min(every, steps - done)
-        solver.run(k)
-        done += k
+    for stop in stops:
+        solver.run(stop - done)
+        done = stop
         u = solver.get("disu_upts")
         if not np.all(np.isfinite(u)):
             raise RuntimeError("non-finite state at t = %g" % (done * dt))
         ts.append(done * dt)
         kes.append(kinetic_energy(u, detj, order))
         if on_sample:
-            on_sample(solver, done * dt)
+            on_sample(solver, done * dt, done in samples)
         if log:
             log("  t = %.3f  KE = %.6f  (%.1f s)" % (done * dt, kes[-1], time.perf_counter() - t0))
     return np.array(ts), np.array(kes), (time.perf_counter() - t0) / max(steps, 1)
@@ -93,7 +95,7 @@ def run(solver, u0, dt, t_end, every, on_sample=None, log=None):
 
 def generate(n_hi, p_hi, re, t_end, every, p_lo=2, sample_times=(), log=None):
     """High-order run: KE history of the filtered field (degree p_lo) and training
-    samples at the sample times."""
+    samples at the steps nearest to the sample times (one snapshot per time)."""
     dt = stable_dt(n_hi, p_hi)
     s = hifr.Solver.box(tgv_deck(p_hi, re, dt), hifr.HEX, (n_hi,) * 3, (TWO_PI,) * 3, (0.0,) * 3)
     u0 = tgv_state(s.get("pos_upts"))
@@ -101,17 +103,17 @@ def generate(n_hi, p_hi, re, t_end, every, p_lo=2, sample_times=(), log=None):
     detj_lo = np.full(((p_lo + 1) ** 3, s.n_eles), (h / 2) ** 3)
     ke_f, X, C, CW, tagged = [], [], [], [], []
 
-    def on_sample(solver, t):
-        want = any(abs(t - ts) < 0.5 * dt * every for ts in sample_times)
+    def on_sample(solver, t, is_sample):
         x, ce, cw, ulo = solver.sgs_samples(p_lo, 1.0, want_state=True)
         ke_f.append(kinetic_energy(ulo, detj_lo, p_lo))
-        if want:
+        if is_sample:
             X.append(x)
             C.append(ce)
             CW.append(cw)
             tagged.append(t)
 
-    ts, ke, ms = run(s, u0, dt, t_end, every, on_sample, log)
+    sample_steps = [int(round(ts / dt)) for ts in sample_times]
+    ts, ke, ms = run(s, u0, dt, t_end, every, on_sample, log, sample_steps)
     return {"t": ts, "ke": ke, "ke_filtered": np.array(ke_f), "dt": dt, "ms_per_step": ms * 1e3,
             "samples": (X, C, CW, tagged)}
 
