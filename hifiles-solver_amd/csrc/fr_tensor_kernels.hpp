@@ -71,6 +71,9 @@ struct StageArgsT {
   const R* fv_in;        // viscous normal flux traces (RESID/ADVANCE, viscous)
   R* fv_out;             // viscous normal flux traces written (TRACE_GRAD)
   R* gf_out;             // optional gradient traces (TRACE_GRAD, may be null)
+  double* gu_out;        // optional corrected physical gradient at the solution points,
+                         // (Nu,N,nf,d) = the reference's grad_disu_upts (TRACE_GRAD,
+                         // quads/hexes, may be null)
   const double* detj;    // per point, or per element (AFF)
   const R* jg;           // per point, or per element (AFF)
   const int* nbr;        // neighbour trace index per element flux point; bit 31: LDG switch -;
@@ -583,6 +586,13 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
             gcol(m, gc);
 #pragma unroll
             for (int k = 0; k < NF; k++) gx[k][m] = gc[k];
+          }
+          if (a.gu_out) {
+            // (a uniform branch: the integral diagnostics ask for the gradient)
+#pragma unroll
+            for (int k = 0; k < NF; k++)
+#pragma unroll
+              for (int m = 0; m < ND; m++) a.gu_out[pe + (long)NU * N * (k + NF * m)] = double(gx[k][m]);
           }
           if constexpr (LES != 0) {
             // transformed SGS flux sgsf(k,l) = sum_m JGinv(l,m) F_sgs(k,m), k >= 1

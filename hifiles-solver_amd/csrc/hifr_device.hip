@@ -43,6 +43,10 @@ struct hifr_ctx {
   int mlp_on;
   int* nbr;
   double* red;  // reduction scratch
+  // integral diagnostics (hifr_set_cubature): corrected gradient at the solution
+  // points, volume cubature operator / weights / detJ, per-element partial sums
+  double *gu, *cub_opp, *cub_w, *cub_detj, *cub_part;
+  int ncub;
   int cur;      // which uf buffer holds the traces of the current state
   // host copies needed to build the interface tables
   std::vector<double> h_tdA, h_norm;
@@ -144,7 +148,7 @@ StageArgsT<R> base_args(hifr_ctx* c) {
 // Buffers of one stage launch, filled by the callers below.
 struct StageIO {
   int uf_in = -1, uf_out = -1;  // trace buffer indices
-  bool fv_in = false, fv_out = false, gf_out = false, div = false;
+  bool fv_in = false, fv_out = false, gf_out = false, gu_out = false, div = false;
   int adv_type = 0;
   double rk_a = 0, rk_b = 0, dt = 0;
 };
@@ -181,6 +185,7 @@ hifr_status launch_io(hifr_ctx* c, int mode, const StageIO& io) {
   a.fv_in = io.fv_in ? (const R*)c->fv : nullptr;
   a.fv_out = io.fv_out ? (R*)c->fv : nullptr;
   a.gf_out = io.gf_out ? (R*)c->gf : nullptr;
+  a.gu_out = io.gu_out ? c->gu : nullptr;
   a.div = io.div ? c->div : nullptr;
   a.adv_type = io.adv_type;
   a.rk_a = io.rk_a;
@@ -343,6 +348,8 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   for (auto& al : allocs) *al.p = nullptr;
   c->detj = c->fgeo = nullptr;  // sized by hifr_set_metrics / hifr_set_interfaces
   c->jg = nullptr;
+  c->gu = c->cub_opp = c->cub_w = c->cub_detj = c->cub_part = nullptr;  // hifr_set_cubature
+  c->ncub = 0;
   c->tI = c->tG = c->tN = c->tD = nullptr;
   c->mlp_buf = nullptr;
   std::memset(&c->mlpw, 0, sizeof(c->mlpw));
@@ -373,8 +380,9 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
 
 hifr_status hifr_destroy(hifr_ctx* c) {
   if (!c) return HIFR_OK;
-  void* ps[] = {c->u,    c->res, c->div, c->uf[0], c->uf[1], c->fv, c->gf,     c->detj, c->jg,
-                c->fgeo, c->nbr, c->red, c->tI,    c->tG,    c->tN, c->tD,     c->mlp_buf};
+  void* ps[] = {c->u,    c->res, c->div, c->uf[0], c->uf[1], c->fv,      c->gf,    c->detj,   c->jg,
+                c->fgeo, c->nbr, c->red, c->tI,    c->tG,    c->tN,      c->tD,    c->mlp_buf, c->gu,
+                c->cub_opp, c->cub_w, c->cub_detj, c->cub_part};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -1021,6 +1029,193 @@ extern "C" hifr_status hifr_norm_residual(hifr_ctx* c, int norm_type, double* ou
     out[k] = v;
   }
   return nan ? HIFR_ERR_NAN : HIFR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// integral diagnostics (CalcIntegralQuantities, src/eles.cpp:6503-6643)
+
+namespace {
+
+constexpr int kMaxIntegrals = 5;
+constexpr int kIntegralBlock = 256;
+
+struct IntegralKinds {
+  int n;
+  int kind[kMaxIntegrals];
+};
+
+// One workgroup per element: the element's state and corrected gradient staged in
+// LDS, one thread per cubature point (looping when there are more points than
+// threads), the integrands weighted by w_j detJ_j and summed over the workgroup.
+template <int ND>
+__global__ void __launch_bounds__(kIntegralBlock)
+integral_partial_kernel(int N, int nu, int ncub, const double* __restrict__ u, const double* __restrict__ gu,
+                        const double* __restrict__ opp, const double* __restrict__ w,
+                        const double* __restrict__ detj, IntegralKinds q, double gamma,
+                        double* __restrict__ part) {
+  constexpr int NF = ND + 2;
+  extern __shared__ double sm[];
+  double* s_u = sm;                 // [k][p]
+  double* s_g = sm + NF * nu;       // [k][m][p]
+  __shared__ double red[kMaxIntegrals][kIntegralBlock];
+  const int e = blockIdx.x, tid = threadIdx.x;
+  for (int i = tid; i < NF * nu; i += kIntegralBlock) {
+    const int k = i / nu, p = i - k * nu;
+    s_u[i] = u[p + (long)nu * (e + (long)N * k)];
+  }
+  for (int i = tid; i < NF * ND * nu; i += kIntegralBlock) {
+    const int km = i / nu, p = i - km * nu;  // km = k * ND + m
+    const int k = km / ND, m = km - k * ND;
+    s_g[i] = gu[p + (long)nu * (e + (long)N * (k + NF * m))];
+  }
+  __syncthreads();
+  double acc[kMaxIntegrals];
+  for (int i = 0; i < kMaxIntegrals; i++) acc[i] = 0.0;
+  for (int j = tid; j < ncub; j += kIntegralBlock) {
+    double uc[NF], g[NF][ND];
+    for (int k = 0; k < NF; k++) {
+      uc[k] = 0.0;
+      for (int m = 0; m < ND; m++) g[k][m] = 0.0;
+    }
+    for (int p = 0; p < nu; p++) {
+      const double o = opp[j + (long)ncub * p];
+      for (int k = 0; k < NF; k++) {
+        uc[k] += o * s_u[k * nu + p];
+        for (int m = 0; m < ND; m++) g[k][m] += o * s_g[(k * ND + m) * nu + p];
+      }
+    }
+    const double irho = 1.0 / uc[0];
+    // velocity gradient dv_i/dx_m = (d(rho v_i)/dx_m - v_i drho/dx_m) / rho
+    double dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int i = 0; i < ND; i++)
+      for (int m = 0; m < ND; m++) dv[i][m] = irho * (g[1 + i][m] - uc[1 + i] * irho * g[0][m]);
+    double tke = 0.0;
+    for (int k = 1; k < NF - 1; k++) tke += 0.5 * uc[k] * uc[k];
+    const double wt = w[j] * detj[j + (long)ncub * e];
+    for (int iq = 0; iq < q.n; iq++) {
+      double diag = 0.0;
+      switch (q.kind[iq]) {
+        case HIFR_KINETIC_ENERGY:
+          diag = irho * tke;
+          break;
+        case HIFR_VORTICITY: {
+          const double wz = dv[1][0] - dv[0][1];
+          diag = wz * wz;
+          if (ND == 3) {
+            const double wx = dv[2][1] - dv[1][2], wy = dv[0][2] - dv[2][0];
+            diag += wx * wx + wy * wy;
+          }
+          diag *= 0.5 / irho;
+          break;
+        }
+        case HIFR_PRESSURE_DILATATION: {
+          const double pr = (gamma - 1.0) * (uc[NF - 1] - irho * tke);
+          double div = 0.0;
+          for (int i = 0; i < ND; i++) div += dv[i][i];
+          diag = pr * div;
+          break;
+        }
+        default: {  // strain-rate (deviatoric) colon product
+          double S[3][3], tr = 0.0;
+          for (int i = 0; i < ND; i++)
+            for (int m = 0; m < ND; m++) S[i][m] = i == m ? dv[i][i] : (dv[i][m] + dv[m][i]) / 2.0;
+          for (int i = 0; i < ND; i++) tr += S[i][i] / 3.0;
+          if (q.kind[iq] == HIFR_DEV_STRAIN_COLON)
+            for (int i = 0; i < ND; i++) S[i][i] -= tr;
+          for (int i = 0; i < ND; i++)
+            for (int m = 0; m < ND; m++) diag += S[i][m] * S[i][m];
+        }
+      }
+      acc[iq] += diag * wt;
+    }
+  }
+  for (int iq = 0; iq < kMaxIntegrals; iq++) red[iq][tid] = acc[iq];
+  __syncthreads();
+  for (int st = kIntegralBlock / 2; st > 0; st >>= 1) {
+    if (tid < st)
+      for (int iq = 0; iq < kMaxIntegrals; iq++) red[iq][tid] += red[iq][tid + st];
+    __syncthreads();
+  }
+  if (tid < q.n) part[(long)tid * N + e] = red[tid][0];
+}
+
+// sum of the per-element partials of integral iq = blockIdx.x (fixed order)
+__global__ void __launch_bounds__(kIntegralBlock)
+integral_sum_kernel(int N, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double red[kIntegralBlock];
+  double acc = 0.0;
+  for (int e = threadIdx.x; e < N; e += kIntegralBlock) acc += part[(long)blockIdx.x * N + e];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = kIntegralBlock / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+extern "C" hifr_status hifr_set_cubature(hifr_ctx* c, int n_cub, const double* opp, const double* weight,
+                                         const double* detjac) {
+  if (!c || n_cub <= 0 || !opp || !weight || !detjac) return HIFR_ERR_ARG;
+  if (c->cfg.ele_type == HIFR_TET) return HIFR_ERR_UNSUPPORTED;
+  const size_t nu = c->nu, N = c->N, nf = c->nf, nd = c->nd;
+  if ((size_t)(nf + nf * nd) * nu * sizeof(double) > 64 * 1024) return HIFR_ERR_UNSUPPORTED;
+  hifr_status st;
+  if ((st = realloc_dev(&c->cub_opp, sizeof(double) * n_cub * nu)) != HIFR_OK) return st;
+  if ((st = realloc_dev(&c->cub_w, sizeof(double) * n_cub)) != HIFR_OK) return st;
+  if ((st = realloc_dev(&c->cub_detj, sizeof(double) * n_cub * N)) != HIFR_OK) return st;
+  if ((st = realloc_dev(&c->cub_part, sizeof(double) * kMaxIntegrals * N)) != HIFR_OK) return st;
+  if (!c->gu) {
+    if ((st = realloc_dev(&c->gu, sizeof(double) * nu * N * nf * nd)) != HIFR_OK) return st;
+    HIP_TRY(hipMemset(c->gu, 0, sizeof(double) * nu * N * nf * nd));
+  }
+  HIP_TRY(hipMemcpy(c->cub_opp, opp, sizeof(double) * n_cub * nu, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->cub_w, weight, sizeof(double) * n_cub, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->cub_detj, detjac, sizeof(double) * n_cub * N, hipMemcpyHostToDevice));
+  c->ncub = n_cub;
+  return HIFR_OK;
+}
+
+extern "C" hifr_status hifr_integral_quantities(hifr_ctx* c, int n_q, const int* kinds, double* out) {
+  hifr_status s = check_ready(c);
+  if (s != HIFR_OK) return s;
+  if (n_q < 0 || n_q > kMaxIntegrals || (n_q > 0 && (!kinds || !out))) return HIFR_ERR_ARG;
+  if (c->ncub == 0) return HIFR_ERR_STATE;
+  IntegralKinds q;
+  q.n = n_q;
+  bool grad = false;
+  for (int i = 0; i < kMaxIntegrals; i++) q.kind[i] = 0;
+  for (int i = 0; i < n_q; i++) {
+    if (kinds[i] < HIFR_KINETIC_ENERGY || kinds[i] > HIFR_DEV_STRAIN_COLON) return HIFR_ERR_ARG;
+    q.kind[i] = kinds[i];
+    grad = grad || kinds[i] != HIFR_KINETIC_ENERGY;
+  }
+  if (n_q == 0) return HIFR_OK;
+  // the gradient comes from the viscous gradient pass of the current state
+  if (grad && !c->cfg.viscous) return HIFR_ERR_UNSUPPORTED;
+  if (grad) {
+    if ((s = trace_u(c)) != HIFR_OK) return s;
+    StageIO io;
+    io.uf_in = c->cur;
+    io.fv_out = true;
+    io.gu_out = true;
+    if ((s = launch(c, hifr::MODE_TRACE_GRAD, io)) != HIFR_OK) return s;
+  }
+  const size_t lds = sizeof(double) * (c->nf + c->nf * c->nd) * c->nu;
+  if (c->nd == 3)
+    hipLaunchKernelGGL(integral_partial_kernel<3>, dim3(c->N), dim3(kIntegralBlock), lds, c->stream, c->N, c->nu,
+                       c->ncub, c->u, c->gu, c->cub_opp, c->cub_w, c->cub_detj, q, c->cfg.gamma, c->cub_part);
+  else
+    hipLaunchKernelGGL(integral_partial_kernel<2>, dim3(c->N), dim3(kIntegralBlock), lds, c->stream, c->N, c->nu,
+                       c->ncub, c->u, c->gu, c->cub_opp, c->cub_w, c->cub_detj, q, c->cfg.gamma, c->cub_part);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(integral_sum_kernel, dim3(n_q), dim3(kIntegralBlock), 0, c->stream, c->N, c->cub_part, c->red);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, c->red, sizeof(double) * n_q, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return HIFR_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -20,7 +20,8 @@ HEX, QUAD, TET = 4, 1, 2
 
 _GET = {"disu_upts": 0, "div_tconf_upts": 1, "disu_fpts": 2, "grad_disu_fpts": 3, "detjac_upts": 4,
         "pos_upts": 5, "opp_0": 6, "opp_3": 7, "opp_4": 8, "JGinv_upts": 9, "tdA_fpts": 10,
-        "norm_fpts": 11, "xv": 12, "c2v": 13, "rk_register": 14}
+        "norm_fpts": 11, "xv": 12, "c2v": 13, "rk_register": 14, "loc_volume_cubpts": 15,
+        "weight_volume_cubpts": 16, "opp_volume_cubpts": 17, "vol_detjac_cubpts": 18}
 
 
 def lib():
@@ -51,6 +52,8 @@ def lib():
         L.hifr_solver_read_restart.argtypes = [vp, cp, cp, C.c_int]
         L.hifr_solver_time.argtypes = [vp]
         L.hifr_solver_time.restype = C.c_double
+        L.hifr_solver_n_integrals.argtypes = [vp]
+        L.hifr_solver_integrals.argtypes = [vp, dp]
         L.hifr_solver_ctx.argtypes = [vp]
         L.hifr_solver_ctx.restype = vp
         L.hifr_solver_dt.argtypes = [vp]
@@ -226,6 +229,15 @@ class Solver:
 
     def time(self):
         return lib().hifr_solver_time(self._h)
+
+    def integrals(self):
+        """The deck's integral_quantities for the current state."""
+        n = lib().hifr_solver_n_integrals(self._h)
+        out = np.zeros(max(n, 1))
+        rc = lib().hifr_solver_integrals(self._h, _dp(out))
+        if rc != 0:
+            raise RuntimeError("integrals failed (%d)" % rc)
+        return out[:n]
 
     def metrics_per_element(self):
         """True when the device keeps one metric set per element (all elements affine)."""

@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <utility>
 
@@ -148,6 +149,118 @@ std::string eles::setup(const input& in, int n) {
   set_opp_3();
   set_opp_4();
   shape.assign((size_t)n_dims * n_spts_per_ele * n_eles, 0.0);
+  return "";
+}
+
+namespace {
+
+// Gauss-Legendre nodes and weights on [-1, 1] (Newton's iteration on P_n from the
+// Chebyshev guesses; weights 2 / ((1 - x^2) P_n'(x)^2)), ascending nodes
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < n; i++) {
+    double z = -std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; it++) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; k++) {
+        const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      const double dz = p1 / dp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-17) break;
+    }
+    double p0 = 1.0, p1 = z;
+    for (int k = 2; k <= n; k++) {
+      const double p2 = ((2 * k - 1) * z * p1 - (k - 1) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = n * (z * p1 - p0) / (z * z - 1.0);
+    x[i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+  }
+  if (n % 2 == 1) x[n / 2] = 0.0;
+}
+
+// x rounded to `dec` decimals (or `dec` significant digits), as the tables print it
+double round_decimals(double x, int dec) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.*f", dec, x);
+  return std::strtod(buf, nullptr);
+}
+double round_significant(double x, int digits) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%.*g", digits, x);
+  return std::strtod(buf, nullptr);
+}
+
+}  // namespace
+
+bool volume_cubature_rule(int nd, int n, std::vector<double>& loc, std::vector<double>& w) {
+  const int dec = nd == 3 ? 12 : 15;
+  if (n < 2 || n > (nd == 3 ? 12 : 20)) return false;
+  std::vector<double> x, wx;
+  gauss_legendre(n, x, wx);
+  const int ncub = nd == 3 ? n * n * n : n * n;
+  loc.assign((size_t)nd * ncub, 0.0);
+  w.assign(ncub, 0.0);
+  for (int q = 0; q < ncub; q++) {
+    int idx[3];
+    if (nd == 3) {
+      idx[0] = q / (n * n);
+      idx[1] = (q / n) % n;
+      idx[2] = q % n;
+    } else {
+      idx[0] = q % n;
+      idx[1] = q / n;
+    }
+    double prod = 1.0;
+    for (int d = 0; d < nd; d++) {
+      loc[d + (size_t)nd * q] = round_decimals(x[idx[d]], dec);
+      prod *= wx[idx[d]];
+    }
+    w[q] = nd == 3 ? round_decimals(prod, dec) : round_significant(prod, 12);
+  }
+  return true;
+}
+
+std::string eles::set_volume_cubature(int rule) {
+  if (ele_type != 1 && ele_type != 4) return "integral diagnostics are built for quads and hexes";
+  const int nd = n_dims, nu = n_upts_per_ele, N = n_eles, ns = n_spts_per_ele;
+  if (!volume_cubature_rule(nd, rule, loc_volume_cubpts, weight_volume_cubpts))
+    return "cubature rule not implemented.";
+  const int ncub = (int)weight_volume_cubpts.size();
+  n_cubpts_per_ele = ncub;
+  opp_volume_cubpts.assign((size_t)ncub * nu, 0.0);
+  for (int i = 0; i < nu; i++)
+    for (int j = 0; j < ncub; j++)
+      opp_volume_cubpts[j + (size_t)ncub * i] = eval_nodal_basis(i, &loc_volume_cubpts[(size_t)nd * j]);
+  vol_detjac_cubpts.assign((size_t)ncub * N, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; i++) {
+    double Nb[8], dN[24], d[3][3];
+    for (int j = 0; j < ncub; j++) {
+      eval_shape(&loc_volume_cubpts[(size_t)nd * j], Nb, dN);
+      for (int a = 0; a < nd; a++)
+        for (int k = 0; k < nd; k++) {
+          d[a][k] = 0.0;
+          for (int sn = 0; sn < ns; sn++) d[a][k] += dN[sn + ns * k] * shape[a + nd * (sn + (size_t)ns * i)];
+        }
+      double dj;
+      if (nd == 2) {
+        dj = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+      } else {
+        dj = d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) - d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0]) +
+             d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+      }
+      vol_detjac_cubpts[j + (size_t)ncub * i] = dj;
+    }
+  }
   return "";
 }
 

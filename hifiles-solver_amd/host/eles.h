@@ -43,6 +43,17 @@ class eles {
   // local faces as shape-node lists (for connectivity)
   virtual const int* face_nodes(int face, int& n) const = 0;
 
+  // Volume cubature of the integral diagnostics (src/eles_hexas.cpp:418-440,
+  // src/eles_quads.cpp set_volume_cubpts, src/eles.cpp:4111-4130, 5318-5352):
+  // the reference's tabulated rule with `rule` points per direction, the nodal
+  // basis at its points and detJ there.  Quads and hexes.
+  std::string set_volume_cubature(int rule);
+  int n_cubpts_per_ele = 0;
+  std::vector<double> loc_volume_cubpts;     // (d, ncub)
+  std::vector<double> weight_volume_cubpts;  // (ncub)
+  std::vector<double> opp_volume_cubpts;     // (ncub, Nu)
+  std::vector<double> vol_detjac_cubpts;     // (ncub, N)
+
  protected:
   virtual void set_points() = 0;
   virtual double eval_nodal_basis(int idx, const double* loc) const = 0;
@@ -104,5 +115,12 @@ double eval_d_vcjh_1d(double r, int mode, int order, double eta);
 double compute_eta(int scheme, int order);
 // 1-D Gauss points as tabulated by data/loc_1d_gauss_pts.dat (orders 0..6)
 std::vector<double> gauss_points(int order);
+// The reference's tensor-product volume cubature rules with n points per direction
+// (data/cubature_hexa.dat, n = 2..12; data/cubature_quad.dat, n = 2..20): Gauss-
+// Legendre points and products of weights, rounded as the tables print them: hexes
+// 12 decimals; quads points 15 decimals, weights 12 significant digits.  Point
+// order: hexes x slowest, z fastest; quads x fastest.  Returns false for a rule the
+// tables do not hold.
+bool volume_cubature_rule(int nd, int n, std::vector<double>& loc, std::vector<double>& w);
 
 }  // namespace hifr_host

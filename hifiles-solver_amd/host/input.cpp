@@ -4,6 +4,7 @@
 // (mu_inf, rt_inf, ...) are the same doubles as the reference's.
 #include "input.h"
 
+#include <cctype>
 #include <cmath>
 #include <fstream>
 #include <map>
@@ -76,6 +77,20 @@ std::string input::read_input_file(const std::string& path) {
   d.get("restart_dump_freq", restart_dump_freq);
   d.get("monitor_res_freq", monitor_res_freq);
   d.get("res_norm_type", res_norm_type);
+  d.get("volume_cub_order", volume_cub_order);
+  integral_quantities.clear();
+  if (d.has("integral_quantities")) {
+    // "integral_quantities n name_1 .. name_n", names case-insensitive
+    std::istringstream ss(d.kv.at("integral_quantities"));
+    int n = 0;
+    if (!(ss >> n) || n < 0) return "Unable to read number of entries for vector option integral_quantities";
+    for (int i = 0; i < n; i++) {
+      std::string q;
+      if (!(ss >> q)) return "Unable to assign all values to vector option integral_quantities";
+      for (char& ch : q) ch = (char)std::tolower((unsigned char)ch);
+      integral_quantities.push_back(q);
+    }
+  }
   d.get("riemann_solve_type", riemann_solve_type);
   d.get("vis_riemann_solve_type", vis_riemann_solve_type);
   d.get("adv_type", adv_type);

@@ -3,6 +3,7 @@
 #pragma once
 #include <limits>
 #include <string>
+#include <vector>
 
 namespace hifr_host {
 
@@ -13,6 +14,11 @@ struct input {
   std::string mesh_file;
   // output / monitoring
   int plot_freq = 500, restart_dump_freq = 0, monitor_res_freq = 100, res_norm_type = 2;
+  // integral diagnostics (kineticenergy, vorticity, pressuredilatation,
+  // straincolonproduct, devstraincolonproduct; src/input.cpp:133-143) and the
+  // points per direction of their volume cubature rule
+  std::vector<std::string> integral_quantities;
+  int volume_cub_order = 3;
   std::string data_file_name = "Mesh";
   // solver
   int riemann_solve_type = 0, vis_riemann_solve_type = 0, adv_type = 3, dt_type = 0;

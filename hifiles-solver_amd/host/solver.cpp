@@ -23,6 +23,7 @@ struct hifr_solver {
   int_inters ii;
   double time = 0.0;
   std::vector<double> u_host;  // the state of a host-only solver (no device context)
+  std::vector<int> integral_kinds;  // the deck's integral_quantities as HIFR_* kinds
 };
 
 namespace {
@@ -198,6 +199,22 @@ std::string setup_solver(hifr_solver* s) {
   if (!err.empty()) return err;
   for (int t : s->ii.bdy_type)
     if (t != 11) return "boundary condition " + std::to_string(t) + " is not built (isothermal walls, bctype 11)";
+  // integral diagnostics (src/eles.cpp:6503-6643): names -> kinds, volume cubature
+  s->integral_kinds.clear();
+  for (const std::string& q : in.integral_quantities) {
+    static const char* names[] = {"kineticenergy", "vorticity", "pressuredilatation", "straincolonproduct",
+                                  "devstraincolonproduct"};
+    int k = -1;
+    for (int i = 0; i < 5; i++)
+      if (q == names[i]) k = i;
+    if (k < 0) return "integral diagnostic quantity not recognized";
+    s->integral_kinds.push_back(k);
+  }
+  if (s->integral_kinds.size() > 5) return "at most 5 integral quantities";
+  if (!s->integral_kinds.empty()) {
+    err = e.set_volume_cubature(in.volume_cub_order);
+    if (!err.empty()) return err;
+  }
   if (in.host_only) {
     // the initial state stays on the host
     err = in.restart_flag ? read_restart(s, s->u_host) : set_ics(s, s->u_host);
@@ -213,6 +230,12 @@ std::string setup_solver(hifr_solver* s) {
   err = check(hifr_set_interfaces(e.ctx, (int)s->ii.left.size(), s->ii.left.data(), s->ii.right.data()),
               "hifr_set_interfaces");
   if (!err.empty()) return err;
+  if (!s->integral_kinds.empty()) {
+    err = check(hifr_set_cubature(e.ctx, e.n_cubpts_per_ele, e.opp_volume_cubpts.data(),
+                                  e.weight_volume_cubpts.data(), e.vol_detjac_cubpts.data()),
+                "hifr_set_cubature");
+    if (!err.empty()) return err;
+  }
   std::vector<double> u;
   err = in.restart_flag ? read_restart(s, u) : set_ics(s, u);
   if (!err.empty()) return err;
@@ -348,6 +371,10 @@ long hifr_solver_get(hifr_solver* s, int what, double* out) {
         for (long i = 0; i < n; i++) out[i] = s->msh.c2v[i];
       return n;
     case 14: dev = HIFR_RK_REGISTER; n = nu * N * nf; break;
+    case 15: v = &e.loc_volume_cubpts; break;
+    case 16: v = &e.weight_volume_cubpts; break;
+    case 17: v = &e.opp_volume_cubpts; break;
+    case 18: v = &e.vol_detjac_cubpts; break;
     default: return -1;
   }
   if (v) {
@@ -390,6 +417,16 @@ int hifr_solver_read_restart(hifr_solver* s, const char* path, char* err, int er
 }
 
 double hifr_solver_time(const hifr_solver* s) { return s->time; }
+
+int hifr_solver_n_integrals(const hifr_solver* s) { return (int)s->integral_kinds.size(); }
+
+int hifr_solver_integrals(hifr_solver* s, double* out) {
+  if (!s->el->ctx) return 1;
+  if (s->integral_kinds.empty()) return 0;
+  const hifr_status st =
+      hifr_integral_quantities(s->el->ctx, (int)s->integral_kinds.size(), s->integral_kinds.data(), out);
+  return st == HIFR_OK ? 0 : (int)st;
+}
 
 long hifr_solver_interfaces(const hifr_solver* s, int* left, int* right) {
   const long n = (long)s->ii.left.size();

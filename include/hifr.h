@@ -167,6 +167,34 @@ hifr_status hifr_run(hifr_ctx* ctx, int n_steps, int adv_type, double dt);
  * norm_type 0 max, 1 L1, 2 L2 of div/J - const_src per field; out[nf]. */
 hifr_status hifr_norm_residual(hifr_ctx* ctx, int norm_type, double* out);
 
+/* Integral diagnostics (eles::CalcIntegralQuantities, src/eles.cpp:6503-6643;
+ * integral_quantities deck key, src/input.cpp:133-143): volume-cubature integrals
+ * over the mesh of rho|v|^2/2 (kineticenergy), rho|omega|^2/2 (vorticity),
+ * p div v (pressuredilatation), S:S (straincolonproduct) and S^d:S^d
+ * (devstraincolonproduct). */
+enum {
+  HIFR_KINETIC_ENERGY = 0,
+  HIFR_VORTICITY = 1,
+  HIFR_PRESSURE_DILATATION = 2,
+  HIFR_STRAIN_COLON = 3,
+  HIFR_DEV_STRAIN_COLON = 4
+};
+
+/* The volume cubature the integrals use (eles_*::set_volume_cubpts,
+ * eles::set_opp_volume_cubpts and the cubature-point detJ, src/eles_hexas.cpp:418-440,
+ * src/eles.cpp:4111-4130, 5318-5352): n_cub points, the nodal basis there
+ * opp_volume_cubpts (n_cub, Nu), weights (n_cub) and detjac (n_cub, N).  Quads and
+ * hexes; HIFR_ERR_UNSUPPORTED for tets. */
+hifr_status hifr_set_cubature(hifr_ctx* ctx, int n_cub, const double* opp_volume_cubpts, const double* weights,
+                              const double* detjac_cubpts);
+
+/* out[i] = integral of quantity kinds[i] (at most 5) for the current state.  The
+ * velocity gradient is the corrected gradient (a viscous gradient pass of the
+ * current state; viscous contexts only, except for the kinetic energy alone).  The
+ * reference takes it from its last residual evaluation, which in its time loop is
+ * the last RK stage's input state.  Synchronises the context's stream. */
+hifr_status hifr_integral_quantities(hifr_ctx* ctx, int n_q, const int* kinds, double* out);
+
 hifr_status hifr_synchronize(hifr_ctx* ctx);
 
 /* Times n_steps of hifr_run with HIP events on the context's stream.

@@ -53,6 +53,8 @@ void hifr_solver_sizes(const hifr_solver* s, int* out6);
  *  4 detjac_upts (Nu,N)   5 pos_upts (Nu,N,d)  6 opp_0  7 opp_3  8 opp_4
  *  9 JGinv_upts (d,d,Nu,N) 10 tdA_fpts (Nfp,N) 11 norm_fpts (Nfp,N,d) 12 xv (n_verts,d) 13 c2v (as doubles)
  *  14 rk register (Nu,N,nf)
+ *  15 loc_volume_cubpts (d,ncub) 16 weight_volume_cubpts (ncub) 17 opp_volume_cubpts (ncub,Nu)
+ *  18 detjac at the cubature points (ncub,N)   (15-18: decks with integral_quantities)
  * Returns the number of doubles (out may be NULL to query). */
 long hifr_solver_get(hifr_solver* s, int what, double* out);
 int hifr_solver_set_state(hifr_solver* s, const double* disu_upts);
@@ -71,6 +73,11 @@ int hifr_solver_write_restart(hifr_solver* s, const char* path);
  * Sets the solution time from the file. */
 int hifr_solver_read_restart(hifr_solver* s, const char* path, char* err, int errlen);
 double hifr_solver_time(const hifr_solver* s);
+/* The deck's integral_quantities (src/input.cpp:133-143) with its volume_cub_order
+ * rule: their number, and their values for the current state
+ * (CalcIntegralQuantities, src/output.cpp:1241-1270, src/eles.cpp:6503-6643). */
+int hifr_solver_n_integrals(const hifr_solver* s);
+int hifr_solver_integrals(hifr_solver* s, double* out);
 hifr_ctx* hifr_solver_ctx(hifr_solver* s);
 double hifr_solver_dt(const hifr_solver* s);
 int hifr_solver_adv_type(const hifr_solver* s);

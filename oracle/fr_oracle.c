@@ -1768,6 +1768,91 @@ void or_residual(or_solver* s) {
   calculate_corrected_divergence(s);
 }
 
+/* eles::CalcIntegralQuantities, src/eles.cpp:6503-6643 (set_opp_volume_cubpts
+ * :4111-4130 and the cubature-point detJ :5318-5352), for the state disu_upts(0) and
+ * the corrected gradient grad_disu_upts of the last or_residual.  The cubature rule
+ * (loc (d, ncub), weights) is an argument.  kinds: 0 kineticenergy, 1 vorticity,
+ * 2 pressuredilatation, 3 straincolonproduct, 4 devstraincolonproduct. */
+void or_integral_quantities(const or_solver* s, int ncub, const double* loc, const double* wts, int nq,
+                            const int* kinds, double* out) {
+  int nu = s->nu, N = s->n_eles, nf = s->nf, nd = s->nd;
+  double* opp = (double*)xcalloc((size_t)ncub * nu, sizeof(double));
+  for (int i = 0; i < nu; i++)
+    for (int j = 0; j < ncub; j++) opp[j + ncub * i] = nodal_basis(s, i, &loc[nd * j]);
+  for (int m = 0; m < nq; m++) out[m] = 0.0;
+  for (int i = 0; i < N; i++) {
+    for (int j = 0; j < ncub; j++) {
+      double d[3][3], detjac, u[5], g[5][3], S[3][3];
+      calc_d_pos(s, &loc[nd * j], i, d);
+      if (nd == 2) detjac = d[0][0] * d[1][1] - d[0][1] * d[1][0];
+      else
+        detjac = d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) - d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0])
+               + d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
+      for (int m = 0; m < nf; m++) {
+        u[m] = 0.;
+        for (int k = 0; k < nu; k++) u[m] += opp[j + ncub * k] * UPT(s->disu_upts[0], k, i, m);
+      }
+      for (int m = 0; m < nf; m++)
+        for (int n = 0; n < nd; n++) {
+          g[m][n] = 0.;
+          for (int k = 0; k < nu; k++) g[m][n] += opp[j + ncub * k] * UPT4(s->grad_disu_upts, k, i, m, n);
+        }
+      double irho = 1. / u[0];
+      double dudx = irho * (g[1][0] - u[1] * irho * g[0][0]);
+      double dudy = irho * (g[1][1] - u[1] * irho * g[0][1]);
+      double dvdx = irho * (g[2][0] - u[2] * irho * g[0][0]);
+      double dvdy = irho * (g[2][1] - u[2] * irho * g[0][1]);
+      double dudz = 0., dvdz = 0., dwdx = 0., dwdy = 0., dwdz = 0.;
+      if (nd == 3) {
+        dudz = irho * (g[1][2] - u[1] * irho * g[0][2]);
+        dvdz = irho * (g[2][2] - u[2] * irho * g[0][2]);
+        dwdx = irho * (g[3][0] - u[3] * irho * g[0][0]);
+        dwdy = irho * (g[3][1] - u[3] * irho * g[0][1]);
+        dwdz = irho * (g[3][2] - u[3] * irho * g[0][2]);
+      }
+      for (int m = 0; m < nq; m++) {
+        double diagnostic = 0.0, tke = 0.0;
+        if (kinds[m] == 0) {
+          for (int n = 1; n < nf - 1; n++) tke += 0.5 * u[n] * u[n];
+          diagnostic = irho * tke;
+        } else if (kinds[m] == 1) {
+          double wz = dvdx - dudy;
+          diagnostic = wz * wz;
+          if (nd == 3) {
+            double wx = dwdy - dvdz, wy = dudz - dwdx;
+            diagnostic += wx * wx + wy * wy;
+          }
+          diagnostic *= 0.5 / irho;
+        } else if (kinds[m] == 2) {
+          for (int n = 1; n < nf - 1; n++) tke += 0.5 * u[n] * u[n];
+          double pressure = (s->ph.gamma - 1.0) * (u[nf - 1] - irho * tke);
+          diagnostic = nd == 2 ? pressure * (dudx + dvdy) : pressure * (dudx + dvdy + dwdz);
+        } else {
+          S[0][0] = dudx;
+          S[0][1] = (dudy + dvdx) / 2.0;
+          S[1][0] = S[0][1];
+          S[1][1] = dvdy;
+          double diag = (S[0][0] + S[1][1]) / 3.0;
+          if (nd == 3) {
+            S[0][2] = (dudz + dwdx) / 2.0;
+            S[1][2] = (dvdz + dwdy) / 2.0;
+            S[2][0] = S[0][2];
+            S[2][1] = S[1][2];
+            S[2][2] = dwdz;
+            diag += S[2][2] / 3.0;
+          }
+          if (kinds[m] == 4)
+            for (int a = 0; a < nd; a++) S[a][a] -= diag;
+          for (int a = 0; a < nd; a++)
+            for (int b = 0; b < nd; b++) diagnostic += S[a][b] * S[a][b];
+        }
+        out[m] += diagnostic * wts[j] * detjac;
+      }
+    }
+  }
+  free(opp);
+}
+
 /* src/eles.cpp:1003-1198, dt_type 0 */
 void or_advance(or_solver* s, int stage, int adv_type) {
   int nu = s->nu, N = s->n_eles, nf = s->nf;

@@ -117,6 +117,12 @@ void or_set_metrics(or_solver* s, const double* detjac_upts, const double* JGinv
                     const double* norm_fpts);
 /* Number of OpenMP threads used by the dense loops (1 = serial). */
 void or_set_threads(int n);
+/* CalcIntegralQuantities (src/eles.cpp:6503-6643) for the state and the corrected
+ * gradient of the last or_residual, with the cubature rule loc (d, ncub), weights
+ * (ncub); kinds 0 kineticenergy, 1 vorticity, 2 pressuredilatation,
+ * 3 straincolonproduct, 4 devstraincolonproduct; out[nq]. */
+void or_integral_quantities(const or_solver* s, int ncub, const double* loc, const double* wts, int nq,
+                            const int* kinds, double* out);
 /* Common normal flux at one point (static mesh): riemann 0 Rusanov, 2 Roe;
  * ul, ur, fn: nd + 2 conserved values; n: unit normal (nd). */
 void or_common_flux(int nd, int riemann, double gamma, const double* ul, const double* ur, const double* n,

@@ -74,6 +74,7 @@ def lib():
         L.or_norm_residual.argtypes = [vp, C.c_int, dp]
         L.or_set_threads.argtypes = [C.c_int]
         L.or_common_flux.argtypes = [C.c_int, C.c_int, C.c_double, dp, dp, dp, dp]
+        L.or_integral_quantities.argtypes = [vp, C.c_int, dp, dp, C.c_int, ip, dp]
         _LIB = L
     return _LIB
 
@@ -213,6 +214,17 @@ class Oracle:
         if flat.size != self.nu * self.n_eles:
             raise ValueError("nu_t size")
         lib().or_set_sgs_nu(self._h, _dp(flat))
+
+    def integral_quantities(self, loc, weights, kinds):
+        """Integral diagnostics of the current state with the corrected gradient of
+        the last residual(); loc (d, ncub), weights (ncub)."""
+        loc = np.ascontiguousarray(np.asarray(loc, dtype=np.float64).reshape(-1, order="F"))
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        k = np.ascontiguousarray(kinds, dtype=np.int32)
+        out = np.zeros(len(k))
+        lib().or_integral_quantities(self._h, len(w), _dp(loc), _dp(w), len(k),
+                                     k.ctypes.data_as(C.POINTER(C.c_int)), _dp(out))
+        return out
 
     def run_steps(self, n):
         lib().or_run_steps(self._h, int(n))

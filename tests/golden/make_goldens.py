@@ -50,7 +50,7 @@ def deck_text(c, restart_iter=0):
         "mesh_file mesh.%s" % c.get("mesh", "neu"),
         "dx_cyclic %.17g" % min(c["cyc"][0], 1e30), "dy_cyclic %.17g" % min(c["cyc"][1], 1e30),
         "dz_cyclic %.17g" % min(c["cyc"][2], 1e30),
-        "p_res 2", "write_type 0", "inters_cub_order 3", "volume_cub_order 3",
+        "p_res 2", "write_type 0", "inters_cub_order 3", "volume_cub_order %d" % c.get("vol_cub", 3),
         "monitor_res_freq 1", "plot_freq 1000000", "data_file_name golden",
         "restart_dump_freq %d" % c["n_steps"], "res_norm_type 1", "error_norm_type 1", "res_norm_field 0",
         "upts_type_quad 0", "vcjh_scheme_quad %d" % c["vcjh"], "eta_quad 0.0", "sparse_quad 0",
@@ -63,6 +63,8 @@ def deck_text(c, restart_iter=0):
         "Mach_wall 0.", "nx_wall 0.", "ny_wall 0.", "nz_wall 0.",
         "rho_bound 1.", "u_bound 1.0", "v_bound 0.0", "w_bound 0.0", "p_bound 71.428571428571431",
     ]
+    if c.get("integrals"):
+        lines.append("integral_quantities %d %s" % (len(c["integrals"]), " ".join(c["integrals"])))
     if c.get("les"):
         lines += ["LES 1", "filter_type 0", "filter_ratio %.17g" % c["filter_ratio"],
                   "SGS_model %d" % c["sgs_model"], "wall_model 0", "wall_layer_thickness 0.0"]
@@ -167,10 +169,15 @@ def run_reference(c, xv, conn_g, u0, loc1d, loc_upts=None):
         out = os.path.join(wd, "Rest_%09d_p0000.dat" % c["n_steps"])
         with open(out) as fh:
             c["rest_text"] = fh.read()
+        if c.get("integrals"):
+            with open(os.path.join(wd, "history.plt")) as fh:
+                c["hist_text"] = fh.read()
         return read_restart(out, *u0.shape), r.stdout
     finally:
         shutil.rmtree(wd, ignore_errors=True)
 
+
+INTEGRALS = ["kineticenergy", "vorticity", "pressuredilatation", "straincolonproduct", "devstraincolonproduct"]
 
 CASES = [
     # name, element, sizes, order, scheme, physics, state
@@ -258,6 +265,17 @@ CASES = [
          mach=0.2, Re=2800.0, mesh="msh"),
     dict(name="quad_p3_ns_gmsh_perturbed_4x3", ele_type=1, n=(4, 3), order=3, vcjh=2, viscous=1, tau=1.0,
          pen_fact=0.5, state="vortex", noise=1e-2, seed=93, perturb=0.1, kind="residual", mesh="msh"),
+    # integral diagnostics (CalcIntegralQuantities): history.plt of one forward-
+    # Euler step with a negligible dt, so the state is u0 to the last bits
+    dict(name="hex_p3_ns_integrals_3x3x3", ele_type=4, n=(3, 3, 3), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=95, perturb=0.1, kind="integrals", vol_cub=4,
+         integrals=INTEGRALS),
+    dict(name="hex_p2_ns_integrals_2x2x2", ele_type=4, n=(2, 2, 2), order=2, vcjh=2, viscous=1, tau=1.0,
+         pen_fact=0.5, state="tgv", noise=1e-2, seed=96, perturb=0.0, kind="integrals", vol_cub=6,
+         integrals=INTEGRALS),
+    dict(name="quad_p3_ns_integrals_4x3", ele_type=1, n=(4, 3), order=3, vcjh=1, viscous=1, tau=1.0,
+         pen_fact=0.5, state="vortex", noise=1e-2, seed=97, perturb=0.1, kind="integrals", vol_cub=5,
+         integrals=INTEGRALS),
     # Roe flux with entropy fix (riemann_solve_type 2, src/inters.cpp:390-520); the
     # reference evaluates it in 2-D only, so these are quads
     dict(name="quad_p3_euler_roe_perturbed_4x3", ele_type=1, n=(4, 3), order=3, vcjh=1, viscous=0,
@@ -269,6 +287,43 @@ CASES = [
          pen_fact=0.5, state="vortex", noise=1e-2, seed=63, perturb=0.05, kind="rk45", steps=3, dt=1e-3,
          riemann=2),
 ]
+
+
+def read_cubature_table(path, nd):
+    """{rule: (loc (d, n), weights (n))} of data/cubature_hexa.dat or cubature_quad.dat."""
+    lines = open(path).read().split("\n")
+    rules, i = {}, 0
+    while i < len(lines):
+        if lines[i].startswith("rule="):
+            r = int(lines[i][5:])
+            n = r ** nd
+            i += 1
+            cols = []
+            for _ in range(nd + 1):
+                cols.append([float(v) for v in lines[i + 1:i + 1 + n]])
+                i += n + 1
+            rules[r] = (np.array(cols[:nd]), np.array(cols[nd]))
+        else:
+            i += 1
+    return rules
+
+
+def cubature_rule(ele_type, rule):
+    nd = 3 if ele_type == 4 else 2
+    name = "cubature_hexa.dat" if ele_type == 4 else "cubature_quad.dat"
+    return read_cubature_table(os.path.join(REF_HOME, "data", name), nd)[rule]
+
+
+def write_cubature_fixture():
+    """The reference's volume cubature tables as data: hex rules 2-8, quad rules 2-12."""
+    rec = {}
+    for ele_type, name, nd, rules in ((4, "cubature_hexa.dat", 3, range(2, 9)),
+                                      (1, "cubature_quad.dat", 2, range(2, 13))):
+        table = read_cubature_table(os.path.join(REF_HOME, "data", name), nd)
+        for r in rules:
+            rec["%s_%d_loc" % ("hex" if nd == 3 else "quad", r)] = table[r][0]
+            rec["%s_%d_w" % ("hex" if nd == 3 else "quad", r)] = table[r][1]
+    np.savez_compressed(os.path.join(HERE, "tables", "cubature_rules.npz"), **rec)
 
 
 def build_case(c):
@@ -324,6 +379,8 @@ def main():
     if not os.path.exists(REF_BIN):
         sys.exit("build the reference first: make -f oracle/Makefile.ref")
     only = sys.argv[1:]
+    if not only or "cubature" in only:
+        write_cubature_fixture()
     for c in CASES:
         if only and c["name"] not in only:
             continue
@@ -331,6 +388,8 @@ def main():
         c2v = frmesh.gambit_to_internal(c["ele_type"], conn_g)
         if c["kind"] == "residual":
             c.update(adv_type=0, dt=DT_BIG, n_steps=1)
+        elif c["kind"] == "integrals":
+            c.update(adv_type=0, dt=1e-16, n_steps=1)
         else:
             c.update(adv_type=3, n_steps=c["steps"])
         bfaces = [(e, f, 11) for e, f in c.get("walls", [])]
@@ -361,7 +420,22 @@ def main():
             rec["rest_text"] = np.array(c["rest_text"])
         if c.get("mesh"):
             rec["mesh_format"] = np.array(c["mesh"])
-        if c["kind"] == "residual":
+        if c["kind"] == "integrals":
+            # last line of history.plt: iteration, residual norms, forces, CL, CD,
+            # the diagnostics, physical and computational time
+            last = [v for v in c["hist_text"].strip().split("\n")][-1]
+            vals = [float(v) for v in last.split(",")]
+            nq = len(c["integrals"])
+            rec["integral_names"] = np.array(c["integrals"])
+            rec["integrals"] = np.array(vals[-2 - nq:-2])
+            rec["vol_cub"] = c["vol_cub"]
+            o.set_state(u0)
+            o.residual()
+            loc, w = cubature_rule(c["ele_type"], c["vol_cub"])
+            got = o.integral_quantities(loc, w, list(range(nq)))
+            rel = np.abs(got - rec["integrals"]) / np.abs(rec["integrals"])
+            print("%-30s ref-vs-oracle integrals rel: %s" % (c["name"], " ".join("%.2e" % v for v in rel)))
+        elif c["kind"] == "residual":
             R = (u0 - u1) / c["dt"]
             rec["R"] = R
             # oracle on the same mesh/state
