@@ -420,6 +420,8 @@ double hifr_solver_time(const hifr_solver* s) { return s->time; }
 
 int hifr_solver_n_integrals(const hifr_solver* s) { return (int)s->integral_kinds.size(); }
 
+long hifr_solver_n_boundary_points(const hifr_solver* s) { return (long)s->ii.bdy_fpt.size(); }
+
 int hifr_solver_integrals(hifr_solver* s, double* out) {
   if (!s->el->ctx) return 1;
   if (s->integral_kinds.empty()) return 0;

@@ -78,6 +78,8 @@ double hifr_solver_time(const hifr_solver* s);
  * (CalcIntegralQuantities, src/output.cpp:1241-1270, src/eles.cpp:6503-6643). */
 int hifr_solver_n_integrals(const hifr_solver* s);
 int hifr_solver_integrals(hifr_solver* s, double* out);
+/* Flux points on physical boundaries (walls). */
+long hifr_solver_n_boundary_points(const hifr_solver* s);
 hifr_ctx* hifr_solver_ctx(hifr_solver* s);
 double hifr_solver_dt(const hifr_solver* s);
 int hifr_solver_adv_type(const hifr_solver* s);

@@ -77,3 +77,34 @@ def test_integrals_mixed_close_to_fp64(require_gpu):
     rel = np.abs(a - b) / np.abs(a)
     print("mixed vs FP64", rel)
     assert np.all(rel[[0, 1, 3, 4]] < 1e-6), rel
+
+
+def test_solver_executable_writes_history(require_gpu, tmp_path):
+    """hifr_run on a deck with integral_quantities writes history.plt in the
+    reference's column layout (HistoryOutput, src/output.cpp:1525-1616): iteration,
+    log10 residual norms, forces, CL, CD, the diagnostics, physical and wall time;
+    the diagnostics equal the library's after the same steps."""
+    import os
+    import subprocess
+    g = cases.load_golden("hex_p2_ns_integrals_2x2x2")
+    ele_type, xv, c2v = cases.mesh_of(g)
+    groups = [("CYCLIC", frmesh.boundary_faces(ele_type, c2v))]
+    frmesh.write_gmsh(str(tmp_path / "mesh.msh"), xv, c2v, ele_type, bc_groups=groups)
+    deck = cases.product_deck(g, mesh_file="mesh.msh", adv_type=3, dt=1e-3, n_steps=4, monitor_res_freq=2,
+                              restart_flag=0, ic_form=7, **_deck(4))
+    (tmp_path / "input").write_text(deck)
+    exe = os.path.join(os.path.dirname(hifr.LIB_PATH), "hifr_run")
+    r = subprocess.run([exe, "input"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = (tmp_path / "history.plt").read_text().strip().split("\n")
+    assert lines[0].startswith("TITLE") and lines[2].startswith("ZONE")
+    header = lines[1].split(",")
+    assert len(header) == 1 + 5 + 3 + 2 + len(KINDS) + 2
+    rows = [[float(v) for v in ln.split(",")] for ln in lines[3:]]
+    assert [int(r[0]) for r in rows] == [1, 2, 4]
+    s = hifr.Solver.from_deck(str(tmp_path / "input"))
+    s.run(4)
+    last = np.array(rows[-1])
+    assert np.allclose(last[1 + 5 + 5:1 + 5 + 5 + len(KINDS)], s.integrals(), rtol=1e-13, atol=0)
+    assert last[-2] == pytest.approx(4e-3, rel=1e-12)
+    assert np.allclose(last[6:11], 0.0)
