@@ -14,6 +14,7 @@ struct LaunchPtrs {
   const void *detj, *jg, *nbr, *fgeo;
   const void *tI, *tG, *tN, *tD;                            // tet operator tiles
   const void *mlp_w1, *mlp_w2, *mlp_w3, *mlp_w4, *mlp_p;   // learned-closure weights
+  const void* nut;                                          // learned closure: eddy viscosity per point
 };
 
 // The variant: mode (0 traces of u, 1 gradient traces, 2 residual, 3 advance, as
@@ -37,7 +38,10 @@ inline const char* missing_input(const LaunchPtrs& p, const LaunchKind& k) {
   };
   const bool flux = k.mode == 2 || k.mode == 3;
   const bool grad = k.mode == 1;
+  // the gradient-trace kernel evaluates the network and stores the eddy viscosity,
+  // the flux kernels read it
   const bool learned = k.les == 3 && k.mode != 0;
+  const bool net = k.les == 3 && k.mode == 1;
   const Need needs[] = {
       {p.u, "state u", true},
       {p.uf_out, "output state traces", k.mode == 0 || k.mode == 3},
@@ -54,11 +58,12 @@ inline const char* missing_input(const LaunchPtrs& p, const LaunchKind& k) {
       {p.tG, "tet gradient tiles", k.tet && k.mode != 0},
       {p.tN, "tet normal-flux tiles", k.tet && k.mode != 0},
       {p.tD, "tet divergence tiles", k.tet && k.mode != 0},
-      {p.mlp_w1, "network W1", learned},
-      {p.mlp_w2, "network W2", learned},
-      {p.mlp_w3, "network W3", learned},
-      {p.mlp_w4, "network w4", learned},
-      {p.mlp_p, "network parameters", learned},
+      {p.mlp_w1, "network W1", net},
+      {p.mlp_w2, "network W2", net},
+      {p.mlp_w3, "network W3", net},
+      {p.mlp_w4, "network w4", net},
+      {p.mlp_p, "network parameters", net},
+      {p.nut, "eddy viscosity per point", learned},
   };
   for (const Need& n : needs)
     if (n.needed && n.ptr == nullptr) return n.name;

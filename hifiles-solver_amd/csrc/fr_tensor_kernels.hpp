@@ -99,6 +99,10 @@ struct StageArgsT {
   float mlp_mean[mlp::NIN], mlp_scale[mlp::NIN], mlp_lo[mlp::NIN], mlp_hi[mlp::NIN];
   float mlp_out_scale;
   int mlp_on;
+  // learned closure: eddy viscosity at the solution points (Nu, N), written by the
+  // gradient-trace kernel and read by the flux kernels of the same stage, which
+  // see the same state and traces and so the same value
+  R* nut;
 };
 using StageArgs = StageArgsT<double>;
 
@@ -239,10 +243,12 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   R* s_g = smem + S::G_AT;   // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD, over s_u and s_q)
   R* s_f = smem + S::F_AT;   // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE, over s_u)
   R* s_s = smem + S::S_AT;   // [EPB][NF*ND][NU] transformed SGS flux (TRACE_GRAD, LES)
-  // learned closure: network weights, standardised inputs and outputs per point
-  __shared__ std::conditional_t<LES == 3, mlp::Lds, char> s_mlp;
-  __shared__ float s_x[LES == 3 ? BLOCK * mlp::NIN : 1];
-  __shared__ float s_ml[LES == 3 ? BLOCK : 1];
+  // learned closure (the gradient-trace kernel evaluates the network): weights,
+  // standardised inputs and outputs per point
+  constexpr bool NET = LES == 3 && MODE == MODE_TRACE_GRAD;
+  __shared__ std::conditional_t<NET, mlp::Lds, char> s_mlp;
+  __shared__ float s_x[NET ? BLOCK * mlp::NIN : 1];
+  __shared__ float s_ml[NET ? BLOCK : 1];
   const R gam = R(a.ph.gamma);
   RefState<R> ref;
   ref.rho = R(a.ref.rho);
@@ -349,7 +355,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   }
   // the network's weights are read only when one is attached (a.mlp_on is the
   // same for every thread of the launch)
-  if constexpr (LES == 3 && MODE != MODE_TRACE_U)
+  if constexpr (NET)
     if (a.mlp_on) mlp::stage(a.mlpw, s_mlp, tid, BLOCK);
   // EARLY: the LDG delta pass takes the own state trace from the trace array, not
   // from u in LDS, so it needs no barrier after the state load: the state load
@@ -434,10 +440,36 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     // ---- learned closure: eddy viscosity of every point before the flux pass.
     //      Inputs x = (g_ij / |g|, log(delta^2 |g| / nu)) of the velocity gradient
     //      g; the network runs on the block's points as 16-point MFMA tiles;
-    //      outside the training range (or without a network) the WALE value ----
+    //      outside the training range (or without a network) the WALE value.  The
+    //      gradient-trace kernel evaluates it and stores it per point; the flux
+    //      kernels of the stage read it back ----
     [[maybe_unused]] R nu_t_pt = R(0);
     [[maybe_unused]] SgsPoint<ND, R> sp;
-    if constexpr (LES == 3 && VISC) {
+    if constexpr (LES == 3 && VISC && !NET) {
+      if (active) {
+        const long vi = vol_idx(p + NU * (long)e, e);
+        R gr[NF][ND];
+        point_grad(gr);
+        const R inv_detj = R(rcp(detj_at(vi)));
+        R gf[NF][ND];
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+#pragma unroll
+          for (int m = 0; m < ND; m++) {
+            R v = 0;
+#pragma unroll
+            for (int d = 0; d < ND; d++) v += gr[k][d] * (jg_at(d * ND + m, vi) * inv_detj);
+            gf[k][m] = v;
+          }
+        R uo[NF], uu[NF];
+#pragma unroll
+        for (int k = 0; k < NF; k++) uo[k] = s_u[(el * NF + k) * NU + p];
+        full(uo, uu);
+        sgs_point<ND, R>(uu, gf, sp);
+        nu_t_pt = a.nut[p + NU * (long)e];
+      }
+    }
+    if constexpr (NET && VISC) {
       bool use_net = false;
       R scale_pt = R(0), nu_w = R(0);
       if (active) {
@@ -500,7 +532,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         }
       }
       __syncthreads();
-      if (active) nu_t_pt = use_net ? R(s_ml[tid]) * R(a.mlp_out_scale) * scale_pt : nu_w;
+      if (active) {
+        nu_t_pt = use_net ? R(s_ml[tid]) * R(a.mlp_out_scale) * scale_pt : nu_w;
+        a.nut[p + NU * (long)e] = nu_t_pt;
+      }
     }
 
     // ---- per solution point: corrected physical gradient, flux, transformed flux ----

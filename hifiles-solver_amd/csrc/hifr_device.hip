@@ -47,6 +47,9 @@ struct hifr_ctx {
   // points, volume cubature operator / weights / detJ, per-element partial sums
   double *gu, *cub_opp, *cub_w, *cub_detj, *cub_part;
   int ncub;
+  // learned closure: eddy viscosity per solution point (Nu, N), element type of
+  // the traces, written by the gradient-trace kernel, read by the flux kernels
+  void* nut;
   int cur;      // which uf buffer holds the traces of the current state
   // host copies needed to build the interface tables
   std::vector<double> h_tdA, h_norm;
@@ -142,6 +145,7 @@ StageArgsT<R> base_args(hifr_ctx* c) {
   }
   a.mlp_out_scale = c->mlpn.out_scale;
   a.mlp_on = c->mlp_on;
+  a.nut = (R*)c->nut;
   return a;
 }
 
@@ -193,7 +197,8 @@ hifr_status launch_io(hifr_ctx* c, int mode, const StageIO& io) {
   a.dt = io.dt;
   const hifr::LaunchPtrs ptrs{a.u,     a.res,      a.div,      a.uf_in,    a.uf_out,    a.fv_in, a.fv_out,
                               a.detj,  a.jg,       a.nbr,      a.fgeo,     a.tI,        a.tG,    a.tN,
-                              a.tD,    a.mlpw.w1,  a.mlpw.w2,  a.mlpw.w3,  a.mlpw.w4,   a.mlpw.params};
+                              a.tD,    a.mlpw.w1,  a.mlpw.w2,  a.mlpw.w3,  a.mlpw.w4,   a.mlpw.params,
+                              a.nut};
   const hifr::LaunchKind kind{mode, c->cfg.viscous != 0, les_kind(c), c->cfg.ele_type == HIFR_TET};
   if (const char* miss = hifr::missing_input(ptrs, kind)) {
     std::fprintf(stderr, "hifr: stage kernel (mode %d) not launched: %s is not set\n", mode, miss);
@@ -350,6 +355,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   c->jg = nullptr;
   c->gu = c->cub_opp = c->cub_w = c->cub_detj = c->cub_part = nullptr;  // hifr_set_cubature
   c->ncub = 0;
+  c->nut = nullptr;
   c->tI = c->tG = c->tN = c->tD = nullptr;
   c->mlp_buf = nullptr;
   std::memset(&c->mlpw, 0, sizeof(c->mlpw));
@@ -369,6 +375,10 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
       return HIFR_ERR_DEVICE;
     }
     c->mlpw = mlp_weights_in(c->mlp_buf);
+    if (hipMalloc(&c->nut, tsz * nu * N) != hipSuccess) {
+      hifr_destroy(c);
+      return HIFR_ERR_DEVICE;
+    }
   }
   if (hipMemset(c->res, 0, sizeof(double) * nu * N * nf) != hipSuccess) {
     hifr_destroy(c);
@@ -382,7 +392,7 @@ hifr_status hifr_destroy(hifr_ctx* c) {
   if (!c) return HIFR_OK;
   void* ps[] = {c->u,    c->res, c->div, c->uf[0], c->uf[1], c->fv,      c->gf,    c->detj,   c->jg,
                 c->fgeo, c->nbr, c->red, c->tI,    c->tG,    c->tN,      c->tD,    c->mlp_buf, c->gu,
-                c->cub_opp, c->cub_w, c->cub_detj, c->cub_part};
+                c->cub_opp, c->cub_w, c->cub_detj, c->cub_part, c->nut};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
@@ -497,6 +507,13 @@ extern "C" {
 
 hifr_status hifr_set_operators(hifr_ctx* c, const double* opp_0, const double* opp_3, const double* opp_4) {
   if (!c || !opp_0 || !opp_3 || !opp_4) return HIFR_ERR_ARG;
+  {
+    const size_t nb = (size_t)c->nfp * c->nu, nd4 = (size_t)c->nu * c->nu * c->nd;
+    for (size_t i = 0; i < nb; i++)
+      if (!std::isfinite(opp_0[i]) || !std::isfinite(opp_3[i])) return HIFR_ERR_ARG;
+    for (size_t i = 0; i < nd4; i++)
+      if (!std::isfinite(opp_4[i])) return HIFR_ERR_ARG;
+  }
   if (c->cfg.ele_type == HIFR_TET) {
     hifr_status st = c->mixed ? upload_dense<float>(c, opp_0, opp_3, opp_4) : upload_dense<double>(c, opp_0, opp_3, opp_4);
     if (st != HIFR_OK) return st;

@@ -28,7 +28,9 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {
   return (unsigned short)(u >> 16);
 }
 __device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
-__device__ __forceinline__ float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// logistic function from the hardware exponential and reciprocal estimates (each
+// within an ulp or two in FP32, far below the BF16 rounding of the activations)
+__device__ __forceinline__ float sigmoidf(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float siluf(float z) { return z * sigmoidf(z); }
 __device__ __forceinline__ float d_siluf(float z) {
   const float s = sigmoidf(z);

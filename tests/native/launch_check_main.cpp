@@ -20,7 +20,7 @@ static void expect(const char* what, const char* got, const char* want) {
 int main() {
   static char mem[64];
   const void* x = mem;
-  LaunchPtrs all{x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x};
+  LaunchPtrs all{x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x};
   // every variant runs with everything set
   for (int mode = 0; mode < 4; mode++)
     for (int visc = 0; visc < 2; visc++)
@@ -32,20 +32,29 @@ int main() {
           expect(name, missing_input(all, LaunchKind{mode, visc != 0, les, tet != 0}), nullptr);
         }
   // the learned closure with no network weights (a context that had never been
-  // given a network kept null weight pointers): every flux pass is refused, the
-  // state traces of mode 0 never read the network
+  // given a network kept null weight pointers): the gradient-trace kernel, which
+  // evaluates the network, is refused; the flux kernels read only the stored eddy
+  // viscosity, and the state traces of mode 0 never read the network
   LaunchPtrs nonet = all;
   nonet.mlp_w1 = nonet.mlp_w2 = nonet.mlp_w3 = nonet.mlp_w4 = nonet.mlp_p = nullptr;
   expect("learned LES, no weights, gradient traces", missing_input(nonet, LaunchKind{1, true, 3, false}),
          "network W1");
-  expect("learned LES, no weights, residual", missing_input(nonet, LaunchKind{2, true, 3, false}), "network W1");
-  expect("learned LES, no weights, advance", missing_input(nonet, LaunchKind{3, true, 3, false}), "network W1");
+  expect("learned LES, no weights, residual", missing_input(nonet, LaunchKind{2, true, 3, false}), nullptr);
+  expect("learned LES, no weights, advance", missing_input(nonet, LaunchKind{3, true, 3, false}), nullptr);
   expect("learned LES, no weights, traces of u", missing_input(nonet, LaunchKind{0, true, 3, false}), nullptr);
   expect("WALE, no network weights, advance", missing_input(nonet, LaunchKind{3, true, 2, false}), nullptr);
   LaunchPtrs nob = all;
   nob.mlp_p = nullptr;
-  expect("learned LES, no parameter vector, residual", missing_input(nob, LaunchKind{2, true, 3, false}),
+  expect("learned LES, no parameter vector, gradient traces", missing_input(nob, LaunchKind{1, true, 3, false}),
          "network parameters");
+  // the stored eddy viscosity: written by the gradient traces, read by the flux passes
+  LaunchPtrs nonu = all;
+  nonu.nut = nullptr;
+  expect("learned LES, no eddy-viscosity array, gradient traces",
+         missing_input(nonu, LaunchKind{1, true, 3, false}), "eddy viscosity per point");
+  expect("learned LES, no eddy-viscosity array, residual", missing_input(nonu, LaunchKind{2, true, 3, false}),
+         "eddy viscosity per point");
+  expect("WALE, no eddy-viscosity array, advance", missing_input(nonu, LaunchKind{3, true, 2, false}), nullptr);
   // arrays of one mode only
   LaunchPtrs p = all;
   p.res = nullptr;

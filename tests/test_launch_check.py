@@ -2,9 +2,9 @@
 (hifiles-solver_amd/csrc/fr_launch_check.hpp), compiled with g++ on the CPU.
 
 The case it exists for: a learned-closure context (LES model 3) whose network
-weight pointers were null -- the flux kernels then read the weights through
-null pointers.  The check must refuse that launch, and every launch whose
-arrays are all set must pass.
+weight pointers were null -- a kernel evaluating the network then reads the
+weights through null pointers.  The check must refuse that launch, and every
+launch whose arrays are all set must pass.
 """
 import os
 import shutil
@@ -24,4 +24,4 @@ def test_launch_check(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
-    assert "learned LES, no weights, residual" in r.stdout
+    assert "learned LES, no weights, gradient traces" in r.stdout
