@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/hifr.h"
@@ -1046,6 +1047,88 @@ extern "C" hifr_status hifr_norm_residual(hifr_ctx* c, int norm_type, double* ou
     out[k] = v;
   }
   return nan ? HIFR_ERR_NAN : HIFR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// residual norms without allocation: the partial sums of norm_partial_kernel go to
+// a file-scope device array instead of the context's reduction buffer
+
+namespace {
+
+constexpr int kNormBlocks = 512;
+constexpr int kNormMaxFields = 5;
+
+// per-block partials (kNormBlocks x fields); every GPU the library runs on holds
+// its own copy until the process ends.  Host code uses the device address from
+// hipGetSymbolAddress (norm_scratch_ptr), never this name.
+static __device__ __align__(256) double scratch[kNormBlocks * kNormMaxFields];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+hifr_status norm_scratch_ptr(double** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return HIFR_ERR_DEVICE;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      std::fprintf(stderr, "hifr: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return HIFR_ERR_DEVICE;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (double*)g_scratch_ptr[dev];
+  return HIFR_OK;
+}
+
+// out[k] from the per-block partials, summed in block order (the order the
+// allocating entry point sums them on the host)
+__global__ void __launch_bounds__(64) norm_final_kernel(const double* __restrict__ part, int nb, int nf,
+                                                        int norm_type, double npts, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= nf) return;
+  double v = 0.0;
+  for (int b = 0; b < nb; b++) {
+    const double x = part[b * nf + k];
+    v = norm_type == 0 ? fmax(v, x) : v + x;
+  }
+  if (norm_type == 1) v = v / npts;
+  else if (norm_type == 2) v = sqrt(v) / npts;
+  out[k] = v;
+}
+
+}  // namespace
+
+extern "C" hifr_status hifr_norm_residual_noalloc(const double* div, const double* detjac, long n_pts,
+                                                  int pts_per_detjac, int n_fields, int norm_type, double const_src,
+                                                  void* hip_stream, double* out) {
+  if (!div || !detjac || !out || n_pts <= 0 || pts_per_detjac <= 0 || norm_type < 0 || norm_type > 2)
+    return HIFR_ERR_ARG;
+  if (n_fields != 4 && n_fields != 5) {
+    std::fprintf(stderr, "hifr_norm_residual_noalloc: %d fields; the scratch array holds 4 or 5 per block\n",
+                 n_fields);
+    return HIFR_ERR_ARG;
+  }
+  double* part = nullptr;
+  hifr_status st = norm_scratch_ptr(&part);
+  if (st != HIFR_OK) return st;
+  const hipStream_t stream = (hipStream_t)hip_stream;
+  // every block writes its partials, so nothing needs clearing between calls
+  if (n_fields == 5)
+    hipLaunchKernelGGL(norm_partial_kernel<5>, dim3(kNormBlocks), dim3(256), 0, stream, div, detjac, n_pts,
+                       pts_per_detjac, norm_type, const_src, part);
+  else
+    hipLaunchKernelGGL(norm_partial_kernel<4>, dim3(kNormBlocks), dim3(256), 0, stream, div, detjac, n_pts,
+                       pts_per_detjac, norm_type, const_src, part);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(64), 0, stream, part, kNormBlocks, n_fields, norm_type,
+                     (double)n_pts, out);
+  HIP_TRY(hipGetLastError());
+  return HIFR_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -167,6 +167,20 @@ hifr_status hifr_run(hifr_ctx* ctx, int n_steps, int adv_type, double dt);
  * norm_type 0 max, 1 L1, 2 L2 of div/J - const_src per field; out[nf]. */
 hifr_status hifr_norm_residual(hifr_ctx* ctx, int norm_type, double* out);
 
+/* No counterpart in the reference: hifr_norm_residual for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  It allocates
+ * nothing and does not synchronise: div (n_pts points x n_fields fields, the layout
+ * of HIFR_DIV_TCONF_UPTS) and detjac (one value per pts_per_detjac consecutive
+ * points: 1 for per-point, Nu for per-element storage) are device arrays, out is a
+ * device array of n_fields doubles written on hip_stream (NULL: the default
+ * stream); a NaN in the residual propagates into out.  The per-block partial sums
+ * live in a file-scope device array of 512 x 5 doubles per GPU, which limits
+ * n_fields to 4 or 5 (HIFR_ERR_ARG otherwise); calls that overlap on one GPU race
+ * on it. */
+hifr_status hifr_norm_residual_noalloc(const double* div, const double* detjac, long n_pts, int pts_per_detjac,
+                                       int n_fields, int norm_type, double const_src, void* hip_stream,
+                                       double* out);
+
 /* Integral diagnostics (eles::CalcIntegralQuantities, src/eles.cpp:6503-6643;
  * integral_quantities deck key, src/input.cpp:133-143): volume-cubature integrals
  * over the mesh of rho|v|^2/2 (kineticenergy), rho|omega|^2/2 (vorticity),
