@@ -575,13 +575,17 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   constexpr int KU = M::KU, KF = M::KF, NT = M::NT, LD = M::LD;
   constexpr int KA = KU + NFP;                       // rows of u + delta
   constexpr int MT = (EPB * NFP + TetP<P>::BLOCK - 1) / TetP<P>::BLOCK;  // flux-point tasks per thread
-  static_assert(2 * NU + KU <= KF && 2 * NU + KU <= KA, "gradient / SGS blocks fit their regions");
-  __shared__ R smem[(KA + KF + NFP) * LD];
+  static_assert(3 * NU <= KF && NFP <= KA, "gradient / SGS / flux-point blocks fit their regions");
+  // Two regions: u and the LDG delta (KA rows) are dead once the gradient is formed,
+  // and then hold one column of the gradient (or SGS flux) at the flux points; the
+  // gradient (KF rows) is overwritten point by point by the transformed SGS flux once
+  // every flux point has its three gradient columns
+  __shared__ R smem[(KA + KF) * LD];
   R* s_u = smem;              // [KU][LD]  u (deviations on MIX)
   R* s_q = s_u + KU * LD;     // [NFP][LD] uo -> LDG delta
-  R* s_s = smem;              // [3 Nu][LD] transformed SGS flux, row l Nu + p (LES; over u and delta)
+  R* s_o = smem;              // [NFP][LD] one column of the gradient (or SGS flux) at the flux points
   R* s_f = smem + KA * LD;    // [KF][LD]  corrected reference gradient -> physical gradient, row m Nu + p
-  R* s_o = s_f + KF * LD;     // [NFP][LD] one column of the gradient (or SGS flux) at the flux points
+  R* s_s = s_f;               // [3 Nu][LD] transformed SGS flux, row l Nu + p (LES)
 
   const R gam = R(a.ph.gamma);
   RefState<R> ref;
@@ -619,7 +623,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
                                                              : (long)fpt + NFP * (long)ee; };
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
-  for (int i = tid; i < (KA + KF + NFP) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
+  for (int i = tid; i < (KA + KF) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
   __syncthreads();
   if (active) {
 #pragma unroll
@@ -676,9 +680,6 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
     for (int k = 0; k < NF; k++)
 #pragma unroll
       for (int c = 0; c < ND; c++) gr[k][c] = s_f[(c * NU + p) * LD + el * NF + k];
-    R u_own[NF];
-#pragma unroll
-    for (int k = 0; k < NF; k++) u_own[k] = s_u[p * LD + el * NF + k];
     R Jg[ND][ND];
 #pragma unroll
     for (int l = 0; l < ND; l++)
@@ -699,28 +700,6 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
     for (int k = 0; k < NF; k++)
 #pragma unroll
       for (int m = 0; m < ND; m++) s_f[(m * NU + p) * LD + el * NF + k] = gf[k][m];
-    if constexpr (LES != 0) {
-      R uu[NF], sgM[ND][ND], sgE[ND];
-      full(u_own, uu);
-      sgs_flux<ND, LES, R>(uu, gf, R(a.detj[vi] * 8. / 6.), R(a.filter_ratio), P, gam, sgM, sgE);
-      // sgsf(k,l) = sum_m JGinv(l,m) F_sgs(k,m)
-#pragma unroll
-      for (int l = 0; l < ND; l++) {
-        R* srow = s_s + (l * NU + p) * LD + el * NF;
-        srow[0] = R(0);
-#pragma unroll
-        for (int i = 0; i < ND; i++) {
-          R v = 0;
-#pragma unroll
-          for (int m = 0; m < ND; m++) v += Jg[l][m] * sgM[i][m];
-          srow[1 + i] = v;
-        }
-        R v = 0;
-#pragma unroll
-        for (int m = 0; m < ND; m++) v += Jg[l][m] * sgE[m];
-        srow[NF - 1] = v;
-      }
-    }
   }
   __syncthreads();
 
@@ -765,6 +744,45 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
     __syncthreads();
   }
   if constexpr (LES != 0) {
+    // transformed SGS flux at the solution points, over the point's own gradient
+    // entries: sgsf(k,l) = sum_m JGinv(l,m) F_sgs(k,m) (src/eles.cpp:2369-2417)
+    if (active) {
+      const long pe = p + NU * (long)e;
+      const long vi = AFF ? (long)e : pe;
+      constexpr long VOL = AFF ? 1 : NU;
+      R gf[NF][ND], uo[NF], uu[NF], sgM[ND][ND], sgE[ND];
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+#pragma unroll
+        for (int m = 0; m < ND; m++) gf[k][m] = s_f[(m * NU + p) * LD + el * NF + k];
+        const double v = a.u[p + NU * (e + NN * k)];
+        uo[k] = MIX ? R(v - ref_of(k)) : R(v);
+      }
+      full(uo, uu);
+      sgs_flux<ND, LES, R>(uu, gf, R(a.detj[vi] * 8. / 6.), R(a.filter_ratio), P, gam, sgM, sgE);
+      R Jg[ND][ND];
+#pragma unroll
+      for (int l = 0; l < ND; l++)
+#pragma unroll
+        for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * VOL + vi];
+#pragma unroll
+      for (int l = 0; l < ND; l++) {
+        R* srow = s_s + (l * NU + p) * LD + el * NF;
+        srow[0] = R(0);
+#pragma unroll
+        for (int i = 0; i < ND; i++) {
+          R v = 0;
+#pragma unroll
+          for (int m = 0; m < ND; m++) v += Jg[l][m] * sgM[i][m];
+          srow[1 + i] = v;
+        }
+        R v = 0;
+#pragma unroll
+        for (int m = 0; m < ND; m++) v += Jg[l][m] * sgE[m];
+        srow[NF - 1] = v;
+      }
+    }
+    __syncthreads();
     // extrapolated transformed SGS flux dotted with the physical normal
 #pragma unroll
     for (int l = 0; l < ND; l++) {
