@@ -77,7 +77,6 @@ struct TetMfma {
   static constexpr int MT_D = (NU + 15) / 16, KS_D = (KF + NFP) / 4;       // [D | L]
   static constexpr int SZ_I = MT_I * KS_I * 64, SZ_G = MT_G * KS_G * 64, SZ_N = MT_N * KS_N * 64,
                        SZ_D = MT_D * KS_D * 64;
-  static constexpr int ROWS = KU + NFP + KF;  // LDS rows: u, q (delta / fn / jump / div), flux
 };
 
 // Combined operators of the tet products (host tile order in hifr_set_operators).
@@ -220,7 +219,7 @@ fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>
 
 // RIE: inviscid interface flux, 0 Rusanov, 2 Roe (riemann_solve_type)
 template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE = 0>
-__global__ void __launch_bounds__(TetP<P>::BLOCK)
+__global__ void __launch_bounds__(TetP<P>::BLOCK, 3)
 fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
   static_assert(RIE == 0 || (RIE == 2 && !MIX), "Roe flux: FP64 kernels only");
@@ -232,10 +231,17 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using M = TetMfma<P>;
   constexpr int NU = M::NU, NFP = M::NFP, NFPF = TetP<P>::NFPF, EPB = M::EPB;
   constexpr int KU = M::KU, KF = M::KF, NT = M::NT, LD = M::LD;
-  __shared__ R smem[M::ROWS * LD];
+  // Two regions.  The first holds u and the LDG delta; the gradient product writes
+  // the corrected gradient over them (a barrier separates its MFMAs from its stores),
+  // the point pass the transformed flux over the gradient, and the new state goes
+  // there once the divergence product has read the flux.  The second holds the
+  // normal flux, the jumps, the divergence and the traces of the new state.
+  constexpr int RA = (KU + NFP) > KF ? (KU + NFP) : KF;
+  __shared__ R smem[(RA + NFP) * LD];
   R* s_u = smem;              // [KU][LD]   u (deviations on MIX), then the new state
-  R* s_q = s_u + KU * LD;     // [NFP][LD]  uo -> LDG delta; fn -> jump - fn; div (rows < Nu)
-  R* s_f = s_q + NFP * LD;    // [KF][LD]   corrected reference gradient -> transformed flux
+  R* s_d = s_u + KU * LD;     // [NFP][LD]  uo -> LDG delta
+  R* s_f = smem;              // [KF][LD]   corrected reference gradient -> transformed flux
+  R* s_q = smem + RA * LD;    // [NFP][LD]  fn -> jump - fn; div (rows < Nu); traces of the new state
 
   const R gam = R(a.ph.gamma);
   RefState<R> ref;
@@ -274,8 +280,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
                                                              : (long)fpt + NFP * (long)ee; };
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
-  // ---- zero the LDS (padding rows and columns stay zero), then u ----
-  for (int i = tid; i < M::ROWS * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
+  // ---- zero the LDS (padding rows and columns stay zero or finite), then u ----
+  for (int i = tid; i < (RA + NFP) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
   __syncthreads();
   if (active) {
 #pragma unroll
@@ -290,7 +296,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     // ---- own traces uo = I u, then the LDG delta = u_c - uo at each flux point
     //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346) ----
     tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
-        a.tI, s_u, nullptr, NFP, s_q, tid);
+        a.tI, s_u, nullptr, NFP, s_d, tid);
     __syncthreads();
 #pragma unroll 1
     for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
@@ -299,7 +305,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       int nb = a.nbr[fp + NFP * te];
       const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
       nb &= 0x7fffffff;
-      R* q = s_q + fp * LD + tl * NF;
+      R* q = s_d + fp * LD + tl * NF;
       R uo[NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) uo[k] = q[k];
@@ -319,9 +325,10 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       }
     }
     __syncthreads();
-    // ---- corrected reference gradient: D_c u + (L o tnorm_c) delta (src/eles.cpp:1667-1870) ----
-    tet_product<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4>(
-        a.tG, s_u, s_q, 3 * NU, s_f, tid);
+    // ---- corrected reference gradient: D_c u + (L o tnorm_c) delta (src/eles.cpp:1667-1870),
+    //      written over u and delta ----
+    tet_product<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4, true>(
+        a.tG, s_u, s_d, 3 * NU, s_f, tid);
     __syncthreads();
   }
 
@@ -336,9 +343,17 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     for (int l = 0; l < ND; l++)
 #pragma unroll
       for (int m = 0; m < ND; m++) Jg[l][m] = a.jg[(l * ND + m) * NN * VOL + vi];
+    // the point's state: from LDS, or (viscous: the gradient is over it) from HBM
     R u_own[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) u_own[k] = s_u[p * LD + el * NF + k];
+    for (int k = 0; k < NF; k++) {
+      if constexpr (VISC) {
+        const double v = a.u[pe + NU * NN * k];
+        u_own[k] = MIX ? R(v - ref_of(k)) : R(v);
+      } else {
+        u_own[k] = s_u[p * LD + el * NF + k];
+      }
+    }
     ViscT<ND, R> vt;
     if constexpr (VISC) {
       const R inv_detj = R(rcp(a.detj[vi]));
@@ -488,9 +503,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   }
   __syncthreads();
 
-  // the RK register and detJ are requested before the divergence product, so that
-  // their latency overlaps it
-  // (and, on the mixed path, the FP64 state)
+  // the RK register, detJ and the FP64 state are requested before the divergence
+  // product, so that their latency overlaps it
   [[maybe_unused]] double resv[NF], uoldv[NF];
   [[maybe_unused]] double detjv = 0.0;
   if constexpr (MODE == MODE_ADVANCE) {
@@ -501,10 +515,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
         for (int k = 0; k < NF; k++) resv[k] = a.res[pe0 + NU * NN * k];
       }
-      if constexpr (MIX) {
 #pragma unroll
-        for (int k = 0; k < NF; k++) uoldv[k] = a.u[pe0 + NU * NN * k];
-      }
+      for (int k = 0; k < NF; k++) uoldv[k] = a.u[pe0 + NU * NN * k];
     }
   }
 
@@ -528,7 +540,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         const long idx = pe + NU * NN * k;
-        const double uold = MIX ? uoldv[k] : double(s_u[p * LD + el * NF + k]);
+        const double uold = uoldv[k];
         double unew;
         if (a.adv_type == 3) {
           // low-storage RK45 stage (src/eles.cpp:1155-1178)
@@ -564,7 +576,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 // [Nfp][LD] LDS buffer, and each flux-point task folds that column into its
 // velocity / energy gradients (registers) before the next product.
 template <int P, int VIS, bool AFF, bool MIX, int LES>
-__global__ void __launch_bounds__(TetP<P>::BLOCK)
+__global__ void __launch_bounds__(TetP<P>::BLOCK, 3)
 fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
   constexpr int ND = 3, NF = 5, NFACE = 4;
@@ -575,17 +587,19 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   constexpr int KU = M::KU, KF = M::KF, NT = M::NT, LD = M::LD;
   constexpr int KA = KU + NFP;                       // rows of u + delta
   constexpr int MT = (EPB * NFP + TetP<P>::BLOCK - 1) / TetP<P>::BLOCK;  // flux-point tasks per thread
-  static_assert(3 * NU <= KF && NFP <= KA, "gradient / SGS / flux-point blocks fit their regions");
-  // Two regions: u and the LDG delta (KA rows) are dead once the gradient is formed,
-  // and then hold one column of the gradient (or SGS flux) at the flux points; the
-  // gradient (KF rows) is overwritten point by point by the transformed SGS flux once
-  // every flux point has its three gradient columns
-  __shared__ R smem[(KA + KF) * LD];
+  constexpr int RA = KA > KF ? KA : KF;               // rows of the first region
+  static_assert(3 * NU <= KF, "gradient / SGS blocks fit their region");
+  // Two regions.  The first holds u and the LDG delta, and the gradient product
+  // writes the corrected gradient over them (a barrier separates its MFMAs from its
+  // stores); the transformed SGS flux later overwrites the gradient point by point
+  // once every flux point has its three gradient columns.  The second holds one
+  // column of the gradient (or SGS flux) at the flux points.
+  __shared__ R smem[(RA + NFP) * LD];
   R* s_u = smem;              // [KU][LD]  u (deviations on MIX)
   R* s_q = s_u + KU * LD;     // [NFP][LD] uo -> LDG delta
-  R* s_o = smem;              // [NFP][LD] one column of the gradient (or SGS flux) at the flux points
-  R* s_f = smem + KA * LD;    // [KF][LD]  corrected reference gradient -> physical gradient, row m Nu + p
+  R* s_f = smem;              // [KF][LD]  corrected reference gradient -> physical gradient, row m Nu + p
   R* s_s = s_f;               // [3 Nu][LD] transformed SGS flux, row l Nu + p (LES)
+  R* s_o = smem + RA * LD;    // [NFP][LD] one column of the gradient (or SGS flux) at the flux points
 
   const R gam = R(a.ph.gamma);
   RefState<R> ref;
@@ -623,7 +637,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
                                                              : (long)fpt + NFP * (long)ee; };
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
-  for (int i = tid; i < (KA + KF) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
+  for (int i = tid; i < (RA + NFP) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
   __syncthreads();
   if (active) {
 #pragma unroll
@@ -665,7 +679,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   }
   __syncthreads();
   // ---- corrected reference gradient (src/eles.cpp:1667-1870) ----
-  tet_product<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4>(
+  tet_product<R, P, TET_OP_G, M::MT_G, NT, M::KS_G, KU / 4, NFP / 4, true>(
       a.tG, s_u, s_q, 3 * NU, s_f, tid);
   __syncthreads();
 
