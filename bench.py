@@ -132,12 +132,12 @@ TRAFFIC_SOURCE = {
 # keyed by (workload, per-element metrics, precision).  Sources: the summaries
 # under profiles/ named for each kernel and workload.
 MEASURED_KIB = {
-    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (280706.3, 122881.4), "rk": (410887.3, 297642.8)},
-    ("tgv32", False, "mixed"): {"n_eles": 32768, "grad": (181605.0, 61440.4), "rk": (332173.5, 236202.7)},
-    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (154141.9, 122880.9), "rk": (278271.2, 297642.8)},
-    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (108851.5, 61440.3), "rk": (231665.1, 236203.1)},
-    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (48003.6, 32400.0), "rk": (99380.2, 66960.0)},
-    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (33392.2, 16200.0), "rk": (79386.7, 50760.0)},
+    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299322.0, 122882.2), "rk": (412772.5, 297642.8)},
+    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (154599.7, 122881.0), "rk": (278321.1, 297643.0)},
+    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (105658.6, 61440.3), "rk": (231731.5, 236203.3)},
+    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (105347.4, 32400.8), "rk": (153496.3, 66960.8)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (60785.3, 32400.2), "rk": (120380.5, 66960.4)},
+    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (44029.7, 16200.0), "rk": (84144.9, 50760.0)},
 }
 
 

@@ -177,6 +177,35 @@ def deploy(n, p, re, t_end, every, model, network=None, precision=0, log=None):
     return ts, ke, ms * 1e3
 
 
+def time_models(n, p, re, steps, log=None):
+    """Device time per RK45 step of the coarse LES with no model, WALE and the
+    network (random weights with a normalisation whose range admits every input,
+    so that every point evaluates the network: its full cost), after 5 untimed
+    steps.  Returns {model: ms per step}."""
+    import time
+    dt = stable_dt(n, p)
+    out = {}
+    for model in ("none", "wale", "learned"):
+        les = model != "none"
+        s = hifr.Solver.box(tgv_deck(p, re, dt, les=les, learned=les), hifr.HEX, (n,) * 3, (TWO_PI,) * 3,
+                            (0.0,) * 3)
+        if model == "learned":
+            m = hifr.MLP(seed=3)
+            m.set_normalization(np.zeros(10, np.float32), np.ones(10, np.float32), np.full(10, -1e30, np.float32),
+                                np.full(10, 1e30, np.float32), 1.0)
+            s.set_sgs_network(m)
+        s.set_state(tgv_state(s.get("pos_upts")))
+        s.run(5)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.run(steps)
+        s.synchronize()
+        out[model] = (time.perf_counter() - t0) / steps * 1e3
+        if log:
+            log("%-8s %.3f ms per RK45 step" % (model, out[model]))
+    return out
+
+
 def dissipation(t, ke):
     return -np.gradient(ke, t)
 
@@ -199,8 +228,13 @@ def main(argv=None):
     ap.add_argument("--samples-at", type=str, default="2,3,4,5")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--out", type=str, default="")
+    ap.add_argument("--time-steps", type=int, default=0,
+                    help="only time this many RK45 steps of each coarse model and exit")
     args = ap.parse_args(argv)
     log = lambda msg: print(msg, flush=True)
+    if args.time_steps > 0:
+        log("coarse LES step time: %d^3 hexes, P=%d, Re %g" % (args.n_lo, args.p_lo, args.re))
+        return time_models(args.n_lo, args.p_lo, args.re, args.time_steps, log)
     times = [float(v) for v in args.samples_at.split(",")]
     log("high-order run: %d^3 hexes, P=%d, Re %g" % (args.n_hi, args.p_hi, args.re))
     gen = generate(args.n_hi, args.p_hi, args.re, args.t_end, 50, args.p_lo, times, log)

@@ -219,9 +219,10 @@ hifr_status hifr_run_timed(hifr_ctx* ctx, int n_steps, int adv_type, double dt, 
                            double* kernel_ms, int* kernel_launches);
 
 /* Measurement aid (no counterpart in the reference): allocates two buffers of
- * n_doubles, runs a plain streaming copy kernel `reps` times on the default
- * stream and returns the average kernel time in *ms_per_copy.  Used to calibrate
- * HBM byte counters and to state the achievable copy bandwidth. */
+ * n_doubles (at most 2^34), runs a plain streaming copy kernel (one element per
+ * thread) `reps` times on the default stream and returns the average kernel time
+ * in *ms_per_copy.  Used to calibrate HBM byte counters and to state the
+ * achievable copy bandwidth. */
 hifr_status hifr_calibrate_copy(long n_doubles, int reps, double* ms_per_copy);
 
 /* Sizes for callers that allocate host buffers. */
