@@ -259,34 +259,38 @@ __device__ __forceinline__ void visc_finish(const ViscPrim<ND, T>& q, const T (&
 template <int ND, int MODEL, class T>
 __device__ __forceinline__ void sgs_flux(const T* u, const T (*g)[ND], T detj_vref, T filter_ratio, int order,
                                          T gamma, T (*M)[ND], T* E) {
+  // Divisions are products with a reciprocal (rcp: within an ulp) and the powers
+  // are products of square roots: a few ulps from the reference's quotients and
+  // pow(), for a quarter of the instructions.
   constexpr int NF = ND + 2;
-  const T rho = u[0];
+  const T rho = u[0], ir = rcp(rho);
+  const T third = T(1.0 / 3.0);
   T uu[ND];
   T ke = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
-    uu[i] = u[i] / rho;
+    uu[i] = u[i] * ir;
     ke += T(0.5) * (uu[i] * uu[i]);
   }
-  const T inte = u[NF - 1] / rho - ke;
-  const T delta = filter_ratio * pow(detj_vref, T(1) / T(ND)) / (T(order) + T(1));
+  const T inte = u[NF - 1] * ir - ke;
+  const T delta = filter_ratio * (ND == 3 ? cbrt(detj_vref) : sqrt(detj_vref)) * rcp(T(order) + T(1));
   T du[ND][ND], de[ND], S[ND][ND];
 #pragma unroll
   for (int i = 0; i < ND; i++) {
     T dke = ke * g[0][i];
 #pragma unroll
     for (int j = 0; j < ND; j++) {
-      du[i][j] = (g[j + 1][i] - uu[j] * g[0][j]) / rho;
+      du[i][j] = (g[j + 1][i] - uu[j] * g[0][j]) * ir;
       dke += rho * uu[j] * du[i][j];
     }
-    de[i] = (g[NF - 1][i] - dke - g[0][i] * inte) / rho;
+    de[i] = (g[NF - 1][i] - dke - g[0][i] * inte) * ir;
   }
   T diag = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
 #pragma unroll
-    for (int j = 0; j < ND; j++) S[i][j] = (du[i][j] + du[j][i]) / T(2);
-    diag += S[i][i] / T(3);
+    for (int j = 0; j < ND; j++) S[i][j] = (du[i][j] + du[j][i]) * T(0.5);
+    diag += S[i][i] * third;
   }
 #pragma unroll
   for (int i = 0; i < ND; i++) S[i][i] -= diag;
@@ -309,9 +313,9 @@ __device__ __forceinline__ void sgs_flux(const T* u, const T (*g)[ND], T detj_vr
       for (int j = 0; j < ND; j++) {
         T q = 0;
 #pragma unroll
-        for (int k = 0; k < ND; ++k) q += (du[i][k] * du[k][j] + du[j][k] * du[k][i]) / T(2);
+        for (int k = 0; k < ND; ++k) q += (du[i][k] * du[k][j] + du[j][k] * du[k][i]) * T(0.5);
         Sq[i][j] = q;
-        d2 += du[i][j] * du[j][i] / T(3);
+        d2 += du[i][j] * du[j][i] * third;
       }
 #pragma unroll
     for (int i = 0; i < ND; i++) Sq[i][i] -= d2;
@@ -322,10 +326,12 @@ __device__ __forceinline__ void sgs_flux(const T* u, const T (*g)[ND], T detj_vr
         num += Sq[i][j] * Sq[i][j];
         den += S[i][j] * S[i][j];
       }
-    den = pow(den, T(2.5)) + pow(num, T(1.25));
-    num = pow(num, T(1.5));
+    // den^(5/2) + num^(5/4) and num^(3/2)
+    const T sn = sqrt(num);
+    den = den * den * sqrt(den) + num * sqrt(sn);
+    num = num * sn;
     const T cw = T(0.5);
-    mu_t = rho * cw * cw * delta * delta * num / (den + T(1.e-12));
+    mu_t = rho * cw * cw * delta * delta * num * rcp(den + T(1.e-12));
   }
   const T ek = -gamma * mu_t / T(0.5);
 #pragma unroll
@@ -352,7 +358,7 @@ struct SgsPoint {
 template <int ND, class T>
 __device__ __forceinline__ void sgs_point(const T* u, const T (*g)[ND], SgsPoint<ND, T>& o) {
   constexpr int NF = ND + 2;
-  const T rho = u[0], ir = T(1) / rho;
+  const T rho = u[0], ir = rcp(rho);
   T v[ND], ke = 0;
 #pragma unroll
   for (int i = 0; i < ND; i++) {
@@ -379,7 +385,7 @@ __device__ __forceinline__ void sgs_point(const T* u, const T (*g)[ND], SgsPoint
 #pragma unroll
   for (int i = 0; i < ND; i++)
 #pragma unroll
-    for (int j = 0; j < ND; j++) o.Sd[i][j] = T(0.5) * (o.vg[i][j] + o.vg[j][i]) - (i == j ? tr / T(3) : T(0));
+    for (int j = 0; j < ND; j++) o.Sd[i][j] = T(0.5) * (o.vg[i][j] + o.vg[j][i]) - (i == j ? tr * T(1.0 / 3.0) : T(0));
   (void)ke;
 }
 
@@ -404,7 +410,7 @@ __device__ __forceinline__ T wale_nu(const SgsPoint<ND, T>& o, T delta) {
 #pragma unroll
     for (int k = 0; k < ND; k++) d2 += o.vg[i][k] * o.vg[k][i];
 #pragma unroll
-  for (int i = 0; i < ND; i++) Sq[i][i] -= d2 / T(3);
+  for (int i = 0; i < ND; i++) Sq[i][i] -= d2 * T(1.0 / 3.0);
 #pragma unroll
   for (int i = 0; i < ND; i++)
 #pragma unroll
@@ -412,8 +418,9 @@ __device__ __forceinline__ T wale_nu(const SgsPoint<ND, T>& o, T delta) {
       num += Sq[i][j] * Sq[i][j];
       den += o.Sd[i][j] * o.Sd[i][j];
     }
-  const T cw = T(0.5);
-  return cw * cw * delta * delta * pow(num, T(1.5)) / (pow(den, T(2.5)) + pow(num, T(1.25)) + T(1.e-10));
+  // the powers as products of square roots, the quotient with a reciprocal
+  const T cw = T(0.5), sn = sqrt(num);
+  return cw * cw * delta * delta * (num * sn) * rcp(den * den * sqrt(den) + num * sqrt(sn) + T(1.e-10));
 }
 
 // SGS flux of an eddy viscosity mu_t (GPU form): M[i][j] = -2 mu_t Sd[i][j],

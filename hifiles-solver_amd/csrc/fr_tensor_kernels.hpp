@@ -493,7 +493,8 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         full(uo, uu);
         sgs_point<ND, R>(uu, gf, sp);
         const R vref = ND == 3 ? R(8) : R(4);
-        const R delta = R(a.filter_ratio) * pow(R(detj_at(vi)) * vref, R(1) / R(ND)) / R(P + 1);
+        const R vol = R(detj_at(vi)) * vref;
+        const R delta = R(a.filter_ratio) * (ND == 3 ? cbrt(vol) : sqrt(vol)) * R(1.0 / (P + 1));
         R G2 = 0;
 #pragma unroll
         for (int i = 0; i < ND; i++)
