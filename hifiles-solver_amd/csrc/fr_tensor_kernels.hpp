@@ -298,13 +298,43 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   constexpr int MT = (EPB * NFP + BLOCK - 1) / BLOCK;
   constexpr bool PREF = !MIX && MODE != MODE_TRACE_U;
   [[maybe_unused]] int pnb[MT];
+  // ... and, in the flux kernels, each task's flux point, local element and line
+  // (face_line) are decoded once for both face passes:
+  // fp | tl << 9 | dir << 15 | side << 17 | base << 18
+  constexpr bool FTC = PREF && MODE != MODE_TRACE_GRAD;
+  [[maybe_unused]] int pft[MT];
+  static_assert(NFP < 512 && EPB * NFP + BLOCK < 64 * NFP && NU < (1 << 13), "task fields fit their bits");
   if constexpr (PREF) {
 #pragma unroll
     for (int t = 0; t < MT; t++) {
       const int task = tid + t * BLOCK, tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
       pnb[t] = (task < EPB * NFP && te < N) ? a.nbr[fp + NFP * te] : BDY_FLAG;
+      if constexpr (FTC) {
+        int dir, side, base;
+        face_line<P, ND>(fp, dir, side, base);
+        pft[t] = fp | (tl << 9) | (dir << 15) | (side << 17) | (base << 18);
+      }
     }
   }
+  struct FaceTask {
+    int tl, fp, dir, side, base;
+  };
+  [[maybe_unused]] auto face_task = [&](const int t, const int task) {
+    FaceTask f;
+    if constexpr (FTC) {
+      const int k = pft[t];
+      f.fp = k & 511;
+      f.tl = (k >> 9) & 63;
+      f.dir = (k >> 15) & 3;
+      f.side = (k >> 17) & 1;
+      f.base = k >> 18;
+    } else {
+      f.tl = task / NFP;
+      f.fp = task - f.tl * NFP;
+      face_line<P, ND>(f.fp, f.dir, f.side, f.base);
+    }
+    return f;
+  };
   // body(t, task) over this thread's flux-point tasks: unrolled when pnb holds the
   // prefetched indices (registers need a constant t), a plain loop otherwise
   [[maybe_unused]] auto each_task = [&](auto&& body) {
@@ -384,10 +414,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346) ----
     if constexpr (VISC) {
       each_task([&](const int t, const int task) {
-        const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
-        if (task >= EPB * NFP || te >= N) return;
-        int dir, side, base;
-        face_line<P, ND>(fp, dir, side, base);
+        if (task >= EPB * NFP) return;
+        const FaceTask ft = face_task(t, task);
+        const int tl = ft.tl, fp = ft.fp, te = blockIdx.x * EPB + tl, dir = ft.dir, side = ft.side, base = ft.base;
+        if (te >= N) return;
         int nb;
         if constexpr (PREF) nb = pnb[t];
         else nb = a.nbr[fp + NFP * te];
@@ -771,10 +801,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       // ---- interface fluxes and jumps at the flux points
       //      (common_invFlux / common_viscFlux, extrapolate_totalFlux, daxpy) ----
       each_task([&](const int t, const int task) {
-        const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
-        if (task >= EPB * NFP || te >= N) return;
-        int dir, side, base;
-        face_line<P, ND>(fp, dir, side, base);
+        if (task >= EPB * NFP) return;
+        const FaceTask ft = face_task(t, task);
+        const int tl = ft.tl, fp = ft.fp, te = blockIdx.x * EPB + tl, dir = ft.dir, side = ft.side, base = ft.base;
+        if (te >= N) return;
         const int st = T::stride(dir);
         const R* lw = side ? ops.lR : ops.lL;
         const long fe = fp + NFP * (long)te;

@@ -175,6 +175,16 @@ __device__ __forceinline__ void tet_product(const R* __restrict__ A, const R* B1
   else mfma_store<R, H, MT - H, H, NT>(acc, M, C, lane);
 }
 
+// Zero rows [r0, r1) of an LDS matrix with row pitch LD.  The kernels zero only the
+// padding rows that an MFMA product reads as its B operand before anything writes
+// them: their A columns are zero, so they must hold finite values.  Every other
+// row a product reads is written first, and columns that no element owns (a
+// partial last block, the 16th column) reach only output columns nothing reads.
+template <class R, int LD, int BLOCK>
+__device__ __forceinline__ void zero_rows(R* base, int r0, int r1, int tid) {
+  for (int i = r0 * LD + tid; i < r1 * LD; i += BLOCK) base[i] = R(0);
+}
+
 // Traces staged in LDS as [f][5 el + k] written to the trace array: the block's
 // elements' traces are one contiguous run, written in order.
 template <int P, class R>
@@ -201,8 +211,7 @@ fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>
   R* s_q = smem + KU * LD;
   const int tid = threadIdx.x, el = tid / NU, p = tid - el * NU, e = blockIdx.x * EPB + el;
   const long NN = a.N;
-  for (int i = tid; i < (KU + NFP) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
-  __syncthreads();
+  zero_rows<R, LD, TetP<P>::BLOCK>(smem, NU, KU, tid);  // u padding
   if (tid < EPB * NU && e < a.N) {
 #pragma unroll
     for (int k = 0; k < NF; k++) {
@@ -280,9 +289,9 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
                                                              : (long)fpt + NFP * (long)ee; };
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
-  // ---- zero the LDS (padding rows and columns stay zero or finite), then u ----
-  for (int i = tid; i < (RA + NFP) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
-  __syncthreads();
+  // ---- zero the padding rows of u and of the flux, then u ----
+  zero_rows<R, LD, TetP<P>::BLOCK>(smem, NU, KU, tid);
+  zero_rows<R, LD, TetP<P>::BLOCK>(smem, 3 * NU, RA, tid);
   if (active) {
 #pragma unroll
     for (int k = 0; k < NF; k++) {
@@ -637,8 +646,10 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
                                                              : (long)fpt + NFP * (long)ee; };
   constexpr long GEO_PER_ELE = AFF ? NFACE : NFP;
 
-  for (int i = tid; i < (RA + NFP) * LD; i += TetP<P>::BLOCK) smem[i] = R(0);
-  __syncthreads();
+  // padding rows of u and of the gradient (the last direction product reads
+  // rows up to 2 Nu + KU)
+  zero_rows<R, LD, TetP<P>::BLOCK>(smem, NU, KU, tid);
+  zero_rows<R, LD, TetP<P>::BLOCK>(smem, 3 * NU, RA, tid);
   if (active) {
 #pragma unroll
     for (int k = 0; k < NF; k++) {
