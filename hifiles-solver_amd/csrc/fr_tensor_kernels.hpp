@@ -51,6 +51,16 @@ struct Ops1DT {
 };
 using Ops1D = Ops1DT<double>;
 
+// The same factors for n points per line, as a kernel keeps them in LDS: its
+// threads index them by their own position along a line, and per-lane indexed
+// reads of the kernel arguments would be vector-memory loads.
+template <class R, int n>
+struct OpsN {
+  R lL[n], lR[n];
+  R D[n][n];
+  R gL[n], gR[n];
+};
+
 enum { MODE_TRACE_U = 0, MODE_TRACE_GRAD = 1, MODE_RESID = 2, MODE_ADVANCE = 3 };
 
 // nbr entry of a flux point on a physical boundary (isothermal wall)
@@ -284,7 +294,32 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   const int p = tid - el * NU;
   const int e = blockIdx.x * EPB + el;
   const bool active = (tid < EPB * NU) && (e < N);
-  const Ops1DT<R>& ops = a.ops;
+  // the 1-D factors: in LDS for the FP64 flux kernels of affine hexes (every use
+  // follows at least one barrier; 5 % faster at P = 3 and 4); the other kernels
+  // read them from the arguments, where the LDS copy would cost the gradient-trace
+  // and mixed kernels a wave or two per SIMD in registers, and others a block per
+  // CU in LDS
+  constexpr bool OPS_LDS =
+      !MIX && ND == 3 && AFF && LES == 0 && VIS != 2 && RIE == 0 && (MODE == MODE_RESID || MODE == MODE_ADVANCE);
+  __shared__ std::conditional_t<OPS_LDS, OpsN<R, n>, char> s_ops;
+  if constexpr (OPS_LDS) {
+    constexpr int CNT = 4 * n + n * n;
+    static_assert(CNT <= BLOCK, "one value per thread");
+    if (tid < CNT) {
+      R v;
+      if (tid < n) v = a.ops.lL[tid];
+      else if (tid < 2 * n) v = a.ops.lR[tid - n];
+      else if (tid < 2 * n + n * n) v = a.ops.D[(tid - 2 * n) / n][(tid - 2 * n) % n];
+      else if (tid < 3 * n + n * n) v = a.ops.gL[tid - 2 * n - n * n];
+      else v = a.ops.gR[tid - 3 * n - n * n];
+      reinterpret_cast<R*>(&s_ops)[tid] = v;
+    }
+  }
+  using OpsT = std::conditional_t<OPS_LDS, OpsN<R, n>, Ops1DT<R>>;
+  const OpsT* ops_p;
+  if constexpr (OPS_LDS) ops_p = &s_ops;
+  else ops_p = &a.ops;
+  const OpsT& ops = *ops_p;
   // metric index of (point p of element e) and of (flux point fp of element te)
   auto vol_idx = [&](long pe, int ee) -> long { return AFF ? (long)ee : pe; };
   auto geo_idx = [&](int fpt, int ee) -> long { return AFF ? (long)(fpt / NFPF) + NFACE * (long)ee
