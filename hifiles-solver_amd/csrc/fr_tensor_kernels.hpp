@@ -221,16 +221,18 @@ __device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base
 // LES: 0 none, 1 Smagorinsky, 2 WALE eddy viscosity (viscous kernels only), as the
 // reference CPU build computes them; 3 learned eddy viscosity with the WALE model
 // of the reference GPU build as fallback (hexes).
-// Register budget (waves per SIMD) of a kernel variant.  The FP64 advance kernels
-// of affine hexes up to P = 4 (constant viscosity, no LES) are compiled for the
-// number of waves that their LDS allows, so that registers do not bind first (at
-// P = 3: 128 VGPRs and 7 blocks per CU instead of 151 and 6); every other variant
-// would spill under that budget and keeps the compiler's choice.
+// Register budget (waves per SIMD) of a kernel variant.  The advance kernels of
+// affine hexes up to P = 4 (constant viscosity, no LES; mixed: P = 3, 4) are
+// compiled for the number of waves that their LDS allows, so that registers do
+// not bind first (FP64 P = 3: 128 VGPRs and 7 blocks per CU instead of 151 and 6);
+// every other variant would spill under that budget and keeps the compiler's
+// choice.
 template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE>
 constexpr int kernel_waves() {
   using S = SmemLayout<P, ND, EPB, MODE, LES, MIX ? 4 : 8>;
-  return (MODE == MODE_ADVANCE && ND == 3 && P <= 4 && AFF && !MIX && LES == 0 && VIS != 2 && RIE == 0) ? S::WAVES
-                                                                                                          : 1;
+  return (MODE == MODE_ADVANCE && ND == 3 && P <= 4 && (!MIX || P >= 3) && AFF && LES == 0 && VIS != 2 && RIE == 0)
+             ? S::WAVES
+             : 1;
 }
 
 // RIE: inviscid interface flux, 0 Rusanov, 2 Roe (riemann_solve_type)
@@ -294,13 +296,14 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   const int p = tid - el * NU;
   const int e = blockIdx.x * EPB + el;
   const bool active = (tid < EPB * NU) && (e < N);
-  // the 1-D factors: in LDS for the FP64 flux kernels of affine hexes (every use
-  // follows at least one barrier; 5 % faster at P = 3 and 4); the other kernels
-  // read them from the arguments, where the LDS copy would cost the gradient-trace
-  // and mixed kernels a wave or two per SIMD in registers, and others a block per
-  // CU in LDS
+  // the 1-D factors: in LDS for the flux kernels of affine hexes (FP64; mixed at
+  // P = 3, 4) (every use follows at least one barrier; 1-11 % faster); the other
+  // kernels read them from the arguments, where the LDS copy would cost the
+  // gradient-trace kernels a wave per SIMD in registers and others a block per CU
+  // in LDS
   constexpr bool OPS_LDS =
-      !MIX && ND == 3 && AFF && LES == 0 && VIS != 2 && RIE == 0 && (MODE == MODE_RESID || MODE == MODE_ADVANCE);
+      (!MIX || P == 3 || P == 4) && ND == 3 && AFF && LES == 0 && VIS != 2 && RIE == 0 &&
+      (MODE == MODE_RESID || MODE == MODE_ADVANCE);
   __shared__ std::conditional_t<OPS_LDS, OpsN<R, n>, char> s_ops;
   if constexpr (OPS_LDS) {
     constexpr int CNT = 4 * n + n * n;
