@@ -12,7 +12,7 @@ namespace hifr {
 struct LaunchPtrs {
   const void *u, *res, *div, *uf_in, *uf_out, *fv_in, *fv_out;
   const void *detj, *jg, *nbr, *fgeo;
-  const void *tI, *tG, *tN, *tD;                            // tet operator tiles
+  const void *tI, *tG, *tD;                                 // tet operator tiles
   const void *mlp_w1, *mlp_w2, *mlp_w3, *mlp_w4, *mlp_p;   // learned-closure weights
   const void* nut;                                          // learned closure: eddy viscosity per point
 };
@@ -56,7 +56,6 @@ inline const char* missing_input(const LaunchPtrs& p, const LaunchKind& k) {
       {p.res, "RK register", k.mode == 3},
       {p.tI, "tet interpolation tiles", k.tet},
       {p.tG, "tet gradient tiles", k.tet && k.mode != 0},
-      {p.tN, "tet normal-flux tiles", k.tet && k.mode != 0},
       {p.tD, "tet divergence tiles", k.tet && k.mode != 0},
       {p.mlp_w1, "network W1", net},
       {p.mlp_w2, "network W2", net},

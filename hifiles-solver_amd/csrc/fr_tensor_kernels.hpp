@@ -97,11 +97,10 @@ struct StageArgsT {
   RefState<double> ref;  // mixed path: reference state of the deviations
   double filter_ratio;   // LES filter width factor
   // dense element operators of tets (I = opp_0, D = opp_4, L = opp_3), combined and
-  // laid out as 16x16x4 MFMA A-operand tiles: I, [D_c | L o tnorm_c], [tnorm_c o I],
-  // [D_0 D_1 D_2 | L]
+  // laid out as 16x16x4 MFMA A-operand tiles: I, [D_c | L o tnorm_c], and
+  // [D' | L] with D' = [D_0 D_1 D_2] - L [tnorm_c o I] (the normal flux folded in)
   const R* tI;
   const R* tG;
-  const R* tN;
   const R* tD;
   // learned eddy viscosity (LES 3): network weights, input standardisation and
   // training range, output scale; mlp_on 0 gives the WALE fallback everywhere

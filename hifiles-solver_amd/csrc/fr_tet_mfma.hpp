@@ -73,37 +73,29 @@ struct TetMfma {
   static constexpr int LD = NC + 1;
   static constexpr int MT_I = (NFP + 15) / 16, KS_I = KU / 4;              // I
   static constexpr int MT_G = (3 * NU + 15) / 16, KS_G = (KU + NFP) / 4;  // [D_c | L tnorm_c]
-  static constexpr int MT_N = (NFP + 15) / 16, KS_N = KF / 4;              // [tnorm_c I]
   static constexpr int MT_D = (NU + 15) / 16, KS_D = (KF + NFP) / 4;       // [D | L]
-  static constexpr int SZ_I = MT_I * KS_I * 64, SZ_G = MT_G * KS_G * 64, SZ_N = MT_N * KS_N * 64,
-                       SZ_D = MT_D * KS_D * 64;
+  static constexpr int SZ_I = MT_I * KS_I * 64, SZ_G = MT_G * KS_G * 64, SZ_D = MT_D * KS_D * 64;
 };
 
 // Combined operators of the tet products (host tile order in hifr_set_operators).
-enum TetOp { TET_OP_I = 0, TET_OP_G = 1, TET_OP_N = 2, TET_OP_D = 3 };
+enum TetOp { TET_OP_I = 0, TET_OP_G = 1, TET_OP_D = 3 };
 
 // Whether tile (mt, ks) of operator OP can hold a nonzero at order P.  The
-// reference tet's face normals give [L o tnorm_c] (in G) and [tnorm_c o I] (N) zero
-// blocks for the faces other than 0 and c + 1; I and [D | L] are dense.
+// reference tet's face normals give [L o tnorm_c] (in G) zero blocks for the faces
+// other than 0 and c + 1; I and [D' | L] are dense.
 template <int P, int OP>
 __host__ __device__ constexpr bool tet_tile_nz(int mt, int ks) {
-  constexpr int NU = TetP<P>::NU, NFPF = TetP<P>::NFPF, NFP = TetP<P>::NFP, KU = (NU + 3) / 4 * 4;
-  if (OP != TET_OP_G && OP != TET_OP_N) return true;
+  constexpr int NU = TetP<P>::NU, NFPF = TetP<P>::NFPF, KU = (NU + 3) / 4 * 4;
+  if (OP != TET_OP_G) return true;
   for (int r = 16 * mt; r < 16 * mt + 16; r++)
     for (int col = 4 * ks; col < 4 * ks + 4; col++) {
-      if (OP == TET_OP_G) {
-        if (r >= 3 * NU) continue;
-        if (col < KU) {
-          if (col < NU) return true;
-          continue;
-        }
-        const int face = (col - KU) / NFPF, c = r / NU;
-        if (face == 0 || face - 1 == c) return true;
-      } else {
-        if (r >= NFP || col >= 3 * NU) continue;
-        const int face = r / NFPF, c = col / NU;
-        if (face == 0 || face - 1 == c) return true;
+      if (r >= 3 * NU) continue;
+      if (col < KU) {
+        if (col < NU) return true;
+        continue;
       }
+      const int face = (col - KU) / NFPF, c = r / NU;
+      if (face == 0 || face - 1 == c) return true;
     }
   return false;
 }
@@ -428,15 +420,12 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       for (int k = 0; k < NF; k++) s_f[(l * NU + p) * LD + el * NF + k] = fi[k];
     }
   }
-  __syncthreads();
 
-  // ---- normal transformed flux at the flux points: fn = (tnorm o I) Ft ----
-  tet_product<R, P, TET_OP_N, M::MT_N, NT, M::KS_N, M::KS_N, 0>(
-      a.tN, s_f, nullptr, NFP, s_q, tid);
-  __syncthreads();
-
-  // ---- interface fluxes and jumps at the flux points
-  //      (common_invFlux / common_viscFlux, extrapolate_totalFlux, daxpy) ----
+  // ---- interface fluxes at the flux points (common_invFlux / common_viscFlux);
+  //      this pass reads no LDS that the point pass writes, so no barrier
+  //      separates them.  The normal transformed discontinuous flux that the
+  //      reference subtracts here (extrapolate_totalFlux, daxpy) is folded into
+  //      the divergence operator: D Ft + L (jump - N Ft) = (D - L N) Ft + L jump ----
 #pragma unroll 1
   for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
     const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
@@ -505,10 +494,9 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         }
       }
     }
-    // minus the normal transformed discontinuous flux (opp_1 = I o tnorm)
     R* q = s_q + fp * LD + tl * NF;
 #pragma unroll
-    for (int k = 0; k < NF; k++) q[k] = jump[k] - q[k];
+    for (int k = 0; k < NF; k++) q[k] = jump[k];
   }
   __syncthreads();
 
@@ -529,7 +517,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     }
   }
 
-  // ---- divergence plus correction lift: [D_0 D_1 D_2 | L] [Ft; jump] ----
+  // ---- divergence plus correction lift: [D' | L] [Ft; jump], D' = D - L N ----
   tet_product<R, P, TET_OP_D, M::MT_D, NT, M::KS_D, KF / 4, NFP / 4, true>(
       a.tD, s_f, s_q, NU, s_q, tid);
   __syncthreads();

@@ -33,7 +33,7 @@ struct hifr_ctx {
   // device buffers
   double *u, *res, *div, *detj, *fgeo;
   void *uf[2], *fv, *gf, *jg;  // element type double, or float when mixed
-  void *tI, *tG, *tN, *tD;     // tets: combined dense operators as MFMA A-operand tiles
+  void *tI, *tG, *tD;          // tets: combined dense operators as MFMA A-operand tiles
                                // (double, or float when mixed)
   // learned SGS closure (sgs_model HIFR_SGS_LEARNED): the context's own copy of the
   // attached network's weights (mlp_buf, zero while none is attached), which the
@@ -135,7 +135,6 @@ StageArgsT<R> base_args(hifr_ctx* c) {
   a.filter_ratio = c->cfg.filter_ratio;
   a.tI = (const R*)c->tI;
   a.tG = (const R*)c->tG;
-  a.tN = (const R*)c->tN;
   a.tD = (const R*)c->tD;
   a.mlpw = c->mlpw;
   for (int f = 0; f < hifr::mlp::NIN; f++) {
@@ -197,9 +196,8 @@ hifr_status launch_io(hifr_ctx* c, int mode, const StageIO& io) {
   a.rk_b = io.rk_b;
   a.dt = io.dt;
   const hifr::LaunchPtrs ptrs{a.u,     a.res,      a.div,      a.uf_in,    a.uf_out,    a.fv_in, a.fv_out,
-                              a.detj,  a.jg,       a.nbr,      a.fgeo,     a.tI,        a.tG,    a.tN,
-                              a.tD,    a.mlpw.w1,  a.mlpw.w2,  a.mlpw.w3,  a.mlpw.w4,   a.mlpw.params,
-                              a.nut};
+                              a.detj,  a.jg,       a.nbr,      a.fgeo,     a.tI,        a.tG,    a.tD,
+                              a.mlpw.w1, a.mlpw.w2, a.mlpw.w3, a.mlpw.w4, a.mlpw.params, a.nut};
   const hifr::LaunchKind kind{mode, c->cfg.viscous != 0, les_kind(c), c->cfg.ele_type == HIFR_TET};
   if (const char* miss = hifr::missing_input(ptrs, kind)) {
     std::fprintf(stderr, "hifr: stage kernel (mode %d) not launched: %s is not set\n", mode, miss);
@@ -357,7 +355,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
   c->gu = c->cub_opp = c->cub_w = c->cub_detj = c->cub_part = nullptr;  // hifr_set_cubature
   c->ncub = 0;
   c->nut = nullptr;
-  c->tI = c->tG = c->tN = c->tD = nullptr;
+  c->tI = c->tG = c->tD = nullptr;
   c->mlp_buf = nullptr;
   std::memset(&c->mlpw, 0, sizeof(c->mlpw));
   std::memset(&c->mlpn, 0, sizeof(c->mlpn));
@@ -392,7 +390,7 @@ hifr_status hifr_create(const hifr_config* cfg, hifr_ctx** out) {
 hifr_status hifr_destroy(hifr_ctx* c) {
   if (!c) return HIFR_OK;
   void* ps[] = {c->u,    c->res, c->div, c->uf[0], c->uf[1], c->fv,      c->gf,    c->detj,   c->jg,
-                c->fgeo, c->nbr, c->red, c->tI,    c->tG,    c->tN,      c->tD,    c->mlp_buf, c->gu,
+                c->fgeo, c->nbr, c->red, c->tI,    c->tG,    c->tD,    c->mlp_buf, c->gu,
                 c->cub_opp, c->cub_w, c->cub_detj, c->cub_part, c->nut};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -442,7 +440,7 @@ void mfma_tiles(int M, int K, F a, std::vector<R>& out) {
 // normals (face 0: (1,1,1)/sqrt(3); faces 1-3: -e_r, -e_s, -e_t).
 template <int P, class R>
 bool tet_tiles(const double* opp_0, const double* opp_3, const double* opp_4, std::vector<R>& tI,
-               std::vector<R>& tG, std::vector<R>& tN, std::vector<R>& tD) {
+               std::vector<R>& tG, std::vector<R>& tD) {
   using M = hifr::TetMfma<P>;
   constexpr int NU = M::NU, NFP = M::NFP, KU = M::KU, KF = M::KF, NFPF = NFP / 4;
   auto Iv = [&](int f, int m) { return opp_0[f + NFP * m]; };
@@ -456,11 +454,18 @@ bool tet_tiles(const double* opp_0, const double* opp_3, const double* opp_4, st
     const int f = col - KU;
     return Lv(p, f) * tn(f / NFPF, c);
   }, tG);
-  mfma_tiles<R>(NFP, KF, [&](int f, int col) {
-    return col < 3 * NU ? tn(f / NFPF, col / NU) * Iv(f, col % NU) : 0.0;
-  }, tN);
+  // divergence with the normal flux folded in: D' = D - L N, N = [tnorm_c o I]
+  // (the flux pass applies [D' | L] to [Ft; jump] instead of [D | L] to
+  // [Ft; jump - N Ft])
+  std::vector<double> Dp((size_t)NU * 3 * NU);
+  for (int p = 0; p < NU; p++)
+    for (int col = 0; col < 3 * NU; col++) {
+      double ln = 0.0;
+      for (int f = 0; f < NFP; f++) ln += Lv(p, f) * (tn(f / NFPF, col / NU) * Iv(f, col % NU));
+      Dp[(size_t)p * 3 * NU + col] = Dv(col / NU, p, col % NU) - ln;
+    }
   mfma_tiles<R>(NU, KF + NFP, [&](int p, int col) {
-    if (col < 3 * NU) return Dv(col / NU, p, col % NU);
+    if (col < 3 * NU) return Dp[(size_t)p * 3 * NU + col];
     return col < KF ? 0.0 : Lv(p, col - KF);
   }, tD);
   // the kernels skip the tiles that the face normals zero (hifr::tet_tile_nz)
@@ -471,8 +476,7 @@ bool tet_tiles(const double* opp_0, const double* opp_3, const double* opp_4, st
     }
     return true;
   };
-  return skipped_are_zero(tG, M::KS_G, [](int mt, int ks) { return hifr::tet_tile_nz<P, hifr::TET_OP_G>(mt, ks); }) &&
-         skipped_are_zero(tN, M::KS_N, [](int mt, int ks) { return hifr::tet_tile_nz<P, hifr::TET_OP_N>(mt, ks); });
+  return skipped_are_zero(tG, M::KS_G, [](int mt, int ks) { return hifr::tet_tile_nz<P, hifr::TET_OP_G>(mt, ks); });
 }
 
 template <class R>
@@ -482,16 +486,16 @@ hifr_status upload_dense(hifr_ctx* c, const double* opp_0, const double* opp_3, 
     if (!std::isfinite(opp_0[i]) || !std::isfinite(opp_3[i])) return HIFR_ERR_ARG;
   for (size_t i = 0; i < nd * nu * nu; i++)
     if (!std::isfinite(opp_4[i])) return HIFR_ERR_ARG;
-  std::vector<R> tI, tG, tN, tD;
+  std::vector<R> tI, tG, tD;
   bool ok = false;
   switch (c->cfg.order) {
-    case 1: ok = tet_tiles<1>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
-    case 2: ok = tet_tiles<2>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
-    case 3: ok = tet_tiles<3>(opp_0, opp_3, opp_4, tI, tG, tN, tD); break;
+    case 1: ok = tet_tiles<1>(opp_0, opp_3, opp_4, tI, tG, tD); break;
+    case 2: ok = tet_tiles<2>(opp_0, opp_3, opp_4, tI, tG, tD); break;
+    case 3: ok = tet_tiles<3>(opp_0, opp_3, opp_4, tI, tG, tD); break;
     default: return HIFR_ERR_UNSUPPORTED;  // hifr_create admits tets of order 1-3 only
   }
   if (!ok) return HIFR_ERR_ARG;
-  struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->tI, &tI}, {&c->tG, &tG}, {&c->tN, &tN},
+  struct { void** p; const std::vector<R>* v; } bufs[] = {{&c->tI, &tI}, {&c->tG, &tG},
                                                           {&c->tD, &tD}};
   for (auto& b : bufs) {
     if (*b.p) (void)hipFree(*b.p);

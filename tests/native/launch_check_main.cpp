@@ -20,7 +20,7 @@ static void expect(const char* what, const char* got, const char* want) {
 int main() {
   static char mem[64];
   const void* x = mem;
-  LaunchPtrs all{x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x};
+  LaunchPtrs all{x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x, x};
   // every variant runs with everything set
   for (int mode = 0; mode < 4; mode++)
     for (int visc = 0; visc < 2; visc++)
