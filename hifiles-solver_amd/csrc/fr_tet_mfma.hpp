@@ -294,11 +294,10 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   __syncthreads();
 
   if constexpr (VISC) {
-    // ---- own traces uo = I u, then the LDG delta = u_c - uo at each flux point
-    //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346) ----
-    tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
-        a.tI, s_u, nullptr, NFP, s_d, tid);
-    __syncthreads();
+    // ---- the LDG delta = u_c - uo at each flux point
+    //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346).
+    //      The own trace uo = I u is the one the previous pass wrote (the same
+    //      product of the same operator tiles and state, so the same values) ----
 #pragma unroll 1
     for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
       const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
@@ -309,7 +308,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       R* q = s_d + fp * LD + tl * NF;
       R uo[NF];
 #pragma unroll
-      for (int k = 0; k < NF; k++) uo[k] = q[k];
+      for (int k = 0; k < NF; k++) uo[k] = a.uf_in[(fp + NFP * (long)te) * NF + k];
       if (nb & BDY_FLAG) {
         // wall: Dirichlet common solution u_c = (u_l + u_r)/2 (src/inters.cpp:664-668)
         R ur[NF];
@@ -647,10 +646,8 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   }
   __syncthreads();
 
-  // ---- own traces, LDG delta (src/int_inters.cpp:325-346) ----
-  tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
-      a.tI, s_u, nullptr, NFP, s_q, tid);
-  __syncthreads();
+  // ---- LDG delta (src/int_inters.cpp:325-346), over the own trace the previous
+  //      pass wrote (the same values as I u here) ----
 #pragma unroll 1
   for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
     const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
@@ -661,7 +658,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
     R* q = s_q + fp * LD + tl * NF;
     R uo[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) uo[k] = q[k];
+    for (int k = 0; k < NF; k++) uo[k] = a.uf_in[(fp + NFP * (long)te) * NF + k];
     if (nb & BDY_FLAG) {
       R ur[NF];
       ghost(uo, ur);
