@@ -132,12 +132,12 @@ TRAFFIC_SOURCE = {
 # keyed by (workload, per-element metrics, precision).  Sources: the summaries
 # under profiles/ named for each kernel and workload.
 MEASURED_KIB = {
-    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299322.0, 122882.2), "rk": (412772.5, 297642.8)},
-    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (154599.7, 122881.0), "rk": (278321.1, 297643.0)},
-    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (105658.6, 61440.3), "rk": (231731.5, 236203.3)},
-    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (105347.4, 32400.8), "rk": (153496.3, 66960.8)},
-    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (60785.3, 32400.2), "rk": (120380.5, 66960.4)},
-    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (44029.7, 16200.0), "rk": (84144.9, 50760.0)},
+    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299191.6, 122881.9), "rk": (413586.9, 297642.8)},
+    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (153833.8, 122881.0), "rk": (270304.3, 297643.0)},
+    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (105993.7, 61440.4), "rk": (227098.6, 236203.9)},
+    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (112806.8, 32404.0), "rk": (165582.1, 66960.6)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (64898.1, 32400.7), "rk": (131679.9, 66960.3)},
+    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45664.8, 16200.1), "rk": (93143.1, 50760.0)},
 }
 
 
