@@ -131,8 +131,12 @@ hipError_t tet_les(const Args<MIX>& a, hipStream_t s) {
   using T = TetP<P>;
   const int grid = (a.N + T::EPB - 1) / T::EPB;
   if constexpr (MODE == MODE_RESID || MODE == MODE_ADVANCE) {
-    // flux pass: operator products on MFMA (EPB elements per block)
-    hipLaunchKernelGGL((fr_tet_mfma_kernel<P, VIS, MODE, AFF, MIX, LES, RIE>), dim3(grid), dim3(T::BLOCK), 0, s,
+    // flux pass: operator products on MFMA (G column groups of EPB elements per
+    // block; two groups at P = 3 halve the operator-tile loads per element but
+    // need twice the LDS per block, and measured 20 % slower)
+    constexpr int G = 1;
+    const int fgrid = (a.N + G * T::EPB - 1) / (G * T::EPB);
+    hipLaunchKernelGGL((fr_tet_mfma_kernel<P, VIS, MODE, AFF, MIX, LES, RIE, G>), dim3(fgrid), dim3(T::BLOCK), 0, s,
                        a);
   } else if constexpr (MODE == MODE_TRACE_GRAD) {
     hipLaunchKernelGGL((fr_tet_mfma_grad_kernel<P, VIS, AFF, MIX, LES>), dim3(grid), dim3(T::BLOCK), 0, s, a);

@@ -60,11 +60,13 @@ struct Mfma<float> {
 };
 
 // Sizes of the tiled products for order P (shared with the host, which prepares
-// the operator tiles in hifr_set_operators).
-template <int P>
+// the operator tiles in hifr_set_operators).  G column groups: a wavefront's
+// MFMA column block holds G groups of 64 / Nu elements, so that each operator
+// tile loaded from L2 feeds G column tiles.
+template <int P, int G = 1>
 struct TetMfma {
   using T = TetP<P>;
-  static constexpr int NU = T::NU, NFP = T::NFP, EPB = T::EPB;
+  static constexpr int NU = T::NU, NFP = T::NFP, EPB = G * T::EPB;
   static constexpr int KU = (NU + 3) / 4 * 4;          // rows of u (zero-padded)
   static constexpr int KF = (3 * NU + 3) / 4 * 4;      // rows of the flux / gradient
   static constexpr int NT = (5 * EPB + 15) / 16, NC = 16 * NT;
@@ -179,9 +181,9 @@ __device__ __forceinline__ void zero_rows(R* base, int r0, int r1, int tid) {
 
 // Traces staged in LDS as [f][5 el + k] written to the trace array: the block's
 // elements' traces are one contiguous run, written in order.
-template <int P, class R>
+template <int P, class R, int G = 1>
 __device__ __forceinline__ void tet_write_traces(const R* s_q, R* uf_out, int N, int tid) {
-  using M = TetMfma<P>;
+  using M = TetMfma<P, G>;
   constexpr int NFP = M::NFP, EPB = M::EPB, LD = M::LD, NF = 5;
   const int nel = (N - (int)blockIdx.x * EPB) < EPB ? (N - (int)blockIdx.x * EPB) : EPB;
   R* out = uf_out + (long)blockIdx.x * EPB * NFP * NF;
@@ -219,8 +221,9 @@ fr_tet_mfma_trace_kernel(const StageArgsT<std::conditional_t<MIX, float, double>
 }
 
 // RIE: inviscid interface flux, 0 Rusanov, 2 Roe (riemann_solve_type)
-template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE = 0>
-__global__ void __launch_bounds__(TetP<P>::BLOCK, 3)
+// G: column groups (TetMfma); the per-point passes loop over them.
+template <int P, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE = 0, int G = 1>
+__global__ void __launch_bounds__(TetP<P>::BLOCK, G == 1 ? 3 : 2)
 fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   using R = std::conditional_t<MIX, float, double>;
   static_assert(RIE == 0 || (RIE == 2 && !MIX), "Roe flux: FP64 kernels only");
@@ -229,8 +232,8 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   constexpr bool SUTH = VIS == 2;
   static_assert(MODE == MODE_RESID || MODE == MODE_ADVANCE, "flux pass only");
   static_assert(LES == 0 || VISC, "LES needs the viscous kernels");
-  using M = TetMfma<P>;
-  constexpr int NU = M::NU, NFP = M::NFP, NFPF = TetP<P>::NFPF, EPB = M::EPB;
+  using M = TetMfma<P, G>;
+  constexpr int NU = M::NU, NFP = M::NFP, NFPF = TetP<P>::NFPF, EPB = M::EPB, E1 = TetP<P>::EPB;
   constexpr int KU = M::KU, KF = M::KF, NT = M::NT, LD = M::LD;
   // Two regions.  The first holds u and the LDG delta; the gradient product writes
   // the corrected gradient over them (a barrier separates its MFMAs from its stores),
@@ -272,10 +275,11 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   const int N = a.N;
   const long NN = (long)N;
   const int tid = threadIdx.x;
-  const int el = tid / NU;
-  const int p = tid - el * NU;
-  const int e = blockIdx.x * EPB + el;
-  const bool active = (tid < EPB * NU) && (e < N);
+  // lane -> (element within a column group, solution point); group g holds the
+  // block's elements g E1 .. g E1 + E1 - 1
+  const int el0 = tid / NU;
+  const int p = tid - el0 * NU;
+  const bool lane_on = tid < E1 * NU;
   auto vol_idx = [&](long pe, int ee) -> long { return AFF ? (long)ee : pe; };
   auto geo_idx = [&](int fpt, int ee) -> long { return AFF ? (long)(fpt / NFPF) + NFACE * (long)ee
                                                              : (long)fpt + NFP * (long)ee; };
@@ -284,11 +288,15 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   // ---- zero the padding rows of u and of the flux, then u ----
   zero_rows<R, LD, TetP<P>::BLOCK>(smem, NU, KU, tid);
   zero_rows<R, LD, TetP<P>::BLOCK>(smem, 3 * NU, RA, tid);
-  if (active) {
 #pragma unroll
-    for (int k = 0; k < NF; k++) {
-      const double v = a.u[p + NU * (e + NN * k)];
-      s_u[p * LD + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
+  for (int g = 0; g < G; g++) {
+    const int el = g * E1 + el0, e = blockIdx.x * EPB + el;
+    if (lane_on && e < N) {
+#pragma unroll
+      for (int k = 0; k < NF; k++) {
+        const double v = a.u[p + NU * (e + NN * k)];
+        s_u[p * LD + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
+      }
     }
   }
   __syncthreads();
@@ -334,7 +342,10 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 
   // ---- per solution point: physical gradient, flux, transformed flux (written
   //      over the gradient of the same point and element) ----
-  if (active) {
+#pragma unroll 1
+  for (int g = 0; g < G; g++) {
+    const int el = g * E1 + el0, e = blockIdx.x * EPB + el;
+    if (!(lane_on && e < N)) continue;
     const long pe = p + NU * (long)e;
     const long vi = vol_idx(pe, e);
     constexpr long VOL = AFF ? 1 : NU;
@@ -501,18 +512,23 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 
   // the RK register, detJ and the FP64 state are requested before the divergence
   // product, so that their latency overlaps it
-  [[maybe_unused]] double resv[NF], uoldv[NF];
-  [[maybe_unused]] double detjv = 0.0;
+  [[maybe_unused]] double resv[G][NF], uoldv[G][NF];
+  [[maybe_unused]] double detjv[G];
   if constexpr (MODE == MODE_ADVANCE) {
-    if (active) {
-      const long pe0 = p + NU * (long)e;
-      detjv = a.detj[vol_idx(pe0, e)];
-      if (a.adv_type == 3) {
 #pragma unroll
-        for (int k = 0; k < NF; k++) resv[k] = a.res[pe0 + NU * NN * k];
+    for (int g = 0; g < G; g++) {
+      const int el = g * E1 + el0, e = blockIdx.x * EPB + el;
+      detjv[g] = 1.0;
+      if (lane_on && e < N) {
+        const long pe0 = p + NU * (long)e;
+        detjv[g] = a.detj[vol_idx(pe0, e)];
+        if (a.adv_type == 3) {
+#pragma unroll
+          for (int k = 0; k < NF; k++) resv[g][k] = a.res[pe0 + NU * NN * k];
+        }
+#pragma unroll
+        for (int k = 0; k < NF; k++) uoldv[g][k] = a.u[pe0 + NU * NN * k];
       }
-#pragma unroll
-      for (int k = 0; k < NF; k++) uoldv[k] = a.u[pe0 + NU * NN * k];
     }
   }
 
@@ -522,7 +538,10 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   __syncthreads();
 
   // ---- output: div, and the RK update with the new state into LDS ----
-  if (active) {
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    const int el = g * E1 + el0, e = blockIdx.x * EPB + el;
+    if (!(lane_on && e < N)) continue;
     R div[NF];
 #pragma unroll
     for (int k = 0; k < NF; k++) div[k] = s_q[p * LD + el * NF + k];
@@ -532,16 +551,16 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       for (int k = 0; k < NF; k++) a.div[pe + NU * NN * k] = double(div[k]);
     }
     if constexpr (MODE == MODE_ADVANCE) {
-      const double idetj = rcp(detjv);
+      const double idetj = rcp(detjv[g]);
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         const long idx = pe + NU * NN * k;
-        const double uold = uoldv[k];
+        const double uold = uoldv[g][k];
         double unew;
         if (a.adv_type == 3) {
           // low-storage RK45 stage (src/eles.cpp:1155-1178)
           const double rhs = -double(div[k]) * idetj + a.ph.const_src;
-          const double r = a.rk_a * resv[k] + a.dt * rhs;
+          const double r = a.rk_a * resv[g][k] + a.dt * rhs;
           a.res[idx] = r;
           unew = uold + a.rk_b * r;
         } else {
@@ -560,7 +579,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
         a.tI, s_u, nullptr, NFP, s_q, tid);
     __syncthreads();
-    tet_write_traces<P, R>(s_q, a.uf_out, N, tid);
+    tet_write_traces<P, R, G>(s_q, a.uf_out, N, tid);
   }
 }
 
