@@ -21,12 +21,12 @@ constexpr int OFF_W1 = 0, OFF_B1 = 640, OFF_W2 = 704, OFF_B2 = 4800, OFF_W3 = 48
 
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ unsigned short to_bf16(float x) {
-  unsigned u = __float_as_uint(x);
-  u = u + 0x7FFFu + ((u >> 16) & 1u);  // round to nearest, ties to even
-  return (unsigned short)(u >> 16);
-}
+// FP32 -> BF16, round to nearest even: the hardware conversion (v_cvt_pk_bf16_f32,
+// two values per instruction)
+__device__ __forceinline__ unsigned short to_bf16(float x) { return __builtin_bit_cast(unsigned short, (__bf16)x); }
 __device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 // logistic function from the hardware exponential and reciprocal estimates (each
 // within an ulp or two in FP32, far below the BF16 rounding of the activations)
@@ -39,12 +39,10 @@ __device__ __forceinline__ float d_siluf(float z) {
 __device__ __forceinline__ float softplusf(float z) { return z > 20.0f ? z : log1pf(__expf(z)); }
 
 __device__ __forceinline__ s4 pack4(float a, float b, float c, float d) {
-  s4 v;
-  v[0] = (short)to_bf16(a);
-  v[1] = (short)to_bf16(b);
-  v[2] = (short)to_bf16(c);
-  v[3] = (short)to_bf16(d);
-  return v;
+  struct {
+    bf2 lo, hi;
+  } v = {__builtin_convertvector((f2){a, b}, bf2), __builtin_convertvector((f2){c, d}, bf2)};
+  return __builtin_bit_cast(s4, v);
 }
 __device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
 
