@@ -268,7 +268,8 @@ class Solver:
         return lib().hifr_solver_adv_type(self._h)
 
     def run_timed(self, n_steps):
-        """(total_ms, kernel_ms[3], launches[3]) for n_steps with per-kernel HIP events."""
+        """(total_ms, kernel_ms[3], launches[3]) for n_steps, eager launches; HIP
+        events bracket each kernel of every step's first stage."""
         tot = C.c_double()
         km = (C.c_double * 3)()
         kl = (C.c_int * 3)()

@@ -211,9 +211,10 @@ hifr_status hifr_integral_quantities(hifr_ctx* ctx, int n_q, const int* kinds, d
 
 hifr_status hifr_synchronize(hifr_ctx* ctx);
 
-/* Times n_steps of hifr_run with HIP events on the context's stream.
- * total_ms: whole loop.  kernel_ms[i] / kernel_launches[i]: summed duration and
- * launch count of kernel kind i (0 trace, 1 gradient trace, 2 flux/divergence/RK),
+/* Times n_steps of hifr_run (eager launches) with HIP events on the context's
+ * stream.  total_ms: whole loop.  kernel_ms[i] / kernel_launches[i]: summed
+ * duration and count of the timed launches of kernel kind i (0 trace, 1 gradient
+ * trace, 2 flux/divergence/RK): the launches of the first stage of every step,
  * each bracketed by its own events.  Arrays may be NULL. */
 hifr_status hifr_run_timed(hifr_ctx* ctx, int n_steps, int adv_type, double dt, double* total_ms,
                            double* kernel_ms, int* kernel_launches);
