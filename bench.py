@@ -9,20 +9,25 @@ RK45 step (5 stages, i.e. 5 residual evaluations + updates); the metric counts
 DOF x stages per second.
 
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
-                  [--workload tgv32 | tgv48-p4 | tet-channel]
+                  [--workload tgv32 | tgv48-p4 | tet-channel | vortex-quads]
 
 Other workloads (same JSON line, for the record of each configuration):
-  tgv48-p4     the same flow on 48^3 hexes at P=4 (69.1 M DOF), dt = 5e-4;
-  tet-channel  24 x 12 x 12 hexes split into 6 tets each (20,736 tets, P=3,
-               2.07 M DOF), 10% vertex perturbation, isothermal no-slip walls,
-               WALE, Mach 0.2, Re_bulk 2800, dt = 1e-3.
+  tgv48-p4      the same flow on 48^3 hexes at P=4 (69.1 M DOF), dt = 5e-4;
+  tet-channel   24 x 12 x 12 hexes split into 6 tets each (20,736 tets, P=3,
+                2.07 M DOF), 10% vertex perturbation, isothermal no-slip walls,
+                WALE, Mach 0.2, Re_bulk 2800, dt = 1e-3;
+  vortex-quads  2-D Euler isentropic vortex (Mach 0.5) on 20^2 periodic quads
+                on [-10,10]^2, P=3 (25,600 DOF), Rusanov, dt = 5e-3: a mesh
+                small enough that launches, not bytes, set the rate, so the line
+                adds `graph_replay`, the rate of hifr_run's graph replays.
 
 Prints one JSON line.  `value` is the FP64 path; `mixed` holds the same
 measurement of the mixed-precision path (FP32 arithmetic and traces, FP64 state
 and RK update) with its error against the FP64 state after the same steps.
 `roofline` is for the dominant kernel (flux, divergence
 and RK update, one launch per stage): algorithmic bytes per launch / its average
-duration from HIP events recorded around every launch in the timed region.
+duration from HIP events recorded around the launches of the first stage of every
+timed step (events around every launch would lengthen the loop by about 9 %).
 `stage.grad_kernel_roofline` is the same figure for the gradient-trace kernel.
 `per_point_fp64` repeats the FP64 measurement with per-point metrics (the storage a
 curved mesh needs; the default keeps one metric set per affine element).
@@ -70,10 +75,15 @@ WORKLOADS = {
     "tgv32": dict(kind="tgv", n=32, order=3, dt=1e-3),
     "tgv48-p4": dict(kind="tgv", n=48, order=4, dt=5e-4),
     "tet-channel": dict(kind="tet", order=3, dt=1e-3),
+    "vortex-quads": dict(kind="vortex", n=20, order=3, dt=5e-3),
 }
+VORTEX_L = 20.0
 
 
 def describe(w, n, order):
+    if w["kind"] == "vortex":
+        return ("2-D Euler isentropic vortex (beta 5, Mach 0.5 advection), %d^2 periodic quads on [-10,10]^2, P=%d, "
+                "Rusanov, RK45 dt=%g, FP64" % (n, order, w["dt"]))
     if w["kind"] == "tet":
         return ("3-D Navier-Stokes + WALE plane channel (Mach 0.2, Re_bulk 2800), 24x12x12 hexes split into "
                 "20,736 tets with 10%% vertex perturbation, P=%d, isothermal no-slip walls, Rusanov+LDG, "
@@ -85,6 +95,16 @@ def describe(w, n, order):
 def make_solver(w, n, order, steps, per_point, precision):
     """Solver with the workload's initial state uploaded; the oracle's inputs."""
     prec = int(precision == "mixed")
+    if w["kind"] == "vortex":
+        import frmesh
+        xv, conn = frmesh.box_quad(n, n, VORTEX_L, VORTEX_L, origin=(-10.0, -10.0))
+        c2v = frmesh.gambit_to_internal(1, conn)
+        d = hifr.deck_text(hifr_metrics=int(per_point), hifr_precision=prec, viscous=0, order=order,
+                           vcjh_scheme_quad=1, dt=w["dt"], adv_type=3, dx_cyclic=VORTEX_L, dy_cyclic=VORTEX_L,
+                           n_steps=steps)
+        s = hifr.Solver.from_arrays(d, 1, xv, c2v)
+        s.set_state(frmesh.shu_vortex_state(s.get("pos_upts")))
+        return s, (xv, c2v)
     if w["kind"] == "tet":
         import cases
         g = cases.channel_tet_case(order=order, dt=w["dt"])
@@ -97,7 +117,7 @@ def make_solver(w, n, order, steps, per_point, precision):
     return s, None
 
 
-def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
+def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False, viscous=True):
     """HBM bytes per element that the two kernels of a Navier-Stokes stage must
     move, counting every array once:
       gradient-trace kernel: read u, neighbour u_f, JGinv, detJ, face normals,
@@ -110,7 +130,8 @@ def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
     u, RK register, detJ, normals and tdA are FP64; traces and JGinv are FP64, or
     FP32 on the mixed path; the neighbour index is int32.  Metrics are per
     solution point / flux point, or with per_element (every element affine) one
-    JGinv + detJ per element and one normal + tdA per face."""
+    JGinv + detJ per element and one normal + tdA per face.  Euler (not viscous):
+    no gradient-trace kernel, and the flux/RK kernel reads no F_v.n traces."""
     nf = nd + 2
     t = 4 if mixed else 8
     nv = 1 if per_element else nu
@@ -118,6 +139,8 @@ def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False):
     fpts = nfaces if per_element else nfp
     grad = 8 * nu * nf + t * nfp * nf + vol + 8 * fpts * nd + t * nfp * nf + 4 * nfp
     rk = 8 * 4 * nu * nf + t * nfp * nf + t * 2 * nfp * nf + vol + 8 * fpts * (nd + 1) + t * nfp * nf + 4 * nfp
+    if not viscous:
+        return 0, rk - t * 2 * nfp * nf
     return grad, rk
 
 
@@ -140,6 +163,9 @@ MEASURED_KIB = {
     ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45664.8, 16200.1), "rk": (93143.1, 50760.0)},
 }
 
+
+# hifr_run steps meshes of at most this many solution points by graph replay
+GRAPH_MAX_POINTS = 16384
 
 COPY_DOUBLES = 1 << 27  # 1 GiB read and 1 GiB written per copy
 
@@ -225,6 +251,14 @@ def cpu_baseline(s, w, order, g):
     threads = max(1, min(16, phys))
 
     def make(nthreads):
+        if w["kind"] == "vortex":
+            xv, c2v = g
+            od = po.default_deck(equation=0, viscous=0, order=order, adv_type=3, dt=w["dt"], vcjh_scheme=1,
+                                 dx_cyclic=VORTEX_L, dy_cyclic=VORTEX_L)
+            o = po.Oracle(od, 1, xv, c2v, threads=nthreads)
+            import frmesh
+            o.set_state(frmesh.shu_vortex_state(o.get("pos_upts")))
+            return o, "the same %d^2 P=%d Euler vortex workload" % (round(s.n_eles ** 0.5), order)
         if w["kind"] == "tet":
             import cases
             o = cases.oracle_of(g, threads=nthreads)
@@ -265,9 +299,9 @@ def cpu_baseline(s, w, order, g):
     t_stage = statistics.median(stages)
     return {"value": round(dof * 5 / t_step / 1e9, 5), "unit": "GDOF-stage/s", "cores": threads, "kind": "port",
             "sample": "median of 5 RK45 steps (after 1 warm-up step) of %s, dense reference-order operators, "
-                      "%d OpenMP threads, %.2f s per step" % (what, threads, t_step),
+                      "%d OpenMP threads, %.3g s per step" % (what, threads, t_step),
             "one_core": {"value": round(dof / t_stage / 1e9, 5), "unit": "GDOF-stage/s",
-                         "sample": "median of 5 RK stages (after 1 warm-up stage), 1 thread, %.2f s per stage"
+                         "sample": "median of 5 RK stages (after 1 warm-up stage), 1 thread, %.3g s per stage"
                                    % t_stage},
             "cpu": model, "physical_cores_available": phys, "calibration": CPU_CALIBRATION}
 
@@ -350,6 +384,17 @@ def main():
                                  "stage": stage_block(pp, pp["solver"], None, w),
                                  "max_rel_l2_diff_vs_per_element": float("%.3e" % l2p.max())}
         del pp
+    if s.n_eles * s.nu <= GRAPH_MAX_POINTS:
+        # hifr_run on a mesh this small replays one captured HIP graph per step
+        # (the last step of a call runs eagerly); `value` above is the eager rate
+        s.run(args.warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.run(args.steps)
+        s.synchronize()
+        tg = time.perf_counter() - t0
+        rec["graph_replay"] = {"value": round(r["dof"] * 5 * args.steps / tg / 1e9, 4), "unit": "GDOF-stage/s",
+                               "ms_per_step": round(tg * 1e3 / args.steps, 4)}
     if not args.no_cpu_baseline:
         rec["cpu_baseline"] = cpu_baseline(s, w, order, fp64["g"])
     else:
@@ -382,7 +427,7 @@ def measure(args, w, n, order, precision, per_point=None):
     per_element = s.metrics_per_element()
     nfaces = 4 if w["kind"] == "tet" else 2 * s.nd
     grad_b, rk_b = algorithmic_bytes(s.nu, s.nfp, nfaces, s.nd, per_element=per_element,
-                                     mixed=precision == "mixed")
+                                     mixed=precision == "mixed", viscous=w["kind"] != "vortex")
     avg_rk_ms = kms[2] / kl[2]
     avg_grad_ms = kms[1] / kl[1] if kl[1] else 0.0
     return {"solver": s, "g": g, "value": dof * 5 * args.steps / wall / 1e9, "wall": wall, "dof": dof,
@@ -394,7 +439,7 @@ def measure(args, w, n, order, precision, per_point=None):
 def stage_block(r, s, traffic, w):
     stage_ms = r["avg_rk_ms"] + r["avg_grad_ms"]
     stage_bytes = (r["grad_b"] + r["rk_b"]) * s.n_eles
-    model = survey_model_bytes(s.nu, s.nfp, s.nd, les=w["kind"] == "tet")
+    model = survey_model_bytes(s.nu, s.nfp, s.nd, les=w["kind"] == "tet") if w["kind"] != "vortex" else None
     grad_gbs = r["grad_b"] * s.n_eles / (r["avg_grad_ms"] * 1e-3) / 1e9 if r["avg_grad_ms"] else 0.0
     return {"avg_grad_kernel_ms": round(r["avg_grad_ms"], 5), "avg_rk_kernel_ms": round(r["avg_rk_ms"], 5),
             "grad_kernel_bytes_per_element": r["grad_b"],
@@ -403,8 +448,8 @@ def stage_block(r, s, traffic, w):
             "grad_kernel_traffic": round(traffic["grad"] * s.n_eles) if traffic else None,
             "algorithmic_bytes": stage_bytes,
             "achieved_GBs": round(stage_bytes / (stage_ms * 1e-3) / 1e9, 1),
-            "three_phase_model_bytes_per_dof": round(model / s.nu / s.nf, 2),
-            "three_phase_model_GBs": round(model * s.n_eles / (stage_ms * 1e-3) / 1e9, 1),
+            "three_phase_model_bytes_per_dof": round(model / s.nu / s.nf, 2) if model else None,
+            "three_phase_model_GBs": round(model * s.n_eles / (stage_ms * 1e-3) / 1e9, 1) if model else None,
             "event_total_ms": round(r["total_ms"], 3)}
 
 
