@@ -981,7 +981,7 @@ hifr_status hifr_run(hifr_ctx* c, int n_steps, int adv_type, double dt) {
   int it = 0;
   // small meshes: every step but the last as a graph replay; the last runs
   // eagerly and writes the divergence
-  if (!c->timing && (size_t)c->N * c->nu <= kGraphMaxPoints)
+  if ((size_t)c->N * c->nu <= kGraphMaxPoints)
     for (; it < n_steps - 1; it++) {
       s = graph_step(c, adv_type, dt);
       if (s != HIFR_OK) return s;

@@ -160,7 +160,12 @@ hifr_status hifr_residual(hifr_ctx* ctx);
  * adv_type 3 = low-storage RK45 stage `stage` (0..4), adv_type 0 = forward Euler. */
 hifr_status hifr_advance(hifr_ctx* ctx, int stage, int adv_type, double dt);
 
-/* n_steps full time steps (5 stages each for adv_type 3), src/HiFiLES.cpp:154-187. */
+/* n_steps full time steps (5 stages each for adv_type 3), src/HiFiLES.cpp:154-187.
+ * adv_type 0 or 3 (else HIFR_ERR_UNSUPPORTED).  On meshes of at most 16,384
+ * solution points every step but the last replays a HIP graph of one step,
+ * captured on the first call (per trace-buffer parity, adv_type and dt; any
+ * set-up call or new state drops it); the results equal the eager stages bit for
+ * bit.  The last step runs eagerly and leaves HIFR_DIV_TCONF_UPTS current. */
 hifr_status hifr_run(hifr_ctx* ctx, int n_steps, int adv_type, double dt);
 
 /* CalcNormResidual (src/output.cpp:1452-1522) on the last residual:
