@@ -64,6 +64,43 @@ def test_graph_replays_equal_eager_steps(require_gpu, name, over, adv_type):
     assert np.array_equal(s.get("disu_upts"), ref1[3])
 
 
+@pytest.mark.parametrize("with_net", [False, True], ids=["wale_fallback", "network"])
+def test_graph_replays_learned_closure(require_gpu, with_net):
+    """The learned closure's kernels in the captured step: the gradient-trace
+    kernel stores each point's eddy viscosity and the flux kernel of the same
+    stage reads it back; with a network attached (which drops the graphs) its
+    weights are read inside the graph."""
+    import frmesh
+    import hifr
+    from oracle import sgs_mlp as om
+    TWO_PI = cases.TWO_PI
+    xv, conn = frmesh.box_hex(3, 3, 3, TWO_PI, TWO_PI, TWO_PI)
+    g = dict(ele_type=4, xv=xv, conn_gambit=conn, order=3, vcjh=1, viscous=1, tau=1.0, pen_fact=0.5,
+             cyc=np.array([TWO_PI] * 3), dt=1e-4, kind="residual", les=1, sgs_model=1, filter_ratio=2.0,
+             Re=5000.0)
+    s = cases.solver_of(g, adv_type=3, dt=1e-4, hifr_learned_sgs=1)
+    pos = s.get("pos_upts")
+    u0 = frmesh.tgv_state(pos, p0=1.0 / (1.4 * 0.1 ** 2))
+    u0 = u0 * (1.0 + frmesh.seeded_perturbation(u0.shape, 2e-2, 5))
+    if with_net:
+        m = hifr.MLP()
+        p = om.init_params(3)
+        p[om.N_PARAMS - 1] = -3.0
+        m.set_params(p)
+        mean, scale = np.zeros(10, np.float32), np.ones(10, np.float32)
+        mean[9], scale[9] = 5.0, 2.0
+        m.set_normalization(mean, scale, np.full(10, -1e30, np.float32), np.full(10, 1e30, np.float32), 0.1)
+        s.run(3)  # graphs captured without the network, then dropped by the attach
+        s.set_sgs_network(m)
+    print("learned closure, network" if with_net else "learned closure, WALE fallback", flush=True)
+    ref = _eager(s, u0, 6, 5)
+    s.set_state(u0)
+    s.run(4)
+    assert np.array_equal(s.get("disu_upts"), ref[3])
+    s.run(2)
+    assert np.array_equal(s.get("disu_upts"), ref[5])
+
+
 def test_graph_follows_a_change_of_dt(require_gpu):
     """hifr_run with one dt, then another on the same context (no new state in
     between): the second run captures new graphs instead of replaying the first's,
