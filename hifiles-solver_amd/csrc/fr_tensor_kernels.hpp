@@ -177,7 +177,7 @@ __device__ __forceinline__ void line_of(int p, int dir, int& pos, int& base, int
 // u after the divergence pass) only where the plain layout would fit fewer than 6
 // blocks in a CU's 160 KB, the most that the registers of those kernels allow;
 // elsewhere the extra barriers cost more than they gain.
-template <int P, int ND, int EPB, int MODE, int LES, int VB>
+template <int P, int ND, int EPB, int MODE, int LES, int VB, bool SPL = false>
 struct SmemLayout {
   using T = TP<P, ND, EPB>;
   static constexpr int mx(int x, int y) { return x > y ? x : y; }
@@ -189,17 +189,28 @@ struct SmemLayout {
   // LES: transformed SGS flux at the solution points (extrapolated to the faces)
   static constexpr int SG = (MODE == MODE_TRACE_GRAD && LES) ? EPB * T::NF * ND * T::NU : 0;
   static constexpr bool ALIAS_F = FLUX && (U + Q + F) * VB > 163840 / 6;
+  // SPLIT (SPL: the FP64 advance kernels of affine hexes at P = 3 that run under
+  // an LDS-derived register budget): the transformed flux is staged two
+  // directions at a time, directions 0 and 1 with the faces across them (4 of the
+  // 6 faces: one flux-point task per thread), then direction 2 with the other two
+  // faces over the same rows, which keeps a block's LDS to 18 KB (8 blocks per CU
+  // instead of 7, at 128 VGPRs)
+  static constexpr bool SPLIT = SPL && ALIAS_F && ND == 3 && P == 3 && EPB == 2 && VB == 8;
+  static constexpr int NS = SPLIT ? 2 : ND;  // direction slots of the flux rows
+  static constexpr int FS = EPB * T::NF * NS * T::NU;
   static constexpr int Q_AT = FLUX ? 0 : U;
   static constexpr int U_AT = FLUX ? Q : 0;
   static constexpr int G_AT = 0;
   static constexpr int F_AT = ALIAS_F ? Q : Q + U;
   static constexpr int S_AT = mx(U + Q, G);
-  static constexpr int TOTAL = FLUX ? (ALIAS_F ? Q + mx(U, F) : Q + U + F)
+  static constexpr int TOTAL = FLUX ? (ALIAS_F ? Q + mx(U, FS) : Q + U + F)
                                     : (MODE == MODE_TRACE_GRAD ? S_AT + SG : U);
-  // waves per SIMD that the LDS allows (rounded up, at most 8)
+  // waves per SIMD that the LDS allows (rounded up, at most 8; SPLIT: 4, the
+  // register budget of its face passes)
   static constexpr int LDS_BLOCKS = 163840 / (TOTAL * VB);
-  static constexpr int WAVES = mx(1, (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4 > 8 ? 8
-                                         : (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4);
+  static constexpr int WAVES_LDS = mx(1, (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4 > 8 ? 8
+                                             : (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4);
+  static constexpr int WAVES = SPLIT && WAVES_LDS > 4 ? 4 : WAVES_LDS;
 };
 
 // Interpolate NF fields from LDS rows (stride NU apart) to the end of a line.
@@ -227,8 +238,12 @@ __device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base
 // every other variant would spill under that budget and keeps the compiler's
 // choice.
 template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE>
+constexpr bool split_flux() {
+  return MODE == MODE_ADVANCE && AFF && !MIX && LES == 0 && VIS != 2 && RIE == 0;
+}
+template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE>
 constexpr int kernel_waves() {
-  using S = SmemLayout<P, ND, EPB, MODE, LES, MIX ? 4 : 8>;
+  using S = SmemLayout<P, ND, EPB, MODE, LES, MIX ? 4 : 8, split_flux<P, ND, EPB, VIS, MODE, AFF, MIX, LES, RIE>()>;
   return (MODE == MODE_ADVANCE && ND == 3 && P <= 4 && (!MIX || P >= 3) && AFF && LES == 0 && VIS != 2 && RIE == 0)
              ? S::WAVES
              : 1;
@@ -245,7 +260,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   static_assert(LES == 0 || VISC, "LES needs the viscous kernels");
   static_assert(RIE == 0 || (RIE == 2 && !MIX), "Roe flux: FP64 kernels only");
   using T = TP<P, ND, EPB>;
-  using S = SmemLayout<P, ND, EPB, MODE, LES, (int)sizeof(R)>;
+  using S = SmemLayout<P, ND, EPB, MODE, LES, (int)sizeof(R), split_flux<P, ND, EPB, VIS, MODE, AFF, MIX, LES, RIE>()>;
   constexpr int n = T::n, NU = T::NU, NFP = T::NFP, NF = T::NF, BLOCK = T::BLOCK;
   constexpr int NFPF = T::NFPF, NFACE = 2 * ND;
   __shared__ R smem[S::TOTAL];
@@ -254,6 +269,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   R* s_g = smem + S::G_AT;   // [EPB][NF*ND][NU] physical gradient (TRACE_GRAD, over s_u and s_q)
   R* s_f = smem + S::F_AT;   // [EPB][NF*ND][NU] transformed flux (RESID/ADVANCE, over s_u)
   R* s_s = smem + S::S_AT;   // [EPB][NF*ND][NU] transformed SGS flux (TRACE_GRAD, LES)
+  // row of direction d of the transformed flux: NS = ND slots, or with SPLIT two
+  // (directions 0 and 1, then direction 2 in slot 0)
+  constexpr int NS = S::NS;
+  constexpr auto fslot = [](int d) { return S::SPLIT && d == 2 ? 0 : d; };
   // learned closure (the gradient-trace kernel evaluates the network): weights,
   // standardised inputs and outputs per point
   constexpr bool NET = LES == 3 && MODE == MODE_TRACE_GRAD;
@@ -341,10 +360,24 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   constexpr bool FTC = PREF && MODE != MODE_TRACE_GRAD;
   [[maybe_unused]] int pft[MT];
   static_assert(NFP < 512 && EPB * NFP + BLOCK < 64 * NFP && NU < (1 << 13), "task fields fit their bits");
+  // task t of this thread: tid + t * BLOCK; with SPLIT, t = 0 is one of the flux
+  // points of faces 1-4 (directions 0 and 1: flux points NFPF .. 5 NFPF - 1) and
+  // t = 1 one of faces 0 and 5 (direction 2), for the first 2 NFPF EPB threads
+  [[maybe_unused]] auto task_of = [&](const int t) -> int {
+    if constexpr (S::SPLIT) {
+      static_assert(MT == 2 && 4 * NFPF == NU && BLOCK == EPB * NU, "SPLIT task map");
+      if (t == 0) return (tid / NU) * NFP + NFPF + tid % NU;
+      if (tid >= 2 * NFPF * EPB) return EPB * NFP;
+      const int j = tid % (2 * NFPF);
+      return (tid / (2 * NFPF)) * NFP + (j < NFPF ? j : 4 * NFPF + j);
+    } else {
+      return tid + t * BLOCK;
+    }
+  };
   if constexpr (PREF) {
 #pragma unroll
     for (int t = 0; t < MT; t++) {
-      const int task = tid + t * BLOCK, tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
+      const int task = task_of(t), tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
       pnb[t] = (task < EPB * NFP && te < N) ? a.nbr[fp + NFP * te] : BDY_FLAG;
       if constexpr (FTC) {
         int dir, side, base;
@@ -377,7 +410,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   [[maybe_unused]] auto each_task = [&](auto&& body) {
     if constexpr (PREF) {
 #pragma unroll
-      for (int t = 0; t < MT; t++) body(t, tid + t * BLOCK);
+      for (int t = 0; t < MT; t++) body(t, task_of(t));
     } else {
 #pragma unroll 1
       for (int task = tid; task < EPB * NFP; task += BLOCK) body(0, task);
@@ -745,7 +778,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
           for (int k = 0; k < NF; k++) {
             if constexpr (S::ALIAS_F) fx[l][k] = fi[k];
-            else s_f[(el * NF * ND + k * ND + l) * NU + p] = fi[k];
+            else s_f[(el * NF * NS + k * NS + l) * NU + p] = fi[k];
           }
         }
       }
@@ -754,9 +787,9 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       __syncthreads();
       if (active) {
 #pragma unroll
-        for (int l = 0; l < ND; l++)
+        for (int l = 0; l < NS; l++)
 #pragma unroll
-          for (int k = 0; k < NF; k++) s_f[(el * NF * ND + k * ND + l) * NU + p] = fx[l][k];
+          for (int k = 0; k < NF; k++) s_f[(el * NF * NS + k * NS + l) * NU + p] = fx[l][k];
       }
     }
     if constexpr (MODE == MODE_TRACE_GRAD) {
@@ -837,7 +870,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     } else {
       // ---- interface fluxes and jumps at the flux points
       //      (common_invFlux / common_viscFlux, extrapolate_totalFlux, daxpy) ----
-      each_task([&](const int t, const int task) {
+      auto face_body = [&](const int t, const int task) {
         if (task >= EPB * NFP) return;
         const FaceTask ft = face_task(t, task);
         const int tl = ft.tl, fp = ft.fp, te = blockIdx.x * EPB + tl, dir = ft.dir, side = ft.side, base = ft.base;
@@ -921,10 +954,48 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
         for (int k = 0; k < NF; k++) {
           R v = 0;
 #pragma unroll
-          for (int m = 0; m < n; m++) v += lw[m] * s_f[(tl * NF * ND + k * ND + dir) * NU + base + m * st];
+          for (int m = 0; m < n; m++) v += lw[m] * s_f[(tl * NF * NS + k * NS + fslot(dir)) * NU + base + m * st];
           s_q[(tl * NFP + fp) * NF + k] = jump[k] - sgn * v;
         }
-      });
+      };
+      // divergence of the transformed flux in directions [d0, d1) plus their
+      // correction lift, into div
+      [[maybe_unused]] R div[NF];
+#pragma unroll
+      for (int k = 0; k < NF; k++) div[k] = 0;
+      auto div_dirs = [&](const int d0, const int d1) {
+#pragma unroll
+        for (int d = 0; d < ND; d++) {
+          if (d < d0 || d >= d1) continue;
+          int pos, lb, fpL, fpR;
+          line_of<P, ND>(p, d, pos, lb, fpL, fpR);
+          const int st = T::stride(d);
+#pragma unroll
+          for (int k = 0; k < NF; k++) {
+            R v = 0;
+#pragma unroll
+            for (int m = 0; m < n; m++) v += ops.D[pos][m] * s_f[(el * NF * NS + k * NS + fslot(d)) * NU + lb + m * st];
+            div[k] += v + (ops.gR[pos] * s_q[(el * NFP + fpR) * NF + k] - ops.gL[pos] * s_q[(el * NFP + fpL) * NF + k]);
+          }
+        }
+      };
+      if constexpr (S::SPLIT) {
+        // directions 0 and 1: faces 1-4, their divergence; then direction 2
+        // over the same flux rows: faces 0 and 5 (tasks of the first 2 NFPF EPB
+        // threads); its divergence completes div below
+        face_body(0, task_of(0));
+        __syncthreads();
+        if (active) div_dirs(0, 2);
+        __syncthreads();
+        if (active) {
+#pragma unroll
+          for (int k = 0; k < NF; k++) s_f[(el * NF * NS + k * NS + fslot(2)) * NU + p] = fx[2][k];
+        }
+        __syncthreads();
+        face_body(1, task_of(1));
+      } else {
+        each_task(face_body);
+      }
       __syncthreads();
 
       // ---- divergence of the transformed flux (calculate_divergence) plus the
@@ -958,22 +1029,7 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
             for (int k = 0; k < NF; k++) resv[k] = __builtin_nontemporal_load(&a.res[pe0 + NU * NN * k]);
           }
         }
-        R div[NF];
-#pragma unroll
-        for (int k = 0; k < NF; k++) div[k] = 0;
-#pragma unroll
-        for (int d = 0; d < ND; d++) {
-          int pos, lb, fpL, fpR;
-          line_of<P, ND>(p, d, pos, lb, fpL, fpR);
-          const int st = T::stride(d);
-#pragma unroll
-          for (int k = 0; k < NF; k++) {
-            R v = 0;
-#pragma unroll
-            for (int m = 0; m < n; m++) v += ops.D[pos][m] * s_f[(el * NF * ND + k * ND + d) * NU + lb + m * st];
-            div[k] += v + (ops.gR[pos] * s_q[(el * NFP + fpR) * NF + k] - ops.gL[pos] * s_q[(el * NFP + fpL) * NF + k]);
-          }
-        }
+        div_dirs(S::SPLIT ? 2 : 0, ND);
         const long pe = p + NU * (long)e;
         if (MODE == MODE_RESID || a.div) {
 #pragma unroll
