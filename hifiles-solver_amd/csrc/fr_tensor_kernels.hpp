@@ -188,14 +188,15 @@ struct SmemLayout {
   static constexpr int F = FLUX ? EPB * T::NF * ND * T::NU : 0;
   // LES: transformed SGS flux at the solution points (extrapolated to the faces)
   static constexpr int SG = (MODE == MODE_TRACE_GRAD && LES) ? EPB * T::NF * ND * T::NU : 0;
-  static constexpr bool ALIAS_F = FLUX && (U + Q + F) * VB > 163840 / 6;
+  static constexpr bool SPLIT_OK = SPL && FLUX && ND == 3 && P == 3 && EPB == 2;
+  static constexpr bool ALIAS_F = FLUX && ((U + Q + F) * VB > 163840 / 6 || SPLIT_OK);
   // SPLIT (SPL: the FP64 advance kernels of affine hexes at P = 3 that run under
   // an LDS-derived register budget): the transformed flux is staged two
   // directions at a time, directions 0 and 1 with the faces across them (4 of the
   // 6 faces: one flux-point task per thread), then direction 2 with the other two
   // faces over the same rows, which keeps a block's LDS to 18 KB (8 blocks per CU
   // instead of 7, at 128 VGPRs)
-  static constexpr bool SPLIT = SPL && ALIAS_F && ND == 3 && P == 3 && EPB == 2 && VB == 8;
+  static constexpr bool SPLIT = SPLIT_OK;
   static constexpr int NS = SPLIT ? 2 : ND;  // direction slots of the flux rows
   static constexpr int FS = EPB * T::NF * NS * T::NU;
   static constexpr int Q_AT = FLUX ? 0 : U;
@@ -210,7 +211,8 @@ struct SmemLayout {
   static constexpr int LDS_BLOCKS = 163840 / (TOTAL * VB);
   static constexpr int WAVES_LDS = mx(1, (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4 > 8 ? 8
                                              : (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4);
-  static constexpr int WAVES = SPLIT && WAVES_LDS > 4 ? 4 : WAVES_LDS;
+  static constexpr int WAVES_MAX = VB == 8 ? 4 : 7;
+  static constexpr int WAVES = SPLIT && WAVES_LDS > WAVES_MAX ? WAVES_MAX : WAVES_LDS;
 };
 
 // Interpolate NF fields from LDS rows (stride NU apart) to the end of a line.
@@ -239,7 +241,7 @@ __device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base
 // choice.
 template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE>
 constexpr bool split_flux() {
-  return MODE == MODE_ADVANCE && AFF && !MIX && LES == 0 && VIS != 2 && RIE == 0;
+  return MODE == MODE_ADVANCE && AFF && LES == 0 && VIS != 2 && RIE == 0;
 }
 template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE>
 constexpr int kernel_waves() {
