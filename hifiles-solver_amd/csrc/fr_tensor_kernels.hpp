@@ -190,12 +190,13 @@ struct SmemLayout {
   static constexpr int SG = (MODE == MODE_TRACE_GRAD && LES) ? EPB * T::NF * ND * T::NU : 0;
   static constexpr bool SPLIT_OK = SPL && FLUX && ND == 3 && P == 3 && EPB == 2;
   static constexpr bool ALIAS_F = FLUX && ((U + Q + F) * VB > 163840 / 6 || SPLIT_OK);
-  // SPLIT (SPL: the FP64 advance kernels of affine hexes at P = 3 that run under
-  // an LDS-derived register budget): the transformed flux is staged two
-  // directions at a time, directions 0 and 1 with the faces across them (4 of the
-  // 6 faces: one flux-point task per thread), then direction 2 with the other two
-  // faces over the same rows, which keeps a block's LDS to 18 KB (8 blocks per CU
-  // instead of 7, at 128 VGPRs)
+  // SPLIT (SPL: the advance kernels of hexes at P = 3 with constant viscosity
+  // and no LES, FP64 and mixed, which run under an LDS-derived register budget):
+  // the transformed flux goes over u and is staged two directions at a time,
+  // directions 0 and 1 with the faces across them (4 of the 6 faces: one
+  // flux-point task per thread), then direction 2 with the other two faces over
+  // the same rows.  A block's LDS is 18 KB in FP64 (8 blocks per CU instead of 7
+  // at 128 VGPRs) and 9 KB mixed (14 instead of 11 at 72 VGPRs)
   static constexpr bool SPLIT = SPLIT_OK;
   static constexpr int NS = SPLIT ? 2 : ND;  // direction slots of the flux rows
   static constexpr int FS = EPB * T::NF * NS * T::NU;
@@ -206,8 +207,8 @@ struct SmemLayout {
   static constexpr int S_AT = mx(U + Q, G);
   static constexpr int TOTAL = FLUX ? (ALIAS_F ? Q + mx(U, FS) : Q + U + F)
                                     : (MODE == MODE_TRACE_GRAD ? S_AT + SG : U);
-  // waves per SIMD that the LDS allows (rounded up, at most 8; SPLIT: 4, the
-  // register budget of its face passes)
+  // waves per SIMD that the LDS allows (rounded up, at most 8; SPLIT: at most 4
+  // in FP64 and 7 mixed, the budgets under which its face passes do not spill)
   static constexpr int LDS_BLOCKS = 163840 / (TOTAL * VB);
   static constexpr int WAVES_LDS = mx(1, (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4 > 8 ? 8
                                              : (LDS_BLOCKS * (T::BLOCK / 64) + 3) / 4);
@@ -234,19 +235,20 @@ __device__ __forceinline__ void interp_line(const R* rows, const R* lw, int base
 // reference CPU build computes them; 3 learned eddy viscosity with the WALE model
 // of the reference GPU build as fallback (hexes).
 // Register budget (waves per SIMD) of a kernel variant.  The advance kernels of
-// affine hexes up to P = 4 (constant viscosity, no LES; mixed: P = 3, 4) are
-// compiled for the number of waves that their LDS allows, so that registers do
-// not bind first (FP64 P = 3: 128 VGPRs and 7 blocks per CU instead of 151 and 6);
-// every other variant would spill under that budget and keeps the compiler's
-// choice.
+// affine hexes up to P = 4 (constant viscosity, no LES; mixed: P = 3, 4), and of
+// hexes with per-point metrics at P = 3, are compiled for the number of waves
+// that their LDS allows, so that registers do not bind first (FP64 P = 3 affine:
+// 128 VGPRs and 8 blocks per CU instead of 151 and 6; per-point: 128 VGPRs with
+// an 8-byte spill); every other variant would spill under that budget and keeps
+// the compiler's choice.
 template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE>
 constexpr bool split_flux() {
-  return MODE == MODE_ADVANCE && AFF && LES == 0 && VIS != 2 && RIE == 0;
+  return MODE == MODE_ADVANCE && LES == 0 && VIS != 2 && RIE == 0;
 }
 template <int P, int ND, int EPB, int VIS, int MODE, bool AFF, bool MIX, int LES, int RIE>
 constexpr int kernel_waves() {
   using S = SmemLayout<P, ND, EPB, MODE, LES, MIX ? 4 : 8, split_flux<P, ND, EPB, VIS, MODE, AFF, MIX, LES, RIE>()>;
-  return (MODE == MODE_ADVANCE && ND == 3 && P <= 4 && (!MIX || P >= 3) && AFF && LES == 0 && VIS != 2 && RIE == 0)
+  return (MODE == MODE_ADVANCE && ND == 3 && P <= 4 && (!MIX || P >= 3) && (AFF || (P == 3 && EPB == 2)) && LES == 0 && VIS != 2 && RIE == 0)
              ? S::WAVES
              : 1;
 }
