@@ -155,9 +155,9 @@ TRAFFIC_SOURCE = {
 # keyed by (workload, per-element metrics, precision).  Sources: the summaries
 # under profiles/ named for each kernel and workload.
 MEASURED_KIB = {
-    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299191.6, 122881.9), "rk": (413586.9, 297642.8)},
-    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (153833.8, 122881.0), "rk": (270304.3, 297643.0)},
-    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (105993.7, 61440.4), "rk": (227098.6, 236203.9)},
+    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299257.9, 122882.4), "rk": (441090.4, 305835.5)},
+    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (153839.0, 122880.8), "rk": (288844.9, 297643.8)},
+    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (106170.1, 61440.1), "rk": (245485.0, 236204.0)},
     ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (112806.8, 32404.0), "rk": (165582.1, 66960.6)},
     ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (64898.1, 32400.7), "rk": (131679.9, 66960.3)},
     ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45664.8, 16200.1), "rk": (93143.1, 50760.0)},
