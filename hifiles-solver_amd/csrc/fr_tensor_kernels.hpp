@@ -464,9 +464,10 @@ fr_tensor_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   // EARLY: the LDG delta pass takes the own state trace from the trace array, not
   // from u in LDS, so it needs no barrier after the state load: the state load
   // and the neighbour gathers are in flight together, and one barrier covers both
-  // (1-4 % faster).  Not in the FP64 advance kernels up to P = 3, which would
-  // spill under their register budget.
-  constexpr bool EARLY = VISC && MODE != MODE_TRACE_U && LES != 3 && !(MODE == MODE_ADVANCE && !MIX && P <= 3);
+  // (1-4 % faster).  Not in the FP64 advance kernels up to P = 3 with the three
+  // flux directions staged at once, which would spill under their register
+  // budget (with two-direction staging: an 8-byte spill, about 1 % faster).
+  constexpr bool EARLY = VISC && MODE != MODE_TRACE_U && LES != 3 && (S::SPLIT || !(MODE == MODE_ADVANCE && !MIX && P <= 3));
   if constexpr (!EARLY) __syncthreads();
 
   if constexpr (MODE == MODE_TRACE_U) {
