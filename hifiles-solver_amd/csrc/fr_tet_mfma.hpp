@@ -738,16 +738,37 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
   for (int t = 0; t < MT; t++)
 #pragma unroll
     for (int k = 0; k < NF; k++) fs[t][k] = R(0);
-  auto own_prim = [&](int task, ViscPrim<ND, R>& q) {
-    const int tl = task / NFP, fp = task - tl * NFP;
-    const long fe = fp + NFP * (long)(blockIdx.x * EPB + tl);
-    R uo[NF], uu[NF];
+  auto valid = [&](int task) { return task < EPB * NFP && blockIdx.x * EPB + task / NFP < N; };
+  // each task's own state trace: in FP64 read once for the four passes that use
+  // it (the mixed kernel would lose a wave per SIMD to the registers and reads
+  // it again each time)
+  constexpr bool KEEP_UO = !MIX;
+  [[maybe_unused]] R uo_t[KEEP_UO ? MT : 1][NF];
+  if constexpr (KEEP_UO) {
 #pragma unroll
-    for (int k = 0; k < NF; k++) uo[k] = a.uf_in[fe * NF + k];
+    for (int t = 0; t < MT; t++) {
+      const int task = tid + TetP<P>::BLOCK * t;
+      const int tl = task / NFP, fp = task - tl * NFP;
+      const long fe = fp + NFP * (long)(blockIdx.x * EPB + tl);
+#pragma unroll
+      for (int k = 0; k < NF; k++) uo_t[t][k] = valid(task) ? a.uf_in[fe * NF + k] : R(1);
+    }
+  }
+  auto own_prim = [&](int t, ViscPrim<ND, R>& q) {
+    R uo[NF], uu[NF];
+    if constexpr (KEEP_UO) {
+#pragma unroll
+      for (int k = 0; k < NF; k++) uo[k] = uo_t[t][k];
+    } else {
+      const int task = tid + TetP<P>::BLOCK * t;
+      const int tl = task / NFP, fp = task - tl * NFP;
+      const long fe = fp + NFP * (long)(blockIdx.x * EPB + tl);
+#pragma unroll
+      for (int k = 0; k < NF; k++) uo[k] = a.uf_in[fe * NF + k];
+    }
     full(uo, uu);
     visc_prim<ND, R>(uu, q);
   };
-  auto valid = [&](int task) { return task < EPB * NFP && blockIdx.x * EPB + task / NFP < N; };
 #pragma unroll
   for (int m = 0; m < ND; m++) {
     tet_product<R, P, TET_OP_I, M::MT_I, NT, M::KS_I, M::KS_I, 0>(
@@ -767,7 +788,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
         for (int k = 0; k < NF; k++) a.gf_out[fe + NFP * NN * (k + NF * m)] = g[k];
       }
       ViscPrim<ND, R> q;
-      own_prim(task, q);
+      own_prim(t, q);
       visc_column<ND, R>(q, g, dv[t], m, de[t][m]);
     }
     __syncthreads();
@@ -842,7 +863,7 @@ fr_tet_mfma_grad_kernel(const StageArgsT<std::conditional_t<MIX, float, double>>
 #pragma unroll
     for (int l = 0; l < ND; l++) nrm[l] = R(a.fgeo[l * NN * GEO_PER_ELE + gi]);
     ViscPrim<ND, R> q;
-    own_prim(task, q);
+    own_prim(t, q);
     ViscT<ND, R> vt;
     visc_finish<ND, SUTH, R>(q, dv[t], de[t], a.ph, vt);
     R fa[NF];
