@@ -288,13 +288,21 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   // ---- zero the padding rows of u and of the flux, then u ----
   zero_rows<R, LD, TetP<P>::BLOCK>(smem, NU, KU, tid);
   zero_rows<R, LD, TetP<P>::BLOCK>(smem, 3 * NU, RA, tid);
+  // FP64: the point's state stays in registers for the flux pass and the RK
+  // update (the LDS that holds it now takes the gradient); the mixed kernel would
+  // lose a wave per SIMD to the registers and reads it again
+  constexpr bool KEEP_U = !MIX;
+  [[maybe_unused]] double ukeep[G][NF];
 #pragma unroll
   for (int g = 0; g < G; g++) {
     const int el = g * E1 + el0, e = blockIdx.x * EPB + el;
+#pragma unroll
+    for (int k = 0; k < NF; k++) ukeep[g][k] = 1.0;
     if (lane_on && e < N) {
 #pragma unroll
       for (int k = 0; k < NF; k++) {
         const double v = a.u[p + NU * (e + NN * k)];
+        ukeep[g][k] = v;
         s_u[p * LD + el * NF + k] = MIX ? R(v - ref_of(k)) : R(v);
       }
     }
@@ -359,7 +367,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
 #pragma unroll
     for (int k = 0; k < NF; k++) {
       if constexpr (VISC) {
-        const double v = a.u[pe + NU * NN * k];
+        const double v = KEEP_U ? ukeep[g][k] : a.u[pe + NU * NN * k];
         u_own[k] = MIX ? R(v - ref_of(k)) : R(v);
       } else {
         u_own[k] = s_u[p * LD + el * NF + k];
@@ -527,7 +535,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
           for (int k = 0; k < NF; k++) resv[g][k] = a.res[pe0 + NU * NN * k];
         }
 #pragma unroll
-        for (int k = 0; k < NF; k++) uoldv[g][k] = a.u[pe0 + NU * NN * k];
+        for (int k = 0; k < NF; k++) uoldv[g][k] = KEEP_U ? ukeep[g][k] : a.u[pe0 + NU * NN * k];
       }
     }
   }
