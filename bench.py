@@ -158,8 +158,8 @@ MEASURED_KIB = {
     ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299257.9, 122882.4), "rk": (441090.4, 305835.5)},
     ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (153839.0, 122880.8), "rk": (288844.9, 297643.8)},
     ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (106170.1, 61440.1), "rk": (245485.0, 236204.0)},
-    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (112806.8, 32404.0), "rk": (165582.1, 66960.6)},
-    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (64898.1, 32400.7), "rk": (131679.9, 66960.3)},
+    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (92535.8, 32400.5), "rk": (145583.9, 66960.3)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (54446.2, 32400.1), "rk": (112275.4, 66960.1)},
     ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45664.8, 16200.1), "rk": (93143.1, 50760.0)},
 }
 
