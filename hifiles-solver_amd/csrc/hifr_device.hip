@@ -1016,10 +1016,15 @@ hifr_status hifr_run_timed(hifr_ctx* c, int n_steps, int adv_type, double dt, do
     c->ev.push_back(e);
   }
   c->ev_kind.assign(c->ev.size() / 2 + 1, -1);
-  hipEvent_t t0, t1;
-  HIP_TRY(hipEventCreate(&t0));
-  HIP_TRY(hipEventCreate(&t1));
-  HIP_TRY(hipEventRecord(t0, c->stream));
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipError_t e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  if (e == hipSuccess) e = hipEventRecord(t0, c->stream);
+  if (e != hipSuccess) {
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    HIP_TRY(e);
+  }
   // events bracket the two launches of each step's first stage: around every
   // launch they would lengthen the loop by about 9 % (an event pair between
   // dependent kernels), and every stage runs the same kernels on the same arrays
@@ -1030,11 +1035,14 @@ hifr_status hifr_run_timed(hifr_ctx* c, int n_steps, int adv_type, double dt, do
       s = advance_impl(c, st, adv_type, dt, it == n_steps - 1 && st == nst - 1);
     }
   c->timing = false;
-  HIP_TRY(hipEventRecord(t1, c->stream));
-  HIP_TRY(hipEventSynchronize(t1));
-  if (s != HIFR_OK) return s;
+  e = hipEventRecord(t1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(t1);
   float ms = 0.f;
-  HIP_TRY(hipEventElapsedTime(&ms, t0, t1));
+  if (e == hipSuccess && s == HIFR_OK) e = hipEventElapsedTime(&ms, t0, t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  if (s != HIFR_OK) return s;
+  HIP_TRY(e);
   if (total_ms) *total_ms = ms;
   double km[3] = {0, 0, 0};
   int kl[3] = {0, 0, 0};
@@ -1051,8 +1059,6 @@ hifr_status hifr_run_timed(hifr_ctx* c, int n_steps, int adv_type, double dt, do
     if (kernel_ms) kernel_ms[k] = km[k];
     if (kernel_launches) kernel_launches[k] = kl[k];
   }
-  (void)hipEventDestroy(t0);
-  (void)hipEventDestroy(t1);
   return HIFR_OK;
 }
 
