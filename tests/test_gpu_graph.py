@@ -14,6 +14,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = [("hex_p3_ns_perturbed_3x3x3", {}), ("hex_p3_les_wale_perturbed_3x3x3", {}),
          ("tet_p3_channel_wale_walls_2x2x2", {}), ("quad_p3_ns_perturbed_4x3", {}),
+         ("quad_p3_ns_roe_channel_walls_4x3", {}), ("hex_p3_ns_tgv_2x2x2", {}),
          ("hex_p3_ns_perturbed_3x3x3", {"hifr_precision": 1}), ("tet_p3_ns_perturbed_2x2x2", {"hifr_precision": 1})]
 
 
