@@ -146,8 +146,9 @@ def algorithmic_bytes(nu, nfp, nfaces, nd=3, per_element=False, mixed=False, vis
 
 # Where the traffic constants below come from: they are not measured by this run.
 TRAFFIC_SOURCE = {
-    "tgv": "constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in profiles/fr_tensor_kernel_tgv32_p3.md",
-    "tet": "constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in profiles/fr_tet_mfma_kernel_tet_channel_p3.md",
+    "tgv32": "constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in profiles/fr_tensor_kernel_tgv32_p3.md",
+    "tgv48-p4": "constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in profiles/fr_tensor_kernel_tgv48_p4.md",
+    "tet-channel": "constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in profiles/fr_tet_mfma_kernel_tet_channel_p3.md",
 }
 
 # HBM traffic per element of each stage kernel from rocprofv3 --pmc FETCH_SIZE (x2.0)
@@ -158,6 +159,9 @@ MEASURED_KIB = {
     ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299257.9, 122882.4), "rk": (441090.4, 305835.5)},
     ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (153839.0, 122880.8), "rk": (288844.9, 297643.8)},
     ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (106170.1, 61440.1), "rk": (245485.0, 236204.0)},
+    ("tgv48-p4", False, "fp64"): {"n_eles": 110592, "grad": (2011828.6, 653347.8), "rk": (2915802.9, 1847838.9)},
+    ("tgv48-p4", True, "fp64"): {"n_eles": 110592, "grad": (1052445.8, 651318.4), "rk": (1916025.3, 1848559.3)},
+    ("tgv48-p4", True, "mixed"): {"n_eles": 110592, "grad": (739545.1, 328894.9), "rk": (1708124.0, 1515354.9)},
     ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (92535.8, 32400.5), "rk": (145583.9, 66960.3)},
     ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (54446.2, 32400.1), "rk": (112275.4, 66960.1)},
     ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45664.8, 16200.1), "rk": (93143.1, 50760.0)},
@@ -353,7 +357,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(r["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(r["achieved"] / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic["rk"] * s.n_eles) if traffic else None,
-                     "traffic_source": TRAFFIC_SOURCE[w["kind"]] if traffic else None,
+                     "traffic_source": TRAFFIC_SOURCE[args.workload] if traffic else None,
                      "kernel": "%s ADVANCE (flux, divergence, RK update, next traces)" % kname,
                      "bytes_per_launch": r["rk_b"] * s.n_eles, "bytes_per_element": r["rk_b"],
                      "avg_launch_ms": round(r["avg_rk_ms"], 5),
