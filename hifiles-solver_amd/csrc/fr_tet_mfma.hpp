@@ -309,16 +309,23 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   }
   __syncthreads();
 
+  // FP64 viscous kernels keep each flux-point task's neighbour index from the
+  // delta pass for the face pass
+  constexpr int MTK = (EPB * NFP + TetP<P>::BLOCK - 1) / TetP<P>::BLOCK;
+  constexpr bool KEEP_NB = VISC && !MIX;
+  [[maybe_unused]] int nbk[KEEP_NB ? MTK : 1];
   if constexpr (VISC) {
     // ---- the LDG delta = u_c - uo at each flux point
     //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346).
     //      The own trace uo = I u is the one the previous pass wrote (the same
     //      product of the same operator tiles and state, so the same values) ----
-#pragma unroll 1
-    for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
+#pragma unroll KEEP_NB ? MTK : 1
+    for (int t = 0; t < MTK; t++) {
+      const int task = tid + TetP<P>::BLOCK * t;
       const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
-      if (te >= N) continue;
+      if (task >= EPB * NFP || te >= N) continue;
       int nb = a.nbr[fp + NFP * te];
+      if constexpr (KEEP_NB) nbk[t] = nb;
       const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
       nb &= 0x7fffffff;
       R* q = s_d + fp * LD + tl * NF;
@@ -444,12 +451,15 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   //      separates them.  The normal transformed discontinuous flux that the
   //      reference subtracts here (extrapolate_totalFlux, daxpy) is folded into
   //      the divergence operator: D Ft + L (jump - N Ft) = (D - L N) Ft + L jump ----
-#pragma unroll 1
-  for (int task = tid; task < EPB * NFP; task += TetP<P>::BLOCK) {
+#pragma unroll KEEP_NB ? MTK : 1
+  for (int t = 0; t < MTK; t++) {
+    const int task = tid + TetP<P>::BLOCK * t;
     const int tl = task / NFP, fp = task - tl * NFP, te = blockIdx.x * EPB + tl;
-    if (te >= N) continue;
+    if (task >= EPB * NFP || te >= N) continue;
     const long fe = fp + NFP * (long)te;
-    int nb = a.nbr[fe];
+    int nb;
+    if constexpr (KEEP_NB) nb = nbk[t];
+    else nb = a.nbr[fe];
     const R pen = R(nb < 0 ? -a.ph.pen_fact : a.ph.pen_fact);
     nb &= 0x7fffffff;
     const long gi = geo_idx(fp, te);
