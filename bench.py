@@ -162,9 +162,9 @@ MEASURED_KIB = {
     ("tgv48-p4", False, "fp64"): {"n_eles": 110592, "grad": (2011828.6, 653347.8), "rk": (2915802.9, 1847838.9)},
     ("tgv48-p4", True, "fp64"): {"n_eles": 110592, "grad": (1052445.8, 651318.4), "rk": (1916025.3, 1848559.3)},
     ("tgv48-p4", True, "mixed"): {"n_eles": 110592, "grad": (739545.1, 328894.9), "rk": (1708124.0, 1515354.9)},
-    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (92535.8, 32400.5), "rk": (145583.9, 66960.3)},
-    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (54446.2, 32400.1), "rk": (112275.4, 66960.1)},
-    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45664.8, 16200.1), "rk": (93143.1, 50760.0)},
+    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (92401.5, 32400.4), "rk": (142952.3, 66960.8)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (54508.8, 32400.1), "rk": (109972.3, 66960.4)},
+    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45627.4, 16200.2), "rk": (92899.9, 50760.1)},
 }
 
 
