@@ -314,6 +314,9 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
   constexpr int MTK = (EPB * NFP + TetP<P>::BLOCK - 1) / TetP<P>::BLOCK;
   constexpr bool KEEP_NB = VISC && !MIX;
   [[maybe_unused]] int nbk[KEEP_NB ? MTK : 1];
+  // ... and, from P = 2, the first task's own trace (the second's would spill)
+  constexpr bool KEEP_UO0 = KEEP_NB && P >= 2;
+  [[maybe_unused]] R uo0[KEEP_UO0 ? NF : 1];
   if constexpr (VISC) {
     // ---- the LDG delta = u_c - uo at each flux point
     //      (int_inters::calculate_common_invFlux viscous part, src/int_inters.cpp:325-346).
@@ -332,6 +335,12 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
       R uo[NF];
 #pragma unroll
       for (int k = 0; k < NF; k++) uo[k] = a.uf_in[(fp + NFP * (long)te) * NF + k];
+      if constexpr (KEEP_UO0) {
+        if (t == 0) {
+#pragma unroll
+          for (int k = 0; k < NF; k++) uo0[k] = uo[k];
+        }
+      }
       if (nb & BDY_FLAG) {
         // wall: Dirichlet common solution u_c = (u_l + u_r)/2 (src/inters.cpp:664-668)
         R ur[NF];
@@ -470,7 +479,7 @@ fr_tet_mfma_kernel(const StageArgsT<std::conditional_t<MIX, float, double>> a) {
     // own state trace, as the previous pass wrote it
     R uo[NF], un[NF], fn[NF];
 #pragma unroll
-    for (int k = 0; k < NF; k++) uo[k] = a.uf_in[fe * NF + k];
+    for (int k = 0; k < NF; k++) uo[k] = (KEEP_UO0 && t == 0) ? uo0[k] : a.uf_in[fe * NF + k];
     const bool wall = (nb & BDY_FLAG) != 0;
     if (wall) {
       ghost(uo, un);
