@@ -162,9 +162,9 @@ MEASURED_KIB = {
     ("tgv48-p4", False, "fp64"): {"n_eles": 110592, "grad": (2011828.6, 653347.8), "rk": (2915802.9, 1847838.9)},
     ("tgv48-p4", True, "fp64"): {"n_eles": 110592, "grad": (1052445.8, 651318.4), "rk": (1916025.3, 1848559.3)},
     ("tgv48-p4", True, "mixed"): {"n_eles": 110592, "grad": (739545.1, 328894.9), "rk": (1708124.0, 1515354.9)},
-    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (92401.5, 32400.4), "rk": (142952.3, 66960.8)},
-    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (54508.8, 32400.1), "rk": (109972.3, 66960.4)},
-    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45627.4, 16200.2), "rk": (92899.9, 50760.1)},
+    ("tet-channel", False, "fp64"): {"n_eles": 20736, "grad": (92341.8, 32400.6), "rk": (137424.9, 66960.5)},
+    ("tet-channel", True, "fp64"): {"n_eles": 20736, "grad": (54497.2, 32400.1), "rk": (105189.8, 66960.6)},
+    ("tet-channel", True, "mixed"): {"n_eles": 20736, "grad": (45626.8, 16200.1), "rk": (92839.0, 50760.1)},
 }
 
 
