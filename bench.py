@@ -156,9 +156,9 @@ TRAFFIC_SOURCE = {
 # keyed by (workload, per-element metrics, precision).  Sources: the summaries
 # under profiles/ named for each kernel and workload.
 MEASURED_KIB = {
-    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299257.9, 122882.4), "rk": (441090.4, 305835.5)},
-    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (153839.0, 122880.8), "rk": (288844.9, 297643.8)},
-    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (106170.1, 61440.1), "rk": (245485.0, 236204.0)},
+    ("tgv32", False, "fp64"): {"n_eles": 32768, "grad": (299091.6, 122882.3), "rk": (445657.0, 297643.6)},
+    ("tgv32", True, "fp64"): {"n_eles": 32768, "grad": (153924.2, 122881.0), "rk": (303791.0, 305837.2)},
+    ("tgv32", True, "mixed"): {"n_eles": 32768, "grad": (105990.1, 61440.3), "rk": (245266.1, 236204.0)},
     ("tgv48-p4", False, "fp64"): {"n_eles": 110592, "grad": (2011828.6, 653347.8), "rk": (2915802.9, 1847838.9)},
     ("tgv48-p4", True, "fp64"): {"n_eles": 110592, "grad": (1052445.8, 651318.4), "rk": (1916025.3, 1848559.3)},
     ("tgv48-p4", True, "mixed"): {"n_eles": 110592, "grad": (739545.1, 328894.9), "rk": (1708124.0, 1515354.9)},
